@@ -1,0 +1,303 @@
+/*
+ * phylo_hip.h -- C ABI of the gfx950 (MI355X) implementation of phylocaml's two
+ * site-pattern hot paths:
+ *
+ *   - Fitch parsimony, the NonAdditive_c character type
+ *       (reference: lib/nonAdditive_c.ml, lib/bitvector/bv.c:46-55,147-161)
+ *   - Felsenstein pruning, the Likelihood_c character type over MlModel
+ *       (reference: lib/likelihood_c.ml:1-33, lib/mlModel.ml, lib/mlmodel.c)
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns an
+ * int status (PHY_OK == 0).  On failure the message is available from
+ * phy_last_error(ctx); the OCaml stubs turn a non-zero status into
+ * caml_failwith(phy_last_error(ctx)), which is how the reference's C stubs report
+ * errors (lib/mlmodel.c:110-120,191-193,249-257).
+ *
+ * Device pointers ("d_" prefix) must point to device memory of the context's
+ * device.  Work is enqueued on the context's stream and is asynchronous unless a
+ * function says otherwise; results written to device pointers are valid after
+ * phy_ctx_sync() or any later stream-ordered operation.
+ *
+ * Data layouts in HBM
+ *   Packed state sets ("nibble blocks"): one 4-bit IUPAC set per site
+ *     (A=1, C=2, G=4, T=8; lib/alphabet.ml:301-307), 16 sites per uint64 word,
+ *     LSB first: site s lives in word s/16, bits 4*(s%16) .. 4*(s%16)+3.
+ *     A block of n_sites sites occupies phy_words(n_sites) words; padding
+ *     nibbles of the last word must be zero on input and are written as zero.
+ *   Tree node arrays: node-major, (2N-1) rows of `stride` words each; rows
+ *     0..N-1 are the leaves (taxa), rows N..2N-2 the internal nodes.
+ *   Partial likelihoods: node-major, then rate-major, then state-major, then
+ *     pattern (structure of arrays): L[v][r][i][p], p contiguous.
+ */
+#ifndef PHYLO_HIP_H
+#define PHYLO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------- */
+enum {
+  PHY_OK = 0,
+  PHY_EINVAL = 1,   /* bad argument, malformed topology, size mismatch        */
+  PHY_EDEVICE = 2,  /* HIP runtime error                                      */
+  PHY_ENOMEM = 3,   /* allocation failure                                     */
+  PHY_ENUMERIC = 4, /* NaN in a rate matrix, non-reversible model             */
+  PHY_ERANGE = 5    /* a size beyond what a kernel supports                   */
+};
+
+/* Likelihood evaluation forms (bit flags). */
+enum {
+  PHY_F64 = 0,      /* FP64 partials, per-pattern int32 exponents            */
+  PHY_F32 = 1,      /* FP32 partials, per-pattern int32 exponents            */
+  PHY_FUSED = 2     /* keep interior partials on chip (lnL only, no stored
+                       partials, no derivatives)                              */
+};
+
+/* Gamma rate-category modes. */
+enum {
+  PHY_GAMMA_REF = 0,  /* rate_i = quantile(i/n) of Gamma(shape a, scale b):
+                         exactly MlModel.gamma_rates (lib/mlModel.ml:93-99)  */
+  PHY_GAMMA_MEAN = 1  /* Yang (1994) mean of each equal-probability bin of
+                         Gamma(shape a, rate a): mean rate 1
+                         (the documented intent, lib/mlModel.mli:12)          */
+};
+
+typedef struct phy_ctx phy_ctx;
+typedef struct phy_topo phy_topo;
+typedef struct phy_model phy_model;
+
+/* Number of uint64 words of a nibble block of n_sites sites. */
+int64_t phy_words(int64_t n_sites);
+
+/* ------------------------------------------------------------------------- */
+/* Context: device, stream, error string.                                    */
+/* Replaces nothing in the reference (the reference is single-threaded OCaml */
+/* with no device); the OCaml stubs keep one context per process.            */
+/* ------------------------------------------------------------------------- */
+int phy_ctx_create(int device, void* hip_stream, phy_ctx** out);
+void phy_ctx_destroy(phy_ctx* ctx);
+const char* phy_last_error(const phy_ctx* ctx);
+int phy_ctx_sync(phy_ctx* ctx);
+int phy_ctx_device(const phy_ctx* ctx);
+void* phy_ctx_stream(const phy_ctx* ctx);
+
+/* Device memory for callers without their own allocator (the OCaml custom
+ * blocks; lib/bitvector/bv.c:163-189 mallocs/frees host vectors the same way). */
+int phy_dev_alloc(phy_ctx* ctx, size_t bytes, void** d_out);
+int phy_dev_free(phy_ctx* ctx, void* d_ptr);
+int phy_copy_h2d(phy_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+int phy_copy_d2h(phy_ctx* ctx, void* h_dst, const void* d_src, size_t bytes); /* synchronous */
+
+/* ------------------------------------------------------------------------- */
+/* Rooted binary topology.                                                   */
+/* Replaces the traversal order of Tree.post_order_edges / pre_order_nodes   */
+/* (lib/tree.ml:123-187) driving Node.Make1D.median_2 (lib/node.ml:183-198): */
+/* the order is compiled once into device-side programs.                     */
+/*   n_leaves  N >= 2; nodes 0..N-1 are leaves, N..2N-2 internal.            */
+/*   children  2*(N-1) ints: children of internal node N+k are               */
+/*             children[2k], children[2k+1].                                 */
+/*   root      an internal node id that is nobody's child.                   */
+/* Returns PHY_EINVAL for a malformed tree (a node with two parents, an      */
+/* unreachable node, a cycle, an id out of range).                           */
+/* ------------------------------------------------------------------------- */
+int phy_topo_create(phy_ctx* ctx, int32_t n_leaves, const int32_t* children,
+                    int32_t root, phy_topo** out);
+void phy_topo_destroy(phy_topo* topo);
+/* Host-side facts about the compiled programs: LDS stack depths of the
+ * post-order and pre-order walks, and the parent of every node (-1 = root). */
+int phy_topo_info(const phy_topo* topo, int32_t* down_stack_depth,
+                  int32_t* up_stack_depth, int32_t* parent_out /* 2N-1 or NULL */);
+
+/* ------------------------------------------------------------------------- */
+/* Fitch parsimony: NodeData operations of NonAdditive_c on nibble blocks.   */
+/* Weights: d_weights[site] (uint32), or NULL for unit weights.  Costs are   */
+/* exact unsigned integers (the reference accumulates float weights,         */
+/* lib/nonAdditive_c.ml:28-31, exact while the sum stays below 2^53).        */
+/* Cost outputs are single uint64 device values that the call overwrites.    */
+/* ------------------------------------------------------------------------- */
+
+/* median_2 x y (lib/nonAdditive_c.ml:19-35; bv_fitch lib/bitvector/bv.c:147-161):
+ * out_s = x_s & y_s if non-empty, else x_s | y_s; cost = sum_s w_s [x_s & y_s = 0]. */
+int phy_fitch_median2(phy_ctx* ctx, const uint64_t* d_x, const uint64_t* d_y,
+                      uint64_t* d_out, int64_t n_sites, const uint32_t* d_weights,
+                      uint64_t* d_cost);
+
+/* median_3 a b c (lib/nonAdditive_c.ml signature slot, TODO in the reference):
+ * the 3-way Fitch median: per site the states contained in the largest number
+ * of {a,b,c}; cost = sum_s w_s (3 - that number). */
+int phy_fitch_median3(phy_ctx* ctx, const uint64_t* d_a, const uint64_t* d_b,
+                      const uint64_t* d_c, uint64_t* d_out, int64_t n_sites,
+                      const uint32_t* d_weights, uint64_t* d_cost);
+
+/* distance_1 a b (bv_distance, lib/bitvector/bv.c:46-55, weighted):
+ * sum_s w_s [a_s & b_s = 0]. */
+int phy_fitch_distance1(phy_ctx* ctx, const uint64_t* d_a, const uint64_t* d_b,
+                        int64_t n_sites, const uint32_t* d_weights, uint64_t* d_cost);
+
+/* distance_2 a b c (TODO in the reference, lib/nonAdditive_c.ml:45): cost of
+ * inserting c on the edge (a,b) = distance_1 (median_2 a b) c. */
+int phy_fitch_distance2(phy_ctx* ctx, const uint64_t* d_a, const uint64_t* d_b,
+                        const uint64_t* d_c, int64_t n_sites,
+                        const uint32_t* d_weights, uint64_t* d_cost);
+
+/* adjust_3 (TODO in the reference, lib/nonAdditive_c.ml:39; called from
+ * Node.Make1D.readjust_3, lib/node.ml:239-256): Fitch (1971) final set of an
+ * internal node v from its preliminary set P_v, its children's preliminary
+ * sets P_a, P_b and its parent's final set F_p:
+ *   F_p subset of P_v          -> F_v = F_p
+ *   P_a & P_b non-empty        -> F_v = P_v | (F_p & (P_a | P_b))
+ *   otherwise                  -> F_v = P_v | F_p
+ * d_changed (nullable) receives the number of sites where F_v != P_v. */
+int phy_fitch_adjust3(phy_ctx* ctx, const uint64_t* d_pv, const uint64_t* d_pa,
+                      const uint64_t* d_pb, const uint64_t* d_fp, uint64_t* d_out,
+                      int64_t n_sites, uint64_t* d_changed);
+
+/* Whole-tree down-pass: Node.Make1D.median_2 (lib/node.ml:183-198) applied in
+ * post-order to every internal node, in one launch.
+ *   d_sets      (2N-1) rows x stride words; rows 0..N-1 (leaves) are read,
+ *               rows N..2N-2 receive the preliminary sets.
+ *   d_site_cost n_sites uint32: number of union events at each site.
+ *   d_node_cost (nullable) N-1 uint64: weighted local cost of each internal
+ *               node N+k (NonAdditive_c.t.cost of that node).
+ *   d_total     1 uint64: sum_s w_s site_cost_s. */
+int phy_fitch_downpass(phy_ctx* ctx, const phy_topo* topo, uint64_t* d_sets,
+                       int64_t stride, int64_t n_sites, const uint32_t* d_weights,
+                       uint32_t* d_site_cost, uint64_t* d_node_cost,
+                       uint64_t* d_total);
+
+/* Whole-tree up-pass (final states; Node.Make1D.final_states / uppass_*,
+ * TODO at lib/node.ml:258-270): pre-order, adjust_3's rule at internal nodes,
+ * F_root = P_root, and at a leaf L with parent final F_p:
+ * F_leaf = F_p & L where non-empty, else L.
+ *   d_sets   as written by phy_fitch_downpass;  d_final (2N-1) rows x stride. */
+int phy_fitch_uppass(phy_ctx* ctx, const phy_topo* topo, const uint64_t* d_sets,
+                     uint64_t* d_final, int64_t stride, int64_t n_sites);
+
+/* ------------------------------------------------------------------------- */
+/* MlModel native stubs (host, FP64, synchronous; no LAPACK).                */
+/* ------------------------------------------------------------------------- */
+
+/* Replaces likelihood_CAML_diagonalize_gtr / diagonalize_gtr
+ * (lib/mlmodel.c:208-271; called from MlModel.diagonalize, lib/mlModel.ml:554-583).
+ * Q (n x n, row-major) must be a reversible rate matrix.  Writes U, D (n x n,
+ * eigenvalues on the diagonal, like the reference's diag(D,n)) and Ui with
+ * Q = U diag(D) Ui, computed by cyclic Jacobi on the symmetric
+ * Pi^1/2 Q Pi^-1/2.  A non-reversible Q gives PHY_ENUMERIC (the reference's
+ * "Imaginary eigenvalues" failure, lib/mlmodel.c:248-250); NaN gives PHY_ENUMERIC
+ * (the assert of lib/mlModel.ml:557-568). */
+int phy_diagonalize_gtr(int32_t n, const double* Q, double* U, double* D, double* Ui);
+
+/* Replaces likelihood_CAML_diagonalize_sym (lib/mlmodel.c:163-205): A (n x n,
+ * symmetric, row-major) is overwritten by its eigenvectors as rows (so that
+ * A = U^T diag(D) U), D (n x n) receives the eigenvalues on its diagonal. */
+int phy_diagonalize_sym(int32_t n, double* A, double* D);
+
+/* Replaces likelihood_CAML_compose_gtr (lib/mlmodel.c:325-365):
+ * P = exp(Q t) row-major (P[i][j] = Pr(j | i, t)); t == -1 -> Q; t < 1e-10 -> I. */
+int phy_compose_gtr(int32_t n, const double* U, const double* D, const double* Ui,
+                    double t, double* P);
+
+/* Replaces likelihood_CAML_compose_sym (lib/mlmodel.c:280-323), including its
+ * single-precision branch length (the `const float t` parameter at :280). */
+int phy_compose_sym(int32_t n, const double* U, const double* D, float t, double* P);
+
+/* Replaces MlModel.gamma_rates (lib/mlModel.ml:93-99) for mode PHY_GAMMA_REF
+ * (shape a, scale b), and gives Yang's mean-1 categories for PHY_GAMMA_MEAN
+ * (shape a; b ignored).  Writes n rates. */
+int phy_gamma_rates(int32_t mode, double a, double b, int32_t n, double* rates);
+
+/* Replaces MlModel.m_gtr_independent + m_meanrate (lib/mlModel.ml:390-421,
+ * 204-213): Q from priors pi (n) and the (n+1)(n-2)/2 free exchangeabilities
+ * co (upper triangle, row-major, the last (n-1,n) rate fixed at 1.0);
+ * diagonal = -row sum; Q /= sum_i -Q_ii pi_i. */
+int phy_gtr_q(int32_t n, const double* pi, const double* co, double* Q);
+
+/* ------------------------------------------------------------------------- */
+/* Likelihood model on the device: MlModel.t (lib/mlModel.ml:53-63) reduced  */
+/* to what pruning needs: pi, the eigen-decomposition of Q, Gamma rates and  */
+/* their probabilities.  n_states K in {4, 20}; n_rates R in 1..8.           */
+/* ------------------------------------------------------------------------- */
+int phy_model_create(phy_ctx* ctx, int32_t n_states, int32_t n_rates,
+                     const double* pi, const double* Q, const double* rates,
+                     const double* probs, phy_model** out);
+void phy_model_destroy(phy_model* model);
+
+/* P matrices of a set of branches on the device, FP64: for branch b and rate
+ * r, P[b][r] = compose(rate_r * t_b) (the reference's caller scales t by the
+ * rate: MlModel.compose takes one scalar, lib/mlModel.ml:585-587);
+ * P1 = dP/dt and P2 = d2P/dt2 (nullable) = rate_r^k U diag(lambda^k
+ * e^(lambda rate_r t_b)) Ui.  Layout [b][r][i][j]. */
+int phy_model_pmatrices(phy_ctx* ctx, const phy_model* model, const double* d_t,
+                        int32_t n_branches, double* d_P, double* d_P1, double* d_P2);
+
+/* ------------------------------------------------------------------------- */
+/* Felsenstein pruning: Likelihood_c (lib/likelihood_c.ml:1-33, all TODO in  */
+/* the reference) for DNA (K = 4) with tips as nibble blocks.                */
+/* ------------------------------------------------------------------------- */
+
+/* Device workspace (bytes) that phy_lk_tree needs for this topology, model,
+ * pattern count and form (PHY_F64 | PHY_F32, optionally | PHY_FUSED). */
+int phy_lk_tree_work_size(const phy_topo* topo, const phy_model* model,
+                          int64_t n_patterns, int32_t form, int32_t want_derivs,
+                          size_t* bytes);
+
+/* Whole-tree evaluation (Node.Make1D.median_2 in post-order + root_cost,
+ * lib/node.ml:183-198,272), one launch for the partials:
+ *   d_tips     N rows x stride words: tip state sets (nibble blocks; a gap is
+ *              all four bits = missing data).
+ *   d_weights  n_patterns uint32 pattern weights.
+ *   d_blen     2N-1 branch lengths (FP64); d_blen[v] is the branch above v,
+ *              d_blen[root] is ignored.
+ *   d_lnl      1 double: sum_p w_p (ln L_p), L_p = sum_r prob_r sum_i pi_i
+ *              L_root[p][r][i] with the pattern's power-of-two exponents.
+ *   d_site_lnl (nullable) n_patterns doubles: ln L_p.
+ *   d_d1,d_d2  (nullable, not with PHY_FUSED) 2N-1 doubles: dlnL/dt_v and
+ *              d2lnL/dt_v^2 for the branch above every non-root node v.  The
+ *              two root branches form one unrooted edge: both receive the
+ *              derivative for that edge.
+ *   d_work     phy_lk_tree_work_size bytes.
+ * The internal partials of node N+k (unless PHY_FUSED) are left in d_work and
+ * can be read with phy_lk_tree_partials. */
+int phy_lk_tree(phy_ctx* ctx, const phy_topo* topo, const phy_model* model,
+                int32_t form, int64_t n_patterns, const uint64_t* d_tips,
+                int64_t stride, const uint32_t* d_weights, const double* d_blen,
+                double* d_lnl, double* d_site_lnl, double* d_d1, double* d_d2,
+                void* d_work, size_t work_bytes);
+
+/* Pointers into d_work of the stored partials and exponents of internal node
+ * v (N <= v <= 2N-2) after phy_lk_tree without PHY_FUSED:
+ * partial L[r][i][p] (FP32 or FP64 by form, row pitch n_patterns_padded) and
+ * exponents e[p] (int32). */
+int phy_lk_tree_partials(const phy_topo* topo, const phy_model* model,
+                         int64_t n_patterns, int32_t form, int32_t want_derivs,
+                         void* d_work, int32_t v, void** d_partial,
+                         int32_t** d_exp, int64_t* pitch);
+
+/* Per-node NodeData operations of Likelihood_c on stored partials
+ * (L[r][i][p] with row pitch `pitch` >= n_patterns, exponents e[p]):
+ *   leaf   (median_1 of a leaf): L_leaf from its tip nibble block; e = 0.
+ *   median2 (median_2, lib/likelihood_c.ml:13):
+ *     L_out[r] = (P(rate_r t_x) L_x[r]) (*) (P(rate_r t_y) L_y[r]),
+ *     e_out = e_x + e_y + rescaling exponent.
+ *   root   (root_cost, lib/likelihood_c.ml:18): d_lnl = sum_p w_p ln L_p. */
+int phy_lk_leaf(phy_ctx* ctx, const phy_model* model, int32_t form,
+                int64_t n_patterns, const uint64_t* d_tip, void* d_out,
+                int32_t* d_eout, int64_t pitch);
+int phy_lk_median2(phy_ctx* ctx, const phy_model* model, int32_t form,
+                   int64_t n_patterns, const void* d_x, const int32_t* d_ex,
+                   double t_x, const void* d_y, const int32_t* d_ey, double t_y,
+                   void* d_out, int32_t* d_eout, int64_t pitch);
+int phy_lk_root(phy_ctx* ctx, const phy_model* model, int32_t form,
+                int64_t n_patterns, const void* d_root, const int32_t* d_eroot,
+                int64_t pitch, const uint32_t* d_weights, double* d_lnl);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHYLO_HIP_H */

@@ -1,0 +1,71 @@
+// Shared definitions of the gfx950 library: context, topology programs,
+// error reporting.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/phylo_hip.h"
+
+struct phy_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+};
+
+// Operand sources of a post-order (down) program entry.
+enum : int { SRC_LEAF = 0, SRC_ACC = 1, SRC_STACK = 2 };
+// Flag bits of a down entry (int4.w): A source in bits 0-1, B source in bits
+// 2-3, bit 4 = push the accumulator to the LDS stack before the entry.
+enum : int { DOWN_SPILL = 16 };
+// Flag bits of an up entry (int4.w): input source in bits 0-1 (SRC_LEAF means
+// "root: read P from the fifo"), bit 2 = child a is a leaf, bit 3 = child b
+// is a leaf, bit 4 = child b (not a) continues in the accumulator, bit 5 =
+// push the other internal child.
+enum : int { UP_A_LEAF = 4, UP_B_LEAF = 8, UP_CONT_B = 16, UP_PUSH = 32 };
+
+struct phy_topo {
+  phy_ctx* ctx = nullptr;
+  int32_t n_leaves = 0;
+  int32_t root = -1;
+  std::vector<int32_t> children;   // 2(N-1)
+  std::vector<int32_t> parent;     // 2N-1
+  std::vector<int32_t> subtree;    // leaves below each node
+  // post-order program: {v, A, B, flags}; A/B are child node ids
+  std::vector<int32_t> down_prog;  // 4 per entry, N-1 entries
+  std::vector<int32_t> down_fifo;  // leaf ids in consumption order (N)
+  int32_t down_depth = 0;          // LDS stack entries needed
+  // pre-order program: {v, a, b, flags}
+  std::vector<int32_t> up_prog;
+  std::vector<int32_t> up_fifo;    // node ids in load order (2N-1)
+  int32_t up_depth = 0;
+  // device copies
+  int4* d_down_prog = nullptr;
+  int32_t* d_down_fifo = nullptr;
+  int4* d_up_prog = nullptr;
+  int32_t* d_up_fifo = nullptr;
+};
+
+inline int set_err(phy_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+#define PHY_HIP(ctx, call)                                                           \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return set_err((ctx), PHY_EDEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+#define PHY_LAUNCHED(ctx)                                                            \
+  do {                                                                               \
+    hipError_t e_ = hipGetLastError();                                               \
+    if (e_ != hipSuccess)                                                            \
+      return set_err((ctx), PHY_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// Zero n uint64 values on the stream with a kernel (graph-capture safe).
+int phy_zero_u64(phy_ctx* ctx, uint64_t* d, int64_t n);

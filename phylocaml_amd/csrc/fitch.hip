@@ -1,0 +1,492 @@
+// Fitch parsimony on packed 4-bit state sets (16 sites per uint64 word).
+//
+// Per-site rule (NonAdditive_c.median_2, lib/nonAdditive_c.ml:19-35, and
+// bv_fitch, lib/bitvector/bv.c:147-161): m = x & y; if m is empty the median
+// is x | y at cost 1, else m at cost 0.  Sixteen sites are handled at once
+// with SWAR arithmetic on each 32-bit half of a word:
+//   nonempty flag in bit 3 of each nibble: ((t & 0x7..7) + 0x7..7 | t) & 0x8..8
+//   (the masked add cannot carry across nibbles), a full-nibble mask from it
+//   by  m8 | (m8 - (m8 >> 3)),  and the median by one bit-field insert.
+//
+// Whole-tree passes: one thread owns one word (16 sites) of every node and
+// walks the uniform program compiled by phy_topo_create.  Intermediate sets
+// live in an accumulator register and a per-thread LDS stack, so the
+// algorithmic HBM traffic is the minimum: leaves read once and every
+// preliminary set written once (down), every set read once and every final
+// set written once (up).  The operand stream (leaves for the down-pass, node
+// sets for the up-pass) is consumed in a fixed order and is prefetched BATCH
+// loads ahead through registers into an LDS ring, which keeps enough bytes in
+// flight to cover HBM latency with one wave per SIMD.
+#include "common.hpp"
+
+namespace {
+
+constexpr uint32_t K1 = 0x11111111u, K7 = 0x77777777u, K8 = 0x88888888u;
+
+__device__ __forceinline__ uint32_t nz8(uint32_t t) { return (((t & K7) + K7) | t) & K8; }
+// nibble mask: 0xF where the nibble of t is non-empty
+__device__ __forceinline__ uint32_t nzmask(uint32_t t) {
+  const uint32_t m = nz8(t);
+  return m | (m - (m >> 3));
+}
+__device__ __forceinline__ uint32_t bfi(uint32_t mask, uint32_t a, uint32_t b) {
+  return (mask & a) | (~mask & b);
+}
+
+// Fitch median of one 32-bit half; returns the empty-intersection flags in
+// bit 0 of each nibble (the per-site costs).
+__device__ __forceinline__ uint32_t fitch32(uint32_t a, uint32_t b, uint32_t* out) {
+  const uint32_t t = a & b, u = a | b;
+  const uint32_t e8 = nz8(t) ^ K8;
+  const uint32_t e0 = e8 >> 3;
+  const uint32_t mask = e8 | (e8 - e0);
+  *out = bfi(mask, u, t);
+  return e0;
+}
+
+struct W2 {
+  uint32_t lo, hi;
+};
+__device__ __forceinline__ W2 split(uint64_t x) { return {(uint32_t)x, (uint32_t)(x >> 32)}; }
+__device__ __forceinline__ uint64_t join(uint32_t lo, uint32_t hi) {
+  return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+__device__ __forceinline__ uint64_t fitch64(uint64_t a, uint64_t b, uint32_t* elo, uint32_t* ehi) {
+  const W2 x = split(a), y = split(b);
+  uint32_t ol, oh;
+  *elo = fitch32(x.lo, y.lo, &ol);
+  *ehi = fitch32(x.hi, y.hi, &oh);
+  return join(ol, oh);
+}
+
+// Fitch (1971) final set of an internal node, per 32-bit half.
+__device__ __forceinline__ uint32_t final32(uint32_t pv, uint32_t pa, uint32_t pb, uint32_t fp) {
+  const uint32_t sub_ok = ~nzmask(fp & ~pv);       // F_p subset of P_v
+  const uint32_t inter = nzmask(pa & pb);          // P_v was an intersection
+  const uint32_t r = bfi(inter, pv | (fp & (pa | pb)), pv | fp);
+  return bfi(sub_ok, fp, r);
+}
+__device__ __forceinline__ uint32_t leaf32(uint32_t fp, uint32_t l) {
+  const uint32_t i = fp & l;
+  return bfi(nzmask(i), i, l);
+}
+
+__device__ __forceinline__ uint64_t final64(uint64_t pv, uint64_t pa, uint64_t pb, uint64_t fp) {
+  return join(final32((uint32_t)pv, (uint32_t)pa, (uint32_t)pb, (uint32_t)fp),
+              final32((uint32_t)(pv >> 32), (uint32_t)(pa >> 32), (uint32_t)(pb >> 32),
+                      (uint32_t)(fp >> 32)));
+}
+__device__ __forceinline__ uint64_t leaf64(uint64_t fp, uint64_t l) {
+  return join(leaf32((uint32_t)fp, (uint32_t)l), leaf32((uint32_t)(fp >> 32), (uint32_t)(l >> 32)));
+}
+
+// weighted count of the flagged sites of word w (flags in bit 0 of nibbles)
+__device__ __forceinline__ uint64_t weigh(uint32_t elo, uint32_t ehi, const uint32_t* __restrict__ wt,
+                                          int64_t w) {
+  uint64_t c = 0;
+  uint64_t e = join(elo, ehi);
+  while (e) {
+    const int k = __builtin_ctzll(e) >> 2;
+    c += wt[w * 16 + k];
+    e &= e - 1;
+  }
+  return c;
+}
+
+// 0xF in every nibble of word w that holds a real site (the last word of a
+// block may end in padding nibbles, which must not count as empty sites)
+__device__ __forceinline__ uint64_t valid_nibbles(int64_t w, int64_t n_sites) {
+  const int64_t r = n_sites - w * 16;
+  return r >= 16 ? ~0ull : ((1ull << (4 * r)) - 1ull);
+}
+
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// block-wide sum of one uint64 per thread, added atomically to *dst
+__device__ __forceinline__ void block_add(uint64_t v, unsigned long long* dst) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, (unsigned long long)v);
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise NodeData operations
+// ---------------------------------------------------------------------------
+enum Op { OP_MEDIAN2 = 0, OP_DIST1 = 1, OP_DIST2 = 2, OP_MEDIAN3 = 3, OP_ADJUST3 = 4 };
+
+template <int OP>
+__global__ __launch_bounds__(256) void k_fitch_elem(const uint64_t* __restrict__ a,
+                                                    const uint64_t* __restrict__ b,
+                                                    const uint64_t* __restrict__ c,
+                                                    const uint64_t* __restrict__ d,
+                                                    uint64_t* __restrict__ out, int64_t W,
+                                                    int64_t n_sites,
+                                                    const uint32_t* __restrict__ wt,
+                                                    unsigned long long* __restrict__ cost) {
+  uint64_t acc = 0;
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < W;
+       w += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t elo = 0, ehi = 0;
+    const uint64_t vm = valid_nibbles(w, n_sites);
+    if (OP == OP_MEDIAN2 || OP == OP_DIST1) {
+      const uint64_t r = fitch64(a[w], b[w], &elo, &ehi);
+      if (OP == OP_MEDIAN2) out[w] = r;
+    } else if (OP == OP_DIST2) {
+      uint32_t t0, t1;
+      const uint64_t m = fitch64(a[w], b[w], &t0, &t1);
+      (void)fitch64(m, c[w], &elo, &ehi);
+    } else if (OP == OP_MEDIAN3) {
+      // states present in all three / at least two / at least one
+      const uint64_t x = a[w], y = b[w], z = c[w];
+      const uint64_t all3 = x & y & z;
+      const uint64_t two = (x & y) | (x & z) | (y & z);
+      const uint64_t any = x | y | z;
+      const W2 s3 = split(all3), s2 = split(two), s1 = split(any);
+      const uint32_t m3l = nzmask(s3.lo), m3h = nzmask(s3.hi);
+      const uint32_t m2l = nzmask(s2.lo), m2h = nzmask(s2.hi);
+      const uint32_t rl = bfi(m3l, s3.lo, bfi(m2l, s2.lo, s1.lo));
+      const uint32_t rh = bfi(m3h, s3.hi, bfi(m2h, s2.hi, s1.hi));
+      out[w] = join(rl, rh);
+      // cost 0 / 1 / 2: flag "not all three" and "not even two" separately
+      const uint32_t vl = (uint32_t)vm & nzmask(s1.lo), vh = (uint32_t)(vm >> 32) & nzmask(s1.hi);
+      const uint32_t n3l = (~m3l) & K1 & vl, n3h = (~m3h) & K1 & vh;
+      const uint32_t n2l = (~m2l) & K1 & vl, n2h = (~m2h) & K1 & vh;
+      if (wt) {
+        acc += weigh(n3l, n3h, wt, w) + weigh(n2l, n2h, wt, w);
+      } else {
+        acc += __popc(n3l) + __popc(n3h) + __popc(n2l) + __popc(n2h);
+      }
+      continue;
+    } else {  // OP_ADJUST3: a = P_v, b = P_a, c = P_b, d = F_p
+      const uint64_t r = final64(a[w], b[w], c[w], d[w]);
+      out[w] = r;
+      const uint64_t diff = r ^ a[w];
+      const W2 s = split(diff);
+      acc += __popc(nzmask(s.lo) & K1) + __popc(nzmask(s.hi) & K1);
+      continue;
+    }
+    elo &= (uint32_t)vm;
+    ehi &= (uint32_t)(vm >> 32);
+    if (wt) acc += weigh(elo, ehi, wt, w);
+    else acc += __popc(elo) + __popc(ehi);
+  }
+  if (cost) block_add(acc, cost);
+}
+
+// ---------------------------------------------------------------------------
+// Whole-tree down-pass
+// ---------------------------------------------------------------------------
+template <int BATCH, bool NODE_COST>
+__global__ __launch_bounds__(256) void k_fitch_down(
+    const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ fifo_ids, int n_fifo,
+    uint64_t* __restrict__ sets, int64_t stride, int64_t W, int64_t n_sites, int N,
+    const uint32_t* __restrict__ wt, uint32_t* __restrict__ site_cost,
+    unsigned long long* __restrict__ node_cost, unsigned long long* __restrict__ total) {
+  extern __shared__ uint64_t lds[];
+  const int tid = threadIdx.x, bd = blockDim.x;
+  const int64_t w = (int64_t)blockIdx.x * bd + tid;
+  const bool act = w < W;
+  const int64_t wc = act ? w : 0;  // clamped column for address arithmetic
+  uint64_t* ring = lds;            // BATCH slots
+  uint64_t* stk = lds + (size_t)BATCH * bd;
+
+  uint64_t pre[BATCH];
+  auto issue = [&](int start) {
+#pragma unroll
+    for (int j = 0; j < BATCH; ++j) {
+      const int idx = start + j;
+      pre[j] = (act && idx < n_fifo) ? sets[(int64_t)fifo_ids[idx] * stride + wc] : 0ull;
+    }
+  };
+  issue(0);
+#pragma unroll
+  for (int j = 0; j < BATCH; ++j) ring[j * bd + tid] = pre[j];
+  issue(BATCH);
+  int next_batch = 2 * BATCH;
+  int fpos = 0;
+  auto next_leaf = [&]() -> uint64_t {
+    if (fpos == BATCH) {
+#pragma unroll
+      for (int j = 0; j < BATCH; ++j) ring[j * bd + tid] = pre[j];
+      issue(next_batch);
+      next_batch += BATCH;
+      fpos = 0;
+    }
+    return ring[(fpos++) * bd + tid];
+  };
+
+  uint32_t cnt_lo = 0, cnt_hi = 0;  // nibble counters, flushed every 15 nodes
+  uint32_t site[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) site[k] = 0;
+  int since = 0;
+  uint64_t acc = 0;
+  int sp = 0;
+  const uint64_t vm = valid_nibbles(wc, n_sites);
+  for (int i = 0; i < n_ops; ++i) {
+    const int4 op = prog[i];
+    const int fl = op.w;
+    if (fl & DOWN_SPILL) {
+      stk[sp * bd + tid] = acc;
+      ++sp;
+    }
+    const int sa = fl & 3, sb = (fl >> 2) & 3;
+    uint64_t x, y;
+    if (sa == SRC_LEAF) x = next_leaf();
+    else if (sa == SRC_ACC) x = acc;
+    else { --sp; x = stk[sp * bd + tid]; }
+    if (sb == SRC_LEAF) y = next_leaf();
+    else y = acc;  // SRC_ACC (B is never taken from the stack)
+    uint32_t elo, ehi;
+    acc = fitch64(x, y, &elo, &ehi);
+    if (act) sets[(int64_t)op.x * stride + w] = acc;
+    cnt_lo += elo;
+    cnt_hi += ehi;
+    if (NODE_COST) {
+      uint64_t c = 0;
+      const uint32_t ml = elo & (uint32_t)vm, mh = ehi & (uint32_t)(vm >> 32);
+      if (act) c = wt ? weigh(ml, mh, wt, w) : (uint64_t)(__popc(ml) + __popc(mh));
+      c = wave_sum(c);
+      if ((tid & 63) == 0 && c) atomicAdd(&node_cost[op.x - N], (unsigned long long)c);
+    }
+    if (++since == 15) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        site[k] += (cnt_lo >> (4 * k)) & 15u;
+        site[8 + k] += (cnt_hi >> (4 * k)) & 15u;
+      }
+      cnt_lo = cnt_hi = 0;
+      since = 0;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    site[k] += (cnt_lo >> (4 * k)) & 15u;
+    site[8 + k] += (cnt_hi >> (4 * k)) & 15u;
+  }
+  uint64_t tot = 0;
+  if (act) {
+    const int64_t s0 = w * 16;
+    if (s0 + 16 <= n_sites) {
+      uint4* dst = reinterpret_cast<uint4*>(site_cost + s0);
+      dst[0] = make_uint4(site[0], site[1], site[2], site[3]);
+      dst[1] = make_uint4(site[4], site[5], site[6], site[7]);
+      dst[2] = make_uint4(site[8], site[9], site[10], site[11]);
+      dst[3] = make_uint4(site[12], site[13], site[14], site[15]);
+      if (wt) {
+        const uint4* ws = reinterpret_cast<const uint4*>(wt + s0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint4 x = ws[q];
+          tot += (uint64_t)site[4 * q] * x.x + (uint64_t)site[4 * q + 1] * x.y +
+                 (uint64_t)site[4 * q + 2] * x.z + (uint64_t)site[4 * q + 3] * x.w;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) tot += site[k];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k)
+        if (s0 + k < n_sites) {
+          site_cost[s0 + k] = site[k];
+          tot += (uint64_t)site[k] * (wt ? wt[s0 + k] : 1u);
+        }
+    }
+  }
+  block_add(tot, total);
+}
+
+// ---------------------------------------------------------------------------
+// Whole-tree up-pass
+// ---------------------------------------------------------------------------
+template <int BATCH>
+__global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog, int n_ops,
+                                                  const int32_t* __restrict__ fifo_ids, int n_fifo,
+                                                  const uint64_t* __restrict__ sets,
+                                                  uint64_t* __restrict__ fin, int64_t stride,
+                                                  int64_t W, int N) {
+  extern __shared__ uint64_t lds[];
+  const int tid = threadIdx.x, bd = blockDim.x;
+  const int64_t w = (int64_t)blockIdx.x * bd + tid;
+  const bool act = w < W;
+  const int64_t wc = act ? w : 0;
+  uint64_t* ring = lds;
+  uint64_t* stk = lds + (size_t)BATCH * bd;  // pairs (P, F)
+
+  uint64_t pre[BATCH];
+  auto issue = [&](int start) {
+#pragma unroll
+    for (int j = 0; j < BATCH; ++j) {
+      const int idx = start + j;
+      pre[j] = (act && idx < n_fifo) ? sets[(int64_t)fifo_ids[idx] * stride + wc] : 0ull;
+    }
+  };
+  issue(0);
+#pragma unroll
+  for (int j = 0; j < BATCH; ++j) ring[j * bd + tid] = pre[j];
+  issue(BATCH);
+  int next_batch = 2 * BATCH;
+  int fpos = 0;
+  auto next_set = [&]() -> uint64_t {
+    if (fpos == BATCH) {
+#pragma unroll
+      for (int j = 0; j < BATCH; ++j) ring[j * bd + tid] = pre[j];
+      issue(next_batch);
+      next_batch += BATCH;
+      fpos = 0;
+    }
+    return ring[(fpos++) * bd + tid];
+  };
+
+  uint64_t accP = 0, accF = 0;
+  int sp = 0;
+  for (int i = 0; i < n_ops; ++i) {
+    const int4 op = prog[i];
+    const int fl = op.w;
+    const int src = fl & 3;
+    uint64_t pv, fp = 0;
+    if (src == SRC_LEAF) pv = next_set();
+    else if (src == SRC_ACC) { pv = accP; fp = accF; }
+    else {
+      --sp;
+      pv = stk[(2 * sp) * bd + tid];
+      fp = stk[(2 * sp + 1) * bd + tid];
+    }
+    const uint64_t pa = next_set();
+    const uint64_t pb = next_set();
+    const uint64_t fv = (src == SRC_LEAF) ? pv : final64(pv, pa, pb, fp);
+    if (act) fin[(int64_t)op.x * stride + w] = fv;
+    if (fl & UP_A_LEAF) { if (act) fin[(int64_t)op.y * stride + w] = leaf64(fv, pa); }
+    if (fl & UP_B_LEAF) { if (act) fin[(int64_t)op.z * stride + w] = leaf64(fv, pb); }
+    if (fl & UP_PUSH) {
+      const uint64_t other = (fl & UP_CONT_B) ? pa : pb;
+      stk[(2 * sp) * bd + tid] = other;
+      stk[(2 * sp + 1) * bd + tid] = fv;
+      ++sp;
+    }
+    accP = (fl & UP_CONT_B) ? pb : pa;
+    accF = fv;
+  }
+}
+
+int block_size_env(const char* name, int dflt) {
+  const char* s = getenv(name);
+  if (!s) return dflt;
+  const int v = atoi(s);
+  return (v == 64 || v == 128 || v == 256) ? v : dflt;
+}
+
+constexpr int kBatch = 16;
+
+}  // namespace
+
+static int elem_launch_checks(phy_ctx* ctx, int64_t n_sites) {
+  if (!ctx) return PHY_EINVAL;
+  if (n_sites < 0) return set_err(ctx, PHY_EINVAL, "n_sites < 0");
+  return PHY_OK;
+}
+
+template <int OP>
+static int run_elem(phy_ctx* ctx, const uint64_t* a, const uint64_t* b, const uint64_t* c,
+                    const uint64_t* d, uint64_t* out, int64_t n_sites, const uint32_t* wt,
+                    uint64_t* cost) {
+  int rc = elem_launch_checks(ctx, n_sites);
+  if (rc) return rc;
+  const int64_t W = phy_words(n_sites);
+  if (cost && (rc = phy_zero_u64(ctx, cost, 1))) return rc;
+  if (W == 0) return PHY_OK;
+  const int bs = 256;
+  int64_t g = (W + bs - 1) / bs;
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(k_fitch_elem<OP>, dim3((unsigned)g), dim3(bs), 0, ctx->stream, a, b, c, d, out,
+                     W, n_sites, wt, (unsigned long long*)cost);
+  PHY_LAUNCHED(ctx);
+  return PHY_OK;
+}
+
+extern "C" {
+
+int phy_fitch_median2(phy_ctx* ctx, const uint64_t* x, const uint64_t* y, uint64_t* out,
+                      int64_t n_sites, const uint32_t* wt, uint64_t* cost) {
+  if (ctx && n_sites > 0 && (!x || !y || !out)) return set_err(ctx, PHY_EINVAL, "median2: null block");
+  return run_elem<OP_MEDIAN2>(ctx, x, y, nullptr, nullptr, out, n_sites, wt, cost);
+}
+
+int phy_fitch_median3(phy_ctx* ctx, const uint64_t* a, const uint64_t* b, const uint64_t* c,
+                      uint64_t* out, int64_t n_sites, const uint32_t* wt, uint64_t* cost) {
+  if (ctx && n_sites > 0 && (!a || !b || !c || !out)) return set_err(ctx, PHY_EINVAL, "median3: null block");
+  return run_elem<OP_MEDIAN3>(ctx, a, b, c, nullptr, out, n_sites, wt, cost);
+}
+
+int phy_fitch_distance1(phy_ctx* ctx, const uint64_t* a, const uint64_t* b, int64_t n_sites,
+                        const uint32_t* wt, uint64_t* cost) {
+  if (ctx && (!cost || (n_sites > 0 && (!a || !b)))) return set_err(ctx, PHY_EINVAL, "distance1: null argument");
+  return run_elem<OP_DIST1>(ctx, a, b, nullptr, nullptr, nullptr, n_sites, wt, cost);
+}
+
+int phy_fitch_distance2(phy_ctx* ctx, const uint64_t* a, const uint64_t* b, const uint64_t* c,
+                        int64_t n_sites, const uint32_t* wt, uint64_t* cost) {
+  if (ctx && (!cost || (n_sites > 0 && (!a || !b || !c)))) return set_err(ctx, PHY_EINVAL, "distance2: null argument");
+  return run_elem<OP_DIST2>(ctx, a, b, c, nullptr, nullptr, n_sites, wt, cost);
+}
+
+int phy_fitch_adjust3(phy_ctx* ctx, const uint64_t* pv, const uint64_t* pa, const uint64_t* pb,
+                      const uint64_t* fp, uint64_t* out, int64_t n_sites, uint64_t* changed) {
+  if (ctx && n_sites > 0 && (!pv || !pa || !pb || !fp || !out)) return set_err(ctx, PHY_EINVAL, "adjust3: null block");
+  return run_elem<OP_ADJUST3>(ctx, pv, pa, pb, fp, out, n_sites, nullptr, changed);
+}
+
+int phy_fitch_downpass(phy_ctx* ctx, const phy_topo* topo, uint64_t* sets, int64_t stride,
+                       int64_t n_sites, const uint32_t* wt, uint32_t* site_cost,
+                       uint64_t* node_cost, uint64_t* total) {
+  if (!ctx) return PHY_EINVAL;
+  if (!topo || !topo->d_down_prog) return set_err(ctx, PHY_EINVAL, "downpass: topology not on a device");
+  const int64_t W = phy_words(n_sites);
+  if (n_sites < 0 || stride < W) return set_err(ctx, PHY_EINVAL, "downpass: stride < words");
+  if (!sets || !site_cost || !total) return set_err(ctx, PHY_EINVAL, "downpass: null output");
+  const int N = topo->n_leaves;
+  int rc;
+  if ((rc = phy_zero_u64(ctx, total, 1))) return rc;
+  if (node_cost && (rc = phy_zero_u64(ctx, node_cost, N - 1))) return rc;
+  if (W == 0) return PHY_OK;
+  const int bs = block_size_env("PHY_FITCH_BLOCK", 64);
+  const int64_t g = (W + bs - 1) / bs;
+  const size_t lds = (size_t)(kBatch + topo->down_depth) * bs * sizeof(uint64_t);
+  if (lds > 160 * 1024) return set_err(ctx, PHY_ERANGE, "downpass: LDS stack too deep for this tree");
+  if (node_cost)
+    hipLaunchKernelGGL((k_fitch_down<kBatch, true>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
+                       topo->d_down_prog, N - 1, topo->d_down_fifo, N, sets, stride, W, n_sites, N,
+                       wt, site_cost, (unsigned long long*)node_cost, (unsigned long long*)total);
+  else
+    hipLaunchKernelGGL((k_fitch_down<kBatch, false>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
+                       topo->d_down_prog, N - 1, topo->d_down_fifo, N, sets, stride, W, n_sites, N,
+                       wt, site_cost, (unsigned long long*)nullptr, (unsigned long long*)total);
+  PHY_LAUNCHED(ctx);
+  return PHY_OK;
+}
+
+int phy_fitch_uppass(phy_ctx* ctx, const phy_topo* topo, const uint64_t* sets, uint64_t* fin,
+                     int64_t stride, int64_t n_sites) {
+  if (!ctx) return PHY_EINVAL;
+  if (!topo || !topo->d_up_prog) return set_err(ctx, PHY_EINVAL, "uppass: topology not on a device");
+  const int64_t W = phy_words(n_sites);
+  if (n_sites < 0 || stride < W) return set_err(ctx, PHY_EINVAL, "uppass: stride < words");
+  if (!sets || !fin) return set_err(ctx, PHY_EINVAL, "uppass: null buffer");
+  if (W == 0) return PHY_OK;
+  const int N = topo->n_leaves;
+  const int bs = block_size_env("PHY_FITCH_BLOCK", 64);
+  const int64_t g = (W + bs - 1) / bs;
+  const size_t lds = (size_t)(kBatch + 2 * topo->up_depth) * bs * sizeof(uint64_t);
+  if (lds > 160 * 1024) return set_err(ctx, PHY_ERANGE, "uppass: LDS stack too deep for this tree");
+  hipLaunchKernelGGL((k_fitch_up<kBatch>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
+                     topo->d_up_prog, N - 1, topo->d_up_fifo, 2 * N - 1, sets, fin, stride, W, N);
+  PHY_LAUNCHED(ctx);
+  return PHY_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,181 @@
+"""Likelihood_c on the GPU: Felsenstein pruning behind the NodeData.S
+signature (lib/nodeData.ml:3-35; every function is TODO in the reference,
+lib/likelihood_c.ml:1-33).
+
+A value holds a node's conditional likelihoods on the device -- partials
+L[r][i][p] (FP32 or FP64, pattern-contiguous) with per-pattern int32
+power-of-two exponents -- plus the length of the branch to its parent.
+NodeData.S has no branch-length argument, so the value carries it (``blen``).
+NodeData costs are minimised, so ``cost`` / ``root_cost`` are -lnL.
+
+``Evaluator`` runs a whole tree in one pass (lnL and optional per-branch
+first/second derivatives for Newton branch-length updates); ``newton_step``
+is the adjust_3 branch-length update built on them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, replace
+
+import numpy as np
+
+from . import device as D
+from ._lib import PHY_F32, PHY_F64, PHY_FUSED, check, lib
+from .mlmodel import Model
+
+
+class DeviceModel:
+    """phy_model: pi, eigen-decomposition, rates and probabilities on the device."""
+
+    def __init__(self, ctx: D.Context, model: Model):
+        self.ctx = ctx
+        self.model = model
+        K = model.q.shape[0]
+        R = model.rates.size
+        h = C.c_void_p()
+        arr = [np.ascontiguousarray(a, np.float64) for a in
+               (model.priors, model.q, model.rates, model.probs)]
+        self._keep = arr
+        ctx.check(lib.phy_model_create(ctx.handle, K, R, *(a.ctypes.data_as(C.POINTER(C.c_double))
+                                                           for a in arr), C.byref(h)),
+                  "model_create")
+        self.handle = h
+        self.K, self.R = K, R
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.phy_model_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class Likelihood:
+    partial: object      # device [R*K, pitch] float32/float64
+    exps: object         # device int32 [pitch]
+    blen: float          # branch to the parent
+    n_patterns: int
+    weights: object = None  # device uint32 [n_patterns] (pattern weights)
+    form: int = PHY_F64
+    lnl: float | None = None
+
+
+def _dtype(form):
+    torch = __import__("torch")
+    return torch.float32 if form & PHY_F32 else torch.float64
+
+
+def leaf(ctx, dm: DeviceModel, tip_sets: np.ndarray, blen: float, weights=None,
+         form: int = PHY_F64) -> Likelihood:
+    """median_1 of a leaf: partial = indicator of the tip's state set."""
+    torch = __import__("torch")
+    n = int(tip_sets.size)
+    pitch = ((n + 63) // 64) * 64
+    tip = D.to_device(D.pack_nibbles(np.asarray(tip_sets, np.uint8)), ctx.device)
+    part = torch.empty((dm.R * dm.K, pitch), dtype=_dtype(form), device=f"cuda:{ctx.device}")
+    ex = torch.zeros((pitch,), dtype=torch.int32, device=f"cuda:{ctx.device}")
+    ctx.check(lib.phy_lk_leaf(ctx.handle, dm.handle, form, n, D.dptr(tip), D.dptr(part),
+                              D.dptr(ex), pitch), "lk_leaf")
+    w = weights if weights is None or not isinstance(weights, np.ndarray) else \
+        D.to_device(np.asarray(weights, np.uint32), ctx.device)
+    return Likelihood(part, ex, float(blen), n, w, form)
+
+
+def median_2(ctx, dm: DeviceModel, prev, x: Likelihood, y: Likelihood,
+             blen: float = 0.0) -> Likelihood:
+    """One pruning step: L = (P(r t_x) L_x) (*) (P(r t_y) L_y) for every rate r."""
+    torch = __import__("torch")
+    pitch = x.partial.shape[1]
+    if prev is not None and prev.partial.shape == x.partial.shape:
+        out, ex = prev.partial, prev.exps
+    else:
+        out, ex = torch.empty_like(x.partial), torch.empty_like(x.exps)
+    ctx.check(lib.phy_lk_median2(ctx.handle, dm.handle, x.form, x.n_patterns, D.dptr(x.partial),
+                                 D.dptr(x.exps), x.blen, D.dptr(y.partial), D.dptr(y.exps),
+                                 y.blen, D.dptr(out), D.dptr(ex), pitch), "lk_median2")
+    return replace(x, partial=out, exps=ex, blen=float(blen), lnl=None)
+
+
+def root_lnl(ctx, dm: DeviceModel, x: Likelihood) -> float:
+    torch = __import__("torch")
+    out = torch.zeros((1,), dtype=torch.float64, device=f"cuda:{ctx.device}")
+    ctx.check(lib.phy_lk_root(ctx.handle, dm.handle, x.form, x.n_patterns, D.dptr(x.partial),
+                              D.dptr(x.exps), x.partial.shape[1], D.dptr(x.weights),
+                              D.dptr(out)), "lk_root")
+    return float(out.cpu().numpy()[0])
+
+
+def root_cost(ctx, dm: DeviceModel, x: Likelihood) -> float:
+    """-lnL of the tree rooted at x (NodeData costs are minimised)."""
+    return -root_lnl(ctx, dm, x)
+
+
+def leaf_cost(x: Likelihood) -> float:
+    return 0.0
+
+
+class Evaluator:
+    """Whole-tree pruning on one compiled topology (phy_lk_tree).
+
+    form: PHY_F64 or PHY_F32, optionally | PHY_FUSED (lnL only)."""
+
+    def __init__(self, ctx, topo: D.Topology, dm: DeviceModel, n_patterns: int, form: int,
+                 derivs: bool = False):
+        torch = __import__("torch")
+        self.ctx, self.topo, self.dm = ctx, topo, dm
+        self.n_patterns, self.form, self.derivs = int(n_patterns), int(form), bool(derivs)
+        nb = C.c_size_t()
+        check(lib.phy_lk_tree_work_size(topo.handle, dm.handle, self.n_patterns, self.form,
+                                        1 if derivs else 0, C.byref(nb)), ctx.handle, "work_size")
+        self.work_bytes = nb.value
+        dev = f"cuda:{ctx.device}"
+        self.work = torch.empty((max(self.work_bytes, 8),), dtype=torch.uint8, device=dev)
+        M = topo.n_nodes
+        self.lnl = torch.zeros((1,), dtype=torch.float64, device=dev)
+        self.site_lnl = torch.zeros((self.n_patterns,), dtype=torch.float64, device=dev)
+        self.d1 = torch.zeros((M,), dtype=torch.float64, device=dev) if derivs else None
+        self.d2 = torch.zeros((M,), dtype=torch.float64, device=dev) if derivs else None
+
+    def run(self, tips, stride: int, weights, blen, site_lnl: bool = False) -> None:
+        """Enqueue one evaluation; tips/weights/blen are device tensors."""
+        self.ctx.check(lib.phy_lk_tree(
+            self.ctx.handle, self.topo.handle, self.dm.handle, self.form, self.n_patterns,
+            D.dptr(tips), stride, D.dptr(weights), D.dptr(blen), D.dptr(self.lnl),
+            D.dptr(self.site_lnl) if site_lnl else None, D.dptr(self.d1), D.dptr(self.d2),
+            D.dptr(self.work), self.work_bytes), "lk_tree")
+
+    def partial(self, v: int):
+        """(partial [R*K, pitch], exps [pitch]) of internal node v after run()."""
+        torch = __import__("torch")
+        pp, pe, pitch = C.c_void_p(), C.c_void_p(), C.c_int64()
+        check(lib.phy_lk_tree_partials(self.topo.handle, self.dm.handle, self.n_patterns,
+                                       self.form, 1 if self.derivs else 0, D.dptr(self.work), v,
+                                       C.byref(pp), C.byref(pe), C.byref(pitch)), None, "partials")
+        esz = 4 if self.form & PHY_F32 else 8
+        base = int(self.work.data_ptr())
+        off_p, off_e = pp.value - base, pe.value - base
+        RK = self.dm.R * self.dm.K
+        nbytes = RK * pitch.value * esz
+        part = self.work[off_p: off_p + nbytes].view(_dtype(self.form)).view(RK, pitch.value)
+        ex = self.work[off_e: off_e + 4 * pitch.value].view(torch.int32)
+        return part, ex
+
+
+def newton_step(blen: np.ndarray, d1: np.ndarray, d2: np.ndarray, root: int,
+                lo: float = 1e-8, hi: float = 10.0) -> np.ndarray:
+    """One Newton-Raphson branch-length update t <- t - d1/d2 where d2 < 0
+    (a maximum), else a gradient step; clamped to [lo, hi]."""
+    t = blen.copy()
+    mask = np.arange(t.size) != root
+    step = np.where(d2 < 0, -d1 / np.where(d2 < 0, d2, -1.0), 0.1 * np.sign(d1) * t)
+    t[mask] = np.clip(t[mask] + step[mask], lo, hi)
+    return t
+
+
+__all__ = ["DeviceModel", "Likelihood", "Evaluator", "leaf", "median_2", "root_lnl",
+           "root_cost", "leaf_cost", "newton_step", "PHY_F32", "PHY_F64", "PHY_FUSED"]

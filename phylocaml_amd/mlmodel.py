@@ -1,0 +1,158 @@
+"""MlModel: substitution models for the likelihood path (lib/mlModel.ml).
+
+A host-side mirror of the parts of the reference's model layer that pruning
+consumes -- model specification, priors, Gamma rate categories, rate matrix,
+eigen-decomposition and P(t) -- with the native stubs (diagonalize_gtr,
+compose_gtr, ...) served by libphylo_hip.so instead of LAPACK.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import PHY_GAMMA_MEAN, PHY_GAMMA_REF, check, lib, ptr
+
+MINIMUM = 1e-13  # lib/mlModel.ml:9
+
+
+class Error(Exception):
+    """MlModel.Error (lib/mlModel.ml:3)."""
+
+
+@dataclass
+class Spec:
+    """MlModel.s (lib/mlModel.ml:38-44) restricted to what pruning uses.
+
+    substitution: ("JC69",) | ("F81",) | ("GTR", co)  with co the
+        (n+1)(n-2)/2 free exchangeabilities (the last one is fixed at 1).
+    site_variation: ("Constant",) | ("DiscreteGamma", n, alpha) |
+        ("DiscreteCustom", [(rate, prob), ...]).
+    base_priors: ("Equal",) | ("Empirical", pi).
+    gamma_mode: PHY_GAMMA_REF reproduces gamma_rates exactly
+        (lib/mlModel.ml:93-99: quantile(i/n) of Gamma(alpha, scale alpha));
+        PHY_GAMMA_MEAN gives Yang's mean-1 categories, the documented intent
+        (lib/mlModel.mli:12).
+    """
+
+    n_states: int
+    substitution: tuple = ("GTR", ())
+    site_variation: tuple = ("Constant",)
+    base_priors: tuple = ("Equal",)
+    gamma_mode: int = PHY_GAMMA_MEAN
+
+
+@dataclass
+class Model:
+    """MlModel.t (lib/mlModel.ml:53-63)."""
+
+    spec: Spec
+    priors: np.ndarray
+    rates: np.ndarray
+    probs: np.ndarray
+    q: np.ndarray
+    u: np.ndarray
+    d: np.ndarray
+    ui: np.ndarray
+    pinvar: float | None = None
+    extra: dict = field(default_factory=dict)
+
+
+def gamma_rates(a: float, b: float, n: int, mode: int = PHY_GAMMA_REF) -> np.ndarray:
+    """MlModel.gamma_rates (lib/mlModel.ml:93-99) in PHY_GAMMA_REF mode."""
+    out = np.zeros(n)
+    check(lib.phy_gamma_rates(int(mode), float(a), float(b), int(n), ptr(out, C.c_double)),
+          None, "gamma_rates")
+    return out
+
+
+def gtr_q(pi, co) -> np.ndarray:
+    """m_gtr_independent + m_meanrate (lib/mlModel.ml:390-421, 204-213)."""
+    pi = np.ascontiguousarray(pi, np.float64)
+    n = pi.size
+    co = np.ascontiguousarray(co, np.float64)
+    if co.size != (n + 1) * (n - 2) // 2:
+        raise Error(f"Length of GTR parameters (I) is incorrect for the alphabet. "
+                    f"They should be {(n + 1) * (n - 2) // 2}, but are {co.size}.")
+    Q = np.zeros((n, n))
+    check(lib.phy_gtr_q(n, ptr(pi, C.c_double), ptr(co, C.c_double), ptr(Q, C.c_double)),
+          None, "gtr_q")
+    return Q
+
+
+def diagonalize_gtr(q: np.ndarray):
+    """likelihood_CAML_diagonalize_gtr replacement: (U, D, Ui), Q = U D Ui."""
+    q = np.ascontiguousarray(q, np.float64)
+    n = q.shape[0]
+    U, Dm, Ui = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+    rc = lib.phy_diagonalize_gtr(n, ptr(q, C.c_double), ptr(U, C.c_double), ptr(Dm, C.c_double),
+                                 ptr(Ui, C.c_double))
+    if rc != 0:
+        raise Error("Imaginary eigenvalues or invalid rate matrix")
+    return U, Dm, Ui
+
+
+def compose(model: Model, t: float) -> np.ndarray:
+    """MlModel.compose (lib/mlModel.ml:585-587): P = exp(Q t)."""
+    n = model.q.shape[0]
+    P = np.zeros((n, n))
+    check(lib.phy_compose_gtr(n, ptr(model.u, C.c_double), ptr(model.d, C.c_double),
+                              ptr(model.ui, C.c_double), float(t), ptr(P, C.c_double)),
+          None, "compose_gtr")
+    return P
+
+
+def substitution_matrix(model: Model, t: float | None = None) -> np.ndarray:
+    """MlModel.substitution_matrix (lib/mlModel.ml:589-620): Q, or Q t."""
+    q = model.q.copy()
+    return q * t if t is not None else q
+
+
+def _priors(spec: Spec) -> np.ndarray:
+    n = spec.n_states
+    if spec.base_priors[0] == "Equal":
+        return np.full(n, 1.0 / n)
+    p = np.maximum(np.asarray(spec.base_priors[1], np.float64), MINIMUM)
+    s = p.sum()
+    if p.size != n:
+        raise Error(f"Priors (length {p.size}) don't match alphabet (length {n})")
+    return p if abs(s - 1.0) <= 2.220446049250313e-16 * 4 else p / s
+
+
+def create(spec: Spec) -> Model:
+    """MlModel.create (lib/mlModel.ml:664-746) for JC69 / F81 / GTR."""
+    n = spec.n_states
+    sv = spec.site_variation
+    if sv[0] == "Constant":
+        rates, probs = np.array([1.0]), np.array([1.0])
+    elif sv[0] == "DiscreteGamma":
+        k, alpha = int(sv[1]), float(sv[2])
+        probs = np.full(k, 1.0 / k)
+        rates = gamma_rates(alpha, alpha, k, spec.gamma_mode)
+    elif sv[0] == "DiscreteCustom":
+        rates = np.array([r for r, _ in sv[1]], np.float64)
+        probs = np.array([p for _, p in sv[1]], np.float64)
+    else:
+        raise Error(f"unsupported site variation {sv[0]}")
+    priors = _priors(spec)
+    sub = spec.substitution
+    if sub[0] == "JC69":
+        q = gtr_q(np.full(n, 1.0 / n), np.ones((n + 1) * (n - 2) // 2))
+    elif sub[0] == "F81":
+        q = gtr_q(priors, np.ones((n + 1) * (n - 2) // 2))
+    elif sub[0] == "GTR":
+        co = np.asarray(sub[1], np.float64)
+        if co.size == 0:
+            co = np.ones((n - 2) * (n + 1) // 2)
+        q = gtr_q(priors, co)
+    else:
+        raise Error(f"unsupported substitution model {sub[0]}")
+    if np.isnan(q).any():
+        raise Error("NaN in rate matrix")
+    u, d, ui = diagonalize_gtr(q)
+    return Model(spec=spec, priors=priors, rates=rates, probs=probs, q=q, u=u, d=d, ui=ui)
+
+
+__all__ = ["Spec", "Model", "Error", "create", "compose", "gamma_rates", "gtr_q",
+           "diagonalize_gtr", "substitution_matrix", "PHY_GAMMA_REF", "PHY_GAMMA_MEAN"]

@@ -1,0 +1,206 @@
+"""GPU parity of the pruning kernels against the FP64 oracle.
+
+Tolerances: FP64 forms match the oracle lnL within 1e-10 relative; the
+FP32 forms (FP32 partials with int32 power-of-two exponents) within 1e-6
+relative.  Derivatives: FP64 within 1e-8 of the derivative's scale
+(sum over patterns of |w_p g_p|); FP32 within 2e-3 of that scale."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import caterpillar, random_tree
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+F64, F32, FUSED = 0, 1, 2
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from phylocaml_amd import device as D
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return D.Context(0)
+
+
+def _model(rng, R):
+    from phylocaml_amd import mlmodel as MM
+
+    pi = rng.dirichlet(np.full(4, 5.0))
+    ex = rng.dirichlet(np.ones(6))
+    spec = MM.Spec(4, ("GTR", (ex / ex[-1])[:-1]), ("DiscreteGamma", R, 0.5),
+                   ("Empirical", pi), MM.PHY_GAMMA_MEAN)
+    if R == 1:
+        spec.site_variation = ("Constant",)
+    return MM.create(spec)
+
+
+def _tips(rng, N, P):
+    t = (1 << rng.integers(0, 4, (N, P))).astype(np.uint8)
+    amb = rng.random((N, P)) < 0.1
+    t[amb] |= (1 << rng.integers(0, 4, amb.sum())).astype(np.uint8)
+    t[rng.random((N, P)) < 0.03] = 15
+    return t
+
+
+def _run(ctx, N, ch, root, bl, model, tips, w, form, derivs):
+    from phylocaml_amd import device as D, likelihood as LK
+
+    topo = D.Topology(N, ch, root, ctx)
+    dm = LK.DeviceModel(ctx, model)
+    P = tips.shape[1]
+    ev = LK.Evaluator(ctx, topo, dm, P, form, derivs=derivs)
+    tp = D.to_device(D.pack_nibbles(tips), ctx.device)
+    ev.run(tp, D.words(P), D.to_device(w, ctx.device), D.to_device(bl, ctx.device),
+           site_lnl=True)
+    ctx.sync()
+    out = {"lnl": float(ev.lnl.cpu()[0]), "site": ev.site_lnl.cpu().numpy()}
+    if derivs:
+        out["d1"], out["d2"] = ev.d1.cpu().numpy(), ev.d2.cpu().numpy()
+    out["ev"] = ev
+    return out
+
+
+def _oracle(N, ch, root, bl, model, tips, w, derivs):
+    return O.lnl(N, ch, root, bl, model.priors, model.q, model.rates, model.probs,
+                 tips.astype(np.uint32), w, derivs=derivs, threads=min(16, os.cpu_count() or 1))
+
+
+def _deriv_scale(N, ch, root, bl, model, tips, w):
+    # scale of each derivative: sum_p |w_p g_p| approximated by the weighted
+    # number of patterns times the per-branch rate scale
+    return float(w.sum())
+
+
+@pytest.mark.parametrize("N", [2, 3, 9, 64, 300])
+@pytest.mark.parametrize("R", [1, 4, 8])
+def test_lnl_and_derivatives_match_oracle(ctx, rng, N, R):
+    for kind in ("random", "caterpillar"):
+        _say(f"prune N={N} R={R} {kind}")
+        ch, root = random_tree(rng, N) if kind == "random" else caterpillar(N)
+        P = int(rng.integers(1, 700))
+        bl = rng.exponential(0.1, 2 * N - 1) + 1e-3
+        bl[root] = 0.0
+        model = _model(rng, R)
+        tips = _tips(rng, N, P)
+        w = rng.integers(1, 6, P).astype(np.uint32)
+        ov, osl, od1, od2 = _oracle(N, ch, root, bl, model, tips, w, True)
+        scale = _deriv_scale(N, ch, root, bl, model, tips, w)
+        g = _run(ctx, N, ch, root, bl, model, tips, w, F64, True)
+        assert abs(g["lnl"] - ov) <= 1e-10 * abs(ov), (g["lnl"], ov)
+        assert np.allclose(g["site"], osl, rtol=1e-11, atol=1e-11)
+        mask = np.arange(2 * N - 1) != root
+        assert np.all(np.abs(g["d1"] - od1)[mask] <= 1e-8 * (np.abs(od1[mask]) + scale))
+        assert np.all(np.abs(g["d2"] - od2)[mask] <= 1e-8 * (np.abs(od2[mask]) + scale))
+        g32 = _run(ctx, N, ch, root, bl, model, tips, w, F32, True)
+        assert abs(g32["lnl"] - ov) <= 1e-6 * abs(ov), (g32["lnl"], ov)
+        assert np.all(np.abs(g32["d1"] - od1)[mask] <= 2e-3 * (np.abs(od1[mask]) + scale))
+        fu32 = _run(ctx, N, ch, root, bl, model, tips, w, F32 | FUSED, False)
+        assert fu32["lnl"] == g32["lnl"]  # same arithmetic, operands from LDS
+        fu64 = _run(ctx, N, ch, root, bl, model, tips, w, F64 | FUSED, False)
+        assert fu64["lnl"] == g["lnl"]
+
+
+def test_stored_partials_match_oracle_pieces(ctx, rng):
+    # the partial of a cherry is (P_a t_a) * (P_b t_b): check one node directly
+    from phylocaml_amd import device as D
+
+    N = 5
+    ch, root = caterpillar(N)
+    bl = rng.exponential(0.2, 2 * N - 1)
+    model = _model(rng, 4)
+    tips = _tips(rng, N, 200)
+    w = np.ones(200, np.uint32)
+    g = _run(ctx, N, ch, root, bl, model, tips, w, F64, False)
+    part, ex = g["ev"].partial(N)  # node N = (leaf 0, leaf 1)
+    part = part.cpu().numpy()[:, :200]
+    ex = ex.cpu().numpy()[:200]
+    for r in range(4):
+        Pa = O.compose(model.q, model.rates[r] * bl[0])
+        Pb = O.compose(model.q, model.rates[r] * bl[1])
+        ta = (tips[0][None, :] >> np.arange(4)[:, None]) & 1
+        tb = (tips[1][None, :] >> np.arange(4)[:, None]) & 1
+        exp = (Pa @ ta) * (Pb @ tb)
+        got = part[4 * r: 4 * r + 4] * np.ldexp(1.0, ex)[None, :]
+        assert np.allclose(got, exp, rtol=1e-13, atol=0)
+
+
+def test_node_api_matches_tree(ctx, rng):
+    from phylocaml_amd import likelihood as LK
+
+    N = 3
+    ch = np.array([0, 1, 3, 2], np.int32)  # node 3 = (0,1), root 4 = (3,2)
+    root = 4
+    bl = np.array([0.1, 0.2, 0.3, 0.15, 0.0])
+    model = _model(rng, 4)
+    P = 333
+    tips = _tips(rng, N, P)
+    w = rng.integers(1, 4, P).astype(np.uint32)
+    dm = LK.DeviceModel(ctx, model)
+    for form in (F64, F32):
+        l0 = LK.leaf(ctx, dm, tips[0], bl[0], w, form)
+        l1 = LK.leaf(ctx, dm, tips[1], bl[1], w, form)
+        l2 = LK.leaf(ctx, dm, tips[2], bl[2], w, form)
+        n3 = LK.median_2(ctx, dm, None, l0, l1, blen=bl[3])
+        rt = LK.median_2(ctx, dm, None, n3, l2)
+        v = LK.root_lnl(ctx, dm, rt)
+        ov, _ = _oracle(N, ch, root, bl, model, tips, w, False)
+        tol = 1e-10 if form == F64 else 1e-6
+        assert abs(v - ov) <= tol * abs(ov)
+        assert LK.root_cost(ctx, dm, rt) == -v
+
+
+def test_config2_full_size(ctx):
+    """512-taxon Yule tree, 200,000 unique GTR+G4 patterns (the pruning
+    benchmark workload): FP64 lnL and derivatives vs the oracle, FP32-scaled
+    lnL error vs FP64."""
+    from phylocaml_amd import device as D, likelihood as LK, mlmodel as MM, synth
+
+    N, P = 512, 200_000
+    _say("config-2: generating")
+    tree = synth.yule(N, 2, 0.05)
+    pi, co = synth.gtr_params(4, 5.0, 2)
+    spec = MM.Spec(4, ("GTR", co), ("DiscreteGamma", 4, 0.5), ("Empirical", pi))
+    model = MM.create(spec)
+    st = synth.simulate(tree, model.priors, model.q, model.rates, P, 2, unique=True)
+    w = synth.poisson_weights(P, 3.0, 2)
+    tips = np.where(st < 4, 1 << np.minimum(st, 3), 15).astype(np.uint8)
+    _say("config-2: GPU")
+    g64 = _run(ctx, N, tree.children, tree.root, tree.blen, model, tips, w, F64, True)
+    g32 = _run(ctx, N, tree.children, tree.root, tree.blen, model, tips, w, F32 | FUSED, False)
+    _say("config-2: oracle")
+    ov, osl, od1, od2 = _oracle(N, tree.children, tree.root, tree.blen, model, tips, w, True)
+    _say(f"lnL oracle {ov!r} fp64 {g64['lnl']!r} fp32 {g32['lnl']!r}")
+    assert abs(g64["lnl"] - ov) <= 1e-10 * abs(ov)
+    assert abs(g32["lnl"] - ov) <= 1e-6 * abs(ov)
+    mask = np.arange(2 * N - 1) != tree.root
+    scale = float(w.sum())
+    assert np.all(np.abs(g64["d1"] - od1)[mask] <= 1e-8 * (np.abs(od1[mask]) + scale))
+    assert np.all(np.abs(g64["d2"] - od2)[mask] <= 1e-8 * (np.abs(od2[mask]) + scale))
+
+
+def test_argument_errors(ctx, rng):
+    from phylocaml_amd import _lib, device as D, likelihood as LK
+
+    N = 4
+    ch, root = caterpillar(N)
+    topo = D.Topology(N, ch, root, ctx)
+    dm = LK.DeviceModel(ctx, _model(rng, 4))
+    with pytest.raises(_lib.PhyError):
+        LK.Evaluator(ctx, topo, dm, 10, F64 | FUSED, derivs=True)
+    ev = LK.Evaluator(ctx, topo, dm, 10, F64)
+    buf = torch.zeros(64, dtype=torch.uint64, device="cuda:0")
+    rc = _lib.lib.phy_lk_tree(ctx.handle, topo.handle, dm.handle, F64, 10, D.dptr(buf), 1,
+                              D.dptr(buf), D.dptr(buf), D.dptr(ev.lnl), None, None, None,
+                              D.dptr(ev.work), 16)
+    assert rc == 1  # workspace too small

@@ -1,0 +1,125 @@
+"""Pins the pruning oracle: brute-force enumeration of internal states on
+tiny trees, the 2-taxon closed form, finite differences for the
+per-branch derivatives, and invariance to the rooting (pulley principle)."""
+import numpy as np
+import pytest
+
+from conftest import caterpillar, random_tree
+from oracle import oracle as O
+
+
+def _model(rng, K=4, R=4):
+    pi = rng.dirichlet(np.full(K, 5.0))
+    ex = rng.dirichlet(np.ones(K * (K - 1) // 2))
+    Q = O.gtr_q(pi, (ex / ex[-1])[:-1])
+    rates = np.sort(rng.gamma(0.5, 2.0, R)) + 0.05
+    probs = np.full(R, 1.0 / R)
+    return pi, Q, rates, probs
+
+
+def _tips(rng, N, P, K=4, ambiguous=0.2):
+    t = (1 << rng.integers(0, K, (N, P))).astype(np.uint32)
+    amb = rng.random((N, P)) < ambiguous
+    t[amb] |= (1 << rng.integers(0, K, amb.sum())).astype(np.uint32)
+    t[rng.random((N, P)) < 0.05] = (1 << K) - 1
+    return t
+
+
+@pytest.mark.parametrize("N", [2, 3, 4, 5])
+def test_lnl_equals_brute_force(rng, N):
+    for trial in range(4):
+        ch, root = random_tree(rng, N) if trial % 2 else caterpillar(N)
+        bl = rng.exponential(0.3, 2 * N - 1)
+        pi, Q, rates, probs = _model(rng)
+        tips = _tips(rng, N, 25)
+        w = rng.integers(1, 5, 25).astype(np.uint32)
+        v, sl = O.lnl(N, ch, root, bl, pi, Q, rates, probs, tips, w)
+        b = O.lnl_brute(N, ch, root, bl, pi, Q, rates, probs, tips, w)
+        assert abs(v - b) <= 1e-12 * abs(b)
+        assert abs((w * sl).sum() - v) <= 1e-12 * abs(v)
+
+
+def test_lnl_brute_force_20_states(rng):
+    N = 3
+    ch, root = random_tree(rng, N)
+    bl = rng.exponential(0.3, 2 * N - 1)
+    pi, Q, rates, probs = _model(rng, K=20, R=2)
+    tips = _tips(rng, N, 7, K=20)
+    v, _ = O.lnl(N, ch, root, bl, pi, Q, rates, probs, tips)
+    b = O.lnl_brute(N, ch, root, bl, pi, Q, rates, probs, tips)
+    assert abs(v - b) <= 1e-12 * abs(b)
+
+
+def test_two_taxon_closed_form(rng):
+    # L = sum_i pi_i P_ij(t1 + t2) for observed states i, j (reversibility)
+    pi, Q, _, _ = _model(rng)
+    ch, root = np.array([0, 1], np.int32), 2
+    bl = np.array([0.13, 0.29, 0.0])
+    tips = np.array([[1, 2, 4, 8, 1], [1, 4, 4, 2, 8]], np.uint32)
+    v, sl = O.lnl(2, ch, root, bl, pi, Q, np.ones(1), np.ones(1), tips)
+    P = O.compose(Q, 0.42)
+    for p in range(5):
+        i, j = int(np.log2(tips[0, p])), int(np.log2(tips[1, p]))
+        assert abs(sl[p] - np.log(pi[i] * P[i, j])) < 1e-13
+
+
+def test_derivatives_match_finite_differences(rng):
+    N = 9
+    ch, root = random_tree(rng, N)
+    bl = rng.exponential(0.2, 2 * N - 1) + 0.02
+    bl[root] = 0
+    pi, Q, rates, probs = _model(rng)
+    tips = _tips(rng, N, 60)
+    w = rng.integers(1, 4, 60).astype(np.uint32)
+    v, _, d1, d2 = O.lnl(N, ch, root, bl, pi, Q, rates, probs, tips, w, derivs=True)
+    ra, rb = ch[2 * (root - N)], ch[2 * (root - N) + 1]
+    for c in range(2 * N - 1):
+        if c == root:
+            continue
+        h = 1e-4 * bl[c]
+        f = []
+        for s in (-1, 0, 1):
+            b2 = bl.copy()
+            b2[c] += s * h
+            f.append(O.lnl(N, ch, root, b2, pi, Q, rates, probs, tips, w)[0])
+        fd1 = (f[2] - f[0]) / (2 * h)
+        fd2 = (f[2] - 2 * f[1] + f[0]) / (h * h)
+        assert abs(d1[c] - fd1) <= 1e-6 * (abs(fd1) + 1), c
+        assert abs(d2[c] - fd2) <= 1e-3 * (abs(fd2) + 1), c
+    assert d1[ra] == d1[rb] and d2[ra] == d2[rb]
+
+
+def test_pulley_principle(rng):
+    # moving length between the two root branches leaves lnL unchanged
+    N = 8
+    ch, root = random_tree(rng, N)
+    bl = rng.exponential(0.2, 2 * N - 1)
+    pi, Q, rates, probs = _model(rng)
+    tips = _tips(rng, N, 40)
+    ra, rb = ch[2 * (root - N)], ch[2 * (root - N) + 1]
+    v1, _ = O.lnl(N, ch, root, bl, pi, Q, rates, probs, tips)
+    b2 = bl.copy()
+    s = b2[ra] + b2[rb]
+    b2[ra], b2[rb] = 0.3 * s, 0.7 * s
+    v2, _ = O.lnl(N, ch, root, b2, pi, Q, rates, probs, tips)
+    assert abs(v1 - v2) <= 1e-12 * abs(v1)
+
+
+def test_scaling_on_deep_tree(rng):
+    # a 700-taxon caterpillar with long branches underflows FP64 without
+    # rescaling (ln L ~ 700 ln(1/4) < -745)
+    N = 700
+    ch, root = caterpillar(N)
+    bl = np.full(2 * N - 1, 8.0)
+    bl[root] = 0
+    pi, Q, rates, probs = _model(rng, R=1)
+    rates[:] = 1.0
+    tips = _tips(rng, N, 8, ambiguous=0.0)
+    v, sl = O.lnl(N, ch, root, bl, pi, Q, rates, probs, tips)
+    assert np.all(np.isfinite(sl)) and np.all(sl < -760)  # below the FP64 range of L
+    # with saturated branches P ~ 1 pi^T, so L_p ~ prod_leaves pi(state)
+    # (a gap tip contributes a factor 1)
+    single = tips != 15
+    idx = np.where(single, np.log2(np.where(single, tips, 1)).astype(int), 0)
+    approx = np.where(single, np.log(pi[idx]), 0.0).sum(0)
+    assert np.allclose(sl, approx, rtol=1e-3)
