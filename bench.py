@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["all", "prune", "fitch"], default="all")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (scale for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -82,6 +84,14 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration:
+        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda:0").fill_(1)
+        b = torch.empty_like(a)
+        ms = time_steps(torch, lambda: b.copy_(a), 10, 2)
+        print(json.dumps({"copy_bytes": 1 << 30, "ms": ms,
+                          "read_plus_write_GBs": 2 * (1 << 30) / (ms / 1e3) / 1e9}), flush=True)
+        return
 
     from phylocaml_amd import device as D, likelihood as LK, mlmodel as MM
     from phylocaml_amd import nonadditive as NA, synth

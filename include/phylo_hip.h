@@ -113,6 +113,12 @@ void phy_topo_destroy(phy_topo* topo);
  * post-order and pre-order walks, and the parent of every node (-1 = root). */
 int phy_topo_info(const phy_topo* topo, int32_t* down_stack_depth,
                   int32_t* up_stack_depth, int32_t* parent_out /* 2N-1 or NULL */);
+/* The compiled programs themselves (for host-side checking): post-order
+ * entries {v, A, B, flags} (4 x (N-1) ints), its leaf order (N), pre-order
+ * entries {v, a, b, flags} (4 x (N-1)), its node load order (2N-1).  Each
+ * argument may be NULL. */
+int phy_topo_programs(const phy_topo* topo, int32_t* down_prog, int32_t* down_fifo,
+                      int32_t* up_prog, int32_t* up_fifo);
 
 /* ------------------------------------------------------------------------- */
 /* Fitch parsimony: NodeData operations of NonAdditive_c on nibble blocks.   */
@@ -296,6 +302,38 @@ int phy_lk_median2(phy_ctx* ctx, const phy_model* model, int32_t form,
 int phy_lk_root(phy_ctx* ctx, const phy_model* model, int32_t form,
                 int64_t n_patterns, const void* d_root, const int32_t* d_eroot,
                 int64_t pitch, const uint32_t* d_weights, double* d_lnl);
+
+
+/* ------------------------------------------------------------------------- */
+/* Batched 20-state (amino-acid) pruning: many independent trees, each with  */
+/* its own GTR-20 model and Gamma rates, in one persistent launch on BF16    */
+/* MFMA (Likelihood_c.median_2 + root_cost over whole trees, lib/node.ml:    */
+/* 183-198,272, for n_states = 20; P(t) per MlModel.compose,                  */
+/* lib/mlModel.ml:585-587).  All per-tree inputs are concatenated in tree    */
+/* order:                                                                    */
+/*   children   sum_t 2(N_t - 1) ints (phy_topo_create layout per tree)      */
+/*   blen       sum_t (2N_t - 1) branch lengths                              */
+/*   pi, Q      T x 20, T x 400;  rates, probs  T x n_rates                  */
+/*   tips       sum_t N_t x P_t bytes, row-major per tree: state index       */
+/*              0..19, >= 20 = missing (all states)                          */
+/*   weights    sum_t P_t pattern weights                                    */
+/* waves: waves per workgroup (1, 2 or 4); a workgroup owns 16*waves         */
+/* patterns of one tree.  The host inputs are copied to the device here.     */
+/* ------------------------------------------------------------------------- */
+typedef struct phy_aa_batch phy_aa_batch;
+int phy_aa_batch_create(phy_ctx* ctx, int32_t n_trees, const int32_t* n_leaves,
+                        const int64_t* n_patterns, const int32_t* children,
+                        const int32_t* roots, const double* blen, const double* pi,
+                        const double* Q, int32_t n_rates, const double* rates,
+                        const double* probs, const uint8_t* tips, const uint32_t* weights,
+                        int32_t waves, phy_aa_batch** out);
+void phy_aa_batch_destroy(phy_aa_batch* batch);
+/* pattern_nodes = sum_t P_t (N_t - 1); n_items = work items of the queue. */
+int phy_aa_batch_info(const phy_aa_batch* batch, int64_t* pattern_nodes, int64_t* n_items,
+                      int32_t* stack_depth);
+/* One evaluation: P operands from the branch lengths, the persistent pruning
+ * launch, per-tree compensated sums into d_lnl[n_trees] (device). */
+int phy_aa_batch_run(phy_ctx* ctx, phy_aa_batch* batch, double* d_lnl);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,11 @@ SIGNATURES = {
     "phy_lk_leaf": (C.c_int, [vp, vp, i32, i64, vp, vp, vp, i64]),
     "phy_lk_median2": (C.c_int, [vp, vp, i32, i64, vp, vp, f64, vp, vp, f64, vp, vp, i64]),
     "phy_lk_root": (C.c_int, [vp, vp, i32, i64, vp, vp, i64, vp, vp]),
+    "phy_aa_batch_create": (C.c_int, [vp, i32, P_i32, P_i64, P_i32, P_i32, P_f64, P_f64, P_f64,
+                                      i32, P_f64, P_f64, P_u8, P_u32, i32, PP]),
+    "phy_aa_batch_destroy": (None, [vp]),
+    "phy_aa_batch_info": (C.c_int, [vp, P_i64, P_i64, P_i32]),
+    "phy_aa_batch_run": (C.c_int, [vp, vp, vp]),
     # phylo_synth.h
     "phy_synth_yule": (C.c_int, [i32, u64, f64, P_i32, P_i32, P_f64]),
     "phy_synth_fitch_leaves": (C.c_int, [i32, i64, u64, P_u64, i64]),

@@ -67,5 +67,9 @@ inline int set_err(phy_ctx* ctx, int code, const std::string& msg) {
       return set_err((ctx), PHY_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e_)); \
   } while (0)
 
+// Compile the post-order / pre-order programs of t (children, root, n_leaves
+// set); returns 0 on success.  Host only.
+int phy_compile_topo(phy_topo* t);
+
 // Zero n uint64 values on the stream with a kernel (graph-capture safe).
 int phy_zero_u64(phy_ctx* ctx, uint64_t* d, int64_t n);

@@ -158,7 +158,7 @@ bool build_post_order(int32_t N, int32_t root, const std::vector<int32_t>& ch,
 
 }  // namespace
 
-static int compile_topo(phy_topo* t) {
+int phy_compile_topo(phy_topo* t) {
   const int32_t N = t->n_leaves, M = 2 * N - 1;
   const std::vector<int32_t>& ch = t->children;
   std::vector<int32_t> post;
@@ -305,7 +305,7 @@ int phy_topo_create(phy_ctx* ctx, int32_t N, const int32_t* children, int32_t ro
   t->root = root;
   t->children.assign(children, children + 2 * (N - 1));
   t->parent = parent;
-  if (compile_topo(t) != 0) {
+  if (phy_compile_topo(t) != 0) {
     delete t;
     return set_err(ctx, PHY_EINVAL, "topology: program compilation failed");
   }

@@ -16,7 +16,7 @@
 // derivative pass and a stacked operand is read back from there.
 //
 // Underflow: after every node the partial is rescaled by an exact power of
-// two when its maximum drops below 2^-60 (FP32) or 2^-300 (FP64); the
+// two when its maximum drops below 2^-24 (FP32) or 2^-128 (FP64); the
 // per-pattern int32 exponent carries the factor, so FP32 partials cover the
 // FP64 dynamic range without rounding from the scaling itself.
 #include <cmath>
@@ -46,13 +46,17 @@ __device__ __forceinline__ int cst_rates(int K) { return 2 * K * K + K; }
 
 template <typename real>
 struct Thr;
+// Rescale thresholds.  The derivative pass multiplies three rescaled
+// factors (M, P_a D_a, P_b D_b), so a threshold T must keep T^3 well inside
+// the format's normal range: 2^-72 (FP32, min normal 2^-126) and 2^-384
+// (FP64, min normal 2^-1022).
 template <>
 struct Thr<float> {
-  static constexpr float lo = 0x1p-60f;
+  static constexpr float lo = 0x1p-24f;
 };
 template <>
 struct Thr<double> {
-  static constexpr double lo = 0x1p-300;
+  static constexpr double lo = 0x1p-128;
 };
 
 // P, P' and P'' of `nb` branches x R rates: one thread per (branch, rate, i, j).

@@ -1,0 +1,75 @@
+"""GPU checks of the batched 20-state BF16-MFMA pruning against the FP64
+oracle.
+
+BF16 operands carry an 8-bit significand, so the per-tree lnL is compared
+with a relative tolerance of 2e-3 (the measured error is reported by
+bench.py); every tree in a batch of uneven trees must meet it, which also
+checks the persistent work queue and the per-tree bookkeeping."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from phylocaml_amd import device as D
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return D.Context(0)
+
+
+def _oracle(t):
+    from phylocaml_amd import aa
+
+    v, _ = O.lnl(t.n_leaves, t.children, t.root, t.blen, t.pi, t.q, t.rates, t.probs,
+                 aa.tips_bitmask(t.tips), t.weights, threads=min(16, os.cpu_count() or 1))
+    return v
+
+
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_batch_matches_oracle(ctx, waves):
+    from phylocaml_amd import aa
+
+    print(f"aa batch waves={waves}", flush=True)
+    rng = np.random.default_rng(7)
+    trees = []
+    for k in range(9):
+        N = int(rng.integers(2, 70))
+        P = int(rng.integers(1, 900))
+        trees.append(aa.synth_tree(1000 + k, N, P, n_sim=300))
+    # ambiguous/missing tips
+    trees[3].tips[:, ::7] = 255
+    b = aa.AABatch(ctx, trees, waves=waves)
+    b.run()
+    ctx.sync()
+    got = b.lnl.cpu().numpy()
+    for k, t in enumerate(trees):
+        ov = _oracle(t)
+        rel = abs(got[k] - ov) / abs(ov)
+        print(f"tree {k}: N={t.n_leaves} P={t.n_patterns} oracle {ov:.6f} gpu {got[k]:.6f} "
+              f"rel {rel:.2e}", flush=True)
+        assert rel <= 2e-3, (k, got[k], ov)
+    # a second run on the same batch gives the same answer (queue reset)
+    b.run()
+    ctx.sync()
+    assert np.array_equal(b.lnl.cpu().numpy(), got)
+
+
+def test_tiny_exact_cases(ctx):
+    # constant tips and a two-taxon tree: L = sum_i pi_i P_ij(t1 + t2)
+    from phylocaml_amd import aa
+
+    t = aa.synth_tree(5, 2, 40, n_sim=40)
+    b = aa.AABatch(ctx, [t], waves=1)
+    b.run()
+    ctx.sync()
+    ov = _oracle(t)
+    assert abs(b.lnl.cpu().numpy()[0] - ov) <= 2e-3 * abs(ov)
