@@ -164,6 +164,14 @@ int phy_fitch_adjust3(phy_ctx* ctx, const uint64_t* d_pv, const uint64_t* d_pa,
                       const uint64_t* d_pb, const uint64_t* d_fp, uint64_t* d_out,
                       int64_t n_sites, uint64_t* d_changed);
 
+/* Batched insertion costs (distance_2 over many queries, e.g. the SPR
+ * neighbourhood): for query q = (c, a, b) (d_q: 3 ints per query, row ids),
+ * d_costs[q] = sum_s w_s [P_c,s & med(F_a,s, F_b,s) = 0], med the Fitch
+ * median, P rows from d_pre and F rows from d_fin (both rows x stride). */
+int phy_fitch_insertion_costs(phy_ctx* ctx, const uint64_t* d_pre, const uint64_t* d_fin,
+                              int64_t stride, int64_t n_sites, const uint32_t* d_weights,
+                              int64_t n_queries, const int32_t* d_q, uint64_t* d_costs);
+
 /* Whole-tree down-pass: Node.Make1D.median_2 (lib/node.ml:183-198) applied in
  * post-order to every internal node, in one launch.
  *   d_sets      (2N-1) rows x stride words; rows 0..N-1 (leaves) are read,
@@ -277,8 +285,9 @@ int phy_lk_tree(phy_ctx* ctx, const phy_topo* topo, const phy_model* model,
                 void* d_work, size_t work_bytes);
 
 /* Pointers into d_work of the stored partials and exponents of internal node
- * v (N <= v <= 2N-2) after phy_lk_tree without PHY_FUSED:
- * partial L[r][i][p] (FP32 or FP64 by form, row pitch n_patterns_padded) and
+ * v (N <= v <= 2N-2) after phy_lk_tree without PHY_FUSED: the partial is
+ * K rows of pitch*R values (FP32 or FP64 by form), row i holding
+ * L[r][i][p] at column p*R + r (pitch = *pitch, the padded pattern count);
  * exponents e[p] (int32). */
 int phy_lk_tree_partials(const phy_topo* topo, const phy_model* model,
                          int64_t n_patterns, int32_t form, int32_t want_derivs,
@@ -303,6 +312,61 @@ int phy_lk_root(phy_ctx* ctx, const phy_model* model, int32_t form,
                 int64_t n_patterns, const void* d_root, const int32_t* d_eroot,
                 int64_t pitch, const uint32_t* d_weights, double* d_lnl);
 
+
+/* ------------------------------------------------------------------------- */
+/* SPR neighbourhood (TreeNeighborhoods.generate_spr, lib/treeNeighborhoods. */
+/* ml:27-75, which never compiled; the operation of Tree.break/Tree.join,    */
+/* lib/tree.ml:299-494, on the rooted layout of phy_topo_create).  Host.     */
+/* A move (s, v): prune non-root node s (its parent p is removed, its        */
+/* sibling x takes p's place with t_x += t_p, or becomes the root), then     */
+/* regraft on the edge above v (p reused: children (v, s), t_p = t_v/2,      */
+/* t_v = t_v/2).  v must lie outside the subtree of s and differ from p, x   */
+/* and the pruned tree's root.                                               */
+/* ------------------------------------------------------------------------- */
+/* Count (and, if moves != NULL, write up to capacity (s, v) pairs of) all
+ * moves of the tree. */
+int phy_spr_moves(int32_t n_leaves, const int32_t* children, int32_t root,
+                  int64_t* n_moves, int32_t* moves, int64_t capacity);
+/* The tree after move (s, v): children_out 2(N-1), root_out, blen_out 2N-1. */
+int phy_spr_apply(int32_t n_leaves, const int32_t* children, int32_t root,
+                  const double* blen, int32_t s, int32_t v, int32_t* children_out,
+                  int32_t* root_out, double* blen_out);
+/* Exact lnL of every SPR neighbour, many candidate trees per fused launch
+ * (chunk candidates at a time; form PHY_F32 or PHY_F64, fused implied):
+ * d_lnl[i] = lnL of the tree after moves[i] (branch lengths as above, not
+ * re-optimised).  children/root/blen/moves are host arrays of the base tree;
+ * tips/weights are device arrays as for phy_lk_tree. */
+int phy_lk_spr_work_size(int32_t n_leaves, const phy_model* model, int64_t n_patterns,
+                         int32_t form, int32_t chunk, size_t* bytes);
+int phy_lk_spr_scores(phy_ctx* ctx, const phy_model* model, int32_t form,
+                      int64_t n_patterns, const uint64_t* d_tips, int64_t stride,
+                      const uint32_t* d_weights, int32_t n_leaves, const int32_t* children,
+                      int32_t root, const double* blen, int64_t n_moves,
+                      const int32_t* moves, int32_t chunk, double* d_lnl, void* d_work,
+                      size_t work_bytes);
+
+/* ------------------------------------------------------------------------- */
+/* Learned SPR proposal: a small MLP (n_in -> H -> H -> 1, GELU) trained and */
+/* run by hand-written kernels (BF16-input / FP32-accumulate MFMA GEMMs,     */
+/* FP32 master weights, Adam).  No counterpart in the reference; it ranks    */
+/* TreeNeighborhoods SPR moves (lib/treeNeighborhoods.ml) before exact       */
+/* re-scoring.  Parameters (flat FP32): W1[n_in][H] b1[H] W2[H][H] b2[H]     */
+/* w3[H] b3.  Loss 0.5 mean((y - t)^2).  Device arrays: x [B][n_in], t, y [B]. */
+/* ------------------------------------------------------------------------- */
+typedef struct phy_mlp phy_mlp;
+int phy_mlp_create(phy_ctx* ctx, int32_t n_in, int32_t n_hidden, uint64_t seed, phy_mlp** out);
+void phy_mlp_destroy(phy_mlp* mlp);
+int64_t phy_mlp_n_params(const phy_mlp* mlp);
+int phy_mlp_get_params(phy_ctx* ctx, const phy_mlp* mlp, float* h_out);      /* synchronous */
+int phy_mlp_set_params(phy_ctx* ctx, phy_mlp* mlp, const float* h_in);       /* synchronous */
+int phy_mlp_forward(phy_ctx* ctx, phy_mlp* mlp, const float* d_x, int64_t batch, float* d_y);
+/* gradients of the loss into d_grads (n_params floats, nullable) and the
+ * loss into d_loss (1 double, nullable); parameters unchanged */
+int phy_mlp_backward(phy_ctx* ctx, phy_mlp* mlp, const float* d_x, const float* d_t,
+                     int64_t batch, float* d_grads, double* d_loss);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on one batch */
+int phy_mlp_step(phy_ctx* ctx, phy_mlp* mlp, const float* d_x, const float* d_t, int64_t batch,
+                 float lr, double* d_loss);
 
 /* ------------------------------------------------------------------------- */
 /* Batched 20-state (amino-acid) pruning: many independent trees, each with  */

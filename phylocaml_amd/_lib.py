@@ -56,6 +56,7 @@ SIGNATURES = {
     "phy_fitch_adjust3": (C.c_int, [vp, vp, vp, vp, vp, vp, i64, vp]),
     "phy_fitch_downpass": (C.c_int, [vp, vp, vp, i64, i64, vp, vp, vp, vp]),
     "phy_fitch_uppass": (C.c_int, [vp, vp, vp, vp, i64, i64]),
+    "phy_fitch_insertion_costs": (C.c_int, [vp, vp, vp, i64, i64, vp, i64, vp, vp]),
     "phy_diagonalize_gtr": (C.c_int, [i32, P_f64, P_f64, P_f64, P_f64]),
     "phy_diagonalize_sym": (C.c_int, [i32, P_f64, P_f64]),
     "phy_compose_gtr": (C.c_int, [i32, P_f64, P_f64, P_f64, f64, P_f64]),
@@ -71,6 +72,19 @@ SIGNATURES = {
     "phy_lk_leaf": (C.c_int, [vp, vp, i32, i64, vp, vp, vp, i64]),
     "phy_lk_median2": (C.c_int, [vp, vp, i32, i64, vp, vp, f64, vp, vp, f64, vp, vp, i64]),
     "phy_lk_root": (C.c_int, [vp, vp, i32, i64, vp, vp, i64, vp, vp]),
+    "phy_spr_moves": (C.c_int, [i32, P_i32, i32, P_i64, P_i32, i64]),
+    "phy_spr_apply": (C.c_int, [i32, P_i32, i32, P_f64, i32, i32, P_i32, P_i32, P_f64]),
+    "phy_lk_spr_work_size": (C.c_int, [i32, vp, i64, i32, i32, C.POINTER(sz)]),
+    "phy_lk_spr_scores": (C.c_int, [vp, vp, i32, i64, vp, i64, vp, i32, P_i32, i32, P_f64, i64,
+                                    P_i32, i32, vp, vp, sz]),
+    "phy_mlp_create": (C.c_int, [vp, i32, i32, u64, PP]),
+    "phy_mlp_destroy": (None, [vp]),
+    "phy_mlp_n_params": (i64, [vp]),
+    "phy_mlp_get_params": (C.c_int, [vp, vp, C.POINTER(C.c_float)]),
+    "phy_mlp_set_params": (C.c_int, [vp, vp, C.POINTER(C.c_float)]),
+    "phy_mlp_forward": (C.c_int, [vp, vp, vp, i64, vp]),
+    "phy_mlp_backward": (C.c_int, [vp, vp, vp, vp, i64, vp, vp]),
+    "phy_mlp_step": (C.c_int, [vp, vp, vp, vp, i64, f32, vp]),
     "phy_aa_batch_create": (C.c_int, [vp, i32, P_i32, P_i64, P_i32, P_i32, P_f64, P_f64, P_f64,
                                       i32, P_f64, P_f64, P_u8, P_u32, i32, PP]),
     "phy_aa_batch_destroy": (None, [vp]),

@@ -15,7 +15,7 @@
 // preliminary set written once (down), every set read once and every final
 // set written once (up).  The operand stream (leaves for the down-pass, node
 // sets for the up-pass) is consumed in a fixed order and is prefetched BATCH
-// loads ahead through registers into an LDS ring, which keeps enough bytes in
+// (32) loads ahead through registers into an LDS ring, which keeps enough bytes in
 // flight to cover HBM latency with one wave per SIMD.
 #include "common.hpp"
 
@@ -374,6 +374,40 @@ __global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog,
   }
 }
 
+// Insertion costs of many (c, a, b) queries: block q sums, over the words of
+// the sites, w_s [P_c & med(F_a, F_b) = 0] with med the Fitch median.
+__global__ __launch_bounds__(256) void k_fitch_insert(const uint64_t* __restrict__ pre,
+                                                      const uint64_t* __restrict__ fin,
+                                                      int64_t stride, int64_t W, int64_t n_sites,
+                                                      const uint32_t* __restrict__ wt,
+                                                      const int32_t* __restrict__ qs,
+                                                      unsigned long long* __restrict__ cost) {
+  const int64_t q = blockIdx.x;
+  const int32_t c = qs[3 * q], a = qs[3 * q + 1], b = qs[3 * q + 2];
+  const uint64_t* pc = pre + (int64_t)c * stride;
+  const uint64_t* fa = fin + (int64_t)a * stride;
+  const uint64_t* fb = fin + (int64_t)b * stride;
+  uint64_t acc = 0;
+  for (int64_t w = threadIdx.x; w < W; w += blockDim.x) {
+    uint32_t t0, t1, elo, ehi;
+    const uint64_t m = fitch64(fa[w], fb[w], &t0, &t1);
+    (void)fitch64(m, pc[w], &elo, &ehi);
+    const uint64_t vm = valid_nibbles(w, n_sites);
+    elo &= (uint32_t)vm;
+    ehi &= (uint32_t)(vm >> 32);
+    acc += wt ? weigh(elo, ehi, wt, w) : (uint64_t)(__popc(elo) + __popc(ehi));
+  }
+  acc = wave_sum(acc);
+  __shared__ unsigned long long sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += sh[k];
+    cost[q] = t;
+  }
+}
+
 int block_size_env(const char* name, int dflt) {
   const char* s = getenv(name);
   if (!s) return dflt;
@@ -381,7 +415,7 @@ int block_size_env(const char* name, int dflt) {
   return (v == 64 || v == 128 || v == 256) ? v : dflt;
 }
 
-constexpr int kBatch = 16;
+constexpr int kBatch = 32;
 
 }  // namespace
 
@@ -485,6 +519,21 @@ int phy_fitch_uppass(phy_ctx* ctx, const phy_topo* topo, const uint64_t* sets, u
   if (lds > 160 * 1024) return set_err(ctx, PHY_ERANGE, "uppass: LDS stack too deep for this tree");
   hipLaunchKernelGGL((k_fitch_up<kBatch>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
                      topo->d_up_prog, N - 1, topo->d_up_fifo, 2 * N - 1, sets, fin, stride, W, N);
+  PHY_LAUNCHED(ctx);
+  return PHY_OK;
+}
+
+int phy_fitch_insertion_costs(phy_ctx* ctx, const uint64_t* pre, const uint64_t* fin,
+                              int64_t stride, int64_t n_sites, const uint32_t* wt, int64_t n_q,
+                              const int32_t* d_q, uint64_t* d_costs) {
+  if (!ctx) return PHY_EINVAL;
+  const int64_t W = phy_words(n_sites);
+  if (n_sites < 0 || stride < W || n_q < 0 || (n_q > 0 && (!pre || !fin || !d_q || !d_costs)))
+    return set_err(ctx, PHY_EINVAL, "insertion costs: bad argument");
+  if (n_q == 0) return PHY_OK;
+  if (n_q > 0x7fffffff) return set_err(ctx, PHY_ERANGE, "insertion costs: too many queries");
+  hipLaunchKernelGGL(k_fitch_insert, dim3((unsigned)n_q), dim3(256), 0, ctx->stream, pre, fin, stride, W,
+                     n_sites, wt, d_q, (unsigned long long*)d_costs);
   PHY_LAUNCHED(ctx);
   return PHY_OK;
 }

@@ -19,7 +19,9 @@
 // two when its maximum drops below 2^-24 (FP32) or 2^-128 (FP64); the
 // per-pattern int32 exponent carries the factor, so FP32 partials cover the
 // FP64 dynamic range without rounding from the scaling itself.
+#include <algorithm>
 #include <cmath>
+#include <thread>
 
 #include "common.hpp"
 #include "model_host.hpp"
@@ -194,26 +196,83 @@ struct TipStream {
 constexpr int kTipBatch = 8;
 
 // ---------------------------------------------------------------------------
-// Whole-tree post-order pass
+// Whole-tree post-order pass.
+//
+// One thread per (pattern, rate): lanes p*R .. p*R+R-1 hold the R rate classes
+// of pattern p (4 states each), so a wave carries 64/R patterns and the grid
+// has R times more waves than patterns/64 -- enough to hide memory latency.
+// The transition matrices of the two children of the node being computed are
+// staged in an LDS double buffer by the block (prefetched one node ahead),
+// since every wave of the block walks the same program.  Per-pattern
+// quantities (rescaling maximum, root sum) are reduced across the R lanes
+// with xor shuffles.
+//
+// Stored partial of internal node v: row i (state) of 4 rows, column
+// p*R + r: part[((v - N) * 4 + i) * (pitch * R) + p * R + r], exponents
+// ex[(v - N) * pitch + p].
 // ---------------------------------------------------------------------------
+template <typename real, int R>
+__device__ __forceinline__ real rate_max(real x) {
+#pragma unroll
+  for (int o = 1; o < R; o <<= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  return x;
+}
+template <int R>
+__device__ __forceinline__ double rate_sum(double x) {
+#pragma unroll
+  for (int o = 1; o < R; o <<= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+// exact power-of-two rescale of a pattern's R x 4 values held 4 per lane
+template <typename real, int R>
+__device__ __forceinline__ void rescale_pattern(real* v, int* e) {
+  real mx = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+  mx = rate_max<real, R>(mx);
+  if (mx < Thr<real>::lo && mx > (real)0) {
+    int ex;
+    (void)frexp(mx, &ex);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = ldexp(v[k], -ex);
+    *e += ex;
+  }
+}
+
+// element j of the matrix block staged for node entry op: matrices ordered
+// (which, child) = P_a, P_b, P'_a, P'_b, P''_a, P''_b, R x 16 reals each
+template <typename real, int R>
+__device__ __forceinline__ real stage_elem(const int4 op, int j, const real* __restrict__ m0,
+                                           const real* __restrict__ m1, const real* __restrict__ m2) {
+  const int mat = j / (R * 16), rem = j % (R * 16);
+  const int which = mat >> 1, c = mat & 1;
+  const int child = c ? op.z : op.y;
+  const real* base = which == 0 ? m0 : (which == 1 ? m1 : m2);
+  return base[(size_t)child * R * 16 + rem];
+}
+
 template <typename real, int R, bool FUSED>
-__global__ __launch_bounds__(256) void k_prune_down(
+__device__ __forceinline__ void prune_down_body(
     const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids, int n_leaf_ids,
     const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
     const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
     real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
-    double* __restrict__ site_lnl, double* __restrict__ block_sums, int N, int stack_depth) {
-  constexpr int RK = R * K4;
+    double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth) {
+  constexpr int NM = 2 * R * 16;
+  constexpr int PF = (NM + 63) / 64;  // prefetch registers per thread (block >= 64)
   extern __shared__ unsigned char lds_raw[];
   const int tid = threadIdx.x, bd = blockDim.x;
-  const int64_t p = (int64_t)blockIdx.x * bd + tid;
+  const int r = tid % R;
+  const int64_t p = (int64_t)blockIdx.x * (bd / R) + tid / R;
   const bool act = p < n_pat;
   const int64_t pc = act ? p : 0;
+  const int64_t rowlen = pitch * R;
+  const int64_t col = p * R + r;
 
-  // LDS: tip ring [kTipBatch][bd] u32 | stack values [depth][RK][bd] | stack exponents [depth][bd]
-  uint32_t* ring = reinterpret_cast<uint32_t*>(lds_raw);
-  real* stk = reinterpret_cast<real*>(lds_raw + (size_t)kTipBatch * bd * sizeof(uint32_t));
-  int32_t* stke = reinterpret_cast<int32_t*>(stk + (size_t)stack_depth * RK * bd);
+  real* Pbuf = reinterpret_cast<real*>(lds_raw);
+  uint32_t* ring = reinterpret_cast<uint32_t*>(Pbuf + 2 * NM);
+  real* stk = reinterpret_cast<real*>(ring + (size_t)kTipBatch * bd);
+  int32_t* stke = reinterpret_cast<int32_t*>(stk + (size_t)stack_depth * 4 * bd);
+  for (int j = tid; j < NM; j += bd) Pbuf[j] = stage_elem<real, R>(prog[0], j, Pm, Pm, Pm);
   TipStream<kTipBatch> ts;
   ts.ring = ring;
   ts.ids = leaf_ids;
@@ -224,92 +283,125 @@ __global__ __launch_bounds__(256) void k_prune_down(
   ts.bd = bd;
   ts.tid = tid;
   ts.init();
+  __syncthreads();
 
-  real acc[RK];
-  int acc_e = 0;
-  int sp = 0;
-#pragma unroll
-  for (int k = 0; k < RK; ++k) acc[k] = (real)0;
-
+  real acc[4] = {(real)0, (real)0, (real)0, (real)0};
+  int acc_e = 0, sp = 0;
   for (int it = 0; it < n_ops; ++it) {
     const int4 op = prog[it];
+    const bool more = it + 1 < n_ops;
+    real nxt[PF];
+    if (more) {
+      const int4 on = prog[it + 1];
+#pragma unroll
+      for (int k = 0; k < PF; ++k) {
+        const int j = tid + k * bd;
+        if (j < NM) nxt[k] = stage_elem<real, R>(on, j, Pm, Pm, Pm);
+      }
+    }
+    const real* Pc = Pbuf + (it & 1) * NM;
+    const real* PA = Pc + r * 16;
+    const real* PB = Pc + (R + r) * 16;
     const int fl = op.w;
     if (FUSED && (fl & DOWN_SPILL)) {
 #pragma unroll
-      for (int k = 0; k < RK; ++k) stk[((size_t)sp * RK + k) * bd + tid] = acc[k];
+      for (int k = 0; k < 4; ++k) stk[((size_t)sp * 4 + k) * bd + tid] = acc[k];
       stke[(size_t)sp * bd + tid] = acc_e;
       ++sp;
     }
     const int sa = fl & 3, sb = (fl >> 2) & 3;
-    real xa[RK], xb[RK];
+    real xa[4], xb[4], src[4];
     int ea = 0, eb = 0;
-    // operand A
     if (sa == SRC_LEAF) {
-      real t[4];
-      tipvec(ts.get(), t);
+      tipvec(ts.get(), src);
+    } else if (sa == SRC_ACC) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) matvec4(Pm + ((size_t)op.y * R + r) * 16, t, xa + r * 4);
+      for (int k = 0; k < 4; ++k) src[k] = acc[k];
+      ea = acc_e;
+    } else if (FUSED) {
+      --sp;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) src[k] = stk[((size_t)sp * 4 + k) * bd + tid];
+      ea = stke[(size_t)sp * bd + tid];
     } else {
-      real src[RK];
-      if (sa == SRC_ACC) {
+      const real* s = part + (size_t)(op.y - N) * 4 * rowlen + (pc * R + r);
 #pragma unroll
-        for (int k = 0; k < RK; ++k) src[k] = acc[k];
-        ea = acc_e;
-      } else if (FUSED) {
-        --sp;
-#pragma unroll
-        for (int k = 0; k < RK; ++k) src[k] = stk[((size_t)sp * RK + k) * bd + tid];
-        ea = stke[(size_t)sp * bd + tid];
-      } else {
-        const real* s = part + (size_t)(op.y - N) * RK * pitch + pc;
-#pragma unroll
-        for (int k = 0; k < RK; ++k) src[k] = s[(size_t)k * pitch];
-        ea = ex[(size_t)(op.y - N) * pitch + pc];
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r) matvec4(Pm + ((size_t)op.y * R + r) * 16, src + r * 4, xa + r * 4);
+      for (int k = 0; k < 4; ++k) src[k] = s[(size_t)k * rowlen];
+      ea = ex[(size_t)(op.y - N) * pitch + pc];
     }
-    // operand B (a leaf or the accumulator)
+    matvec4(PA, src, xa);
     if (sb == SRC_LEAF) {
-      real t[4];
-      tipvec(ts.get(), t);
-#pragma unroll
-      for (int r = 0; r < R; ++r) matvec4(Pm + ((size_t)op.z * R + r) * 16, t, xb + r * 4);
+      tipvec(ts.get(), src);
     } else {
 #pragma unroll
-      for (int r = 0; r < R; ++r) matvec4(Pm + ((size_t)op.z * R + r) * 16, acc + r * 4, xb + r * 4);
+      for (int k = 0; k < 4; ++k) src[k] = acc[k];
       eb = acc_e;
     }
+    matvec4(PB, src, xb);
 #pragma unroll
-    for (int k = 0; k < RK; ++k) acc[k] = xa[k] * xb[k];
+    for (int k = 0; k < 4; ++k) acc[k] = xa[k] * xb[k];
     acc_e = ea + eb;
-    rescale<real, RK>(acc, &acc_e);
+    rescale_pattern<real, R>(acc, &acc_e);
     if (!FUSED && act) {
-      real* d = part + (size_t)(op.x - N) * RK * pitch + p;
+      real* d = part + (size_t)(op.x - N) * 4 * rowlen + col;
 #pragma unroll
-      for (int k = 0; k < RK; ++k) d[(size_t)k * pitch] = acc[k];
-      ex[(size_t)(op.x - N) * pitch + p] = acc_e;
+      for (int k = 0; k < 4; ++k) d[(size_t)k * rowlen] = acc[k];
+      if (r == 0) ex[(size_t)(op.x - N) * pitch + p] = acc_e;
     }
+    if (more) {
+#pragma unroll
+      for (int k = 0; k < PF; ++k) {
+        const int j = tid + k * bd;
+        if (j < NM) Pbuf[((it + 1) & 1) * NM + j] = nxt[k];
+      }
+    }
+    __syncthreads();
   }
   // root: ln L_p = ln sum_r prob_r sum_i pi_i L[r][i] + e ln 2
-  double lk = 0.0;
+  double s = 0.0;
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) s += pi[i] * (double)acc[r * 4 + i];
-    lk += probs[r] * s;
-  }
-  const double slnl = log(lk) + (double)acc_e * 0.6931471805599453;
+  for (int i = 0; i < 4; ++i) s += pi[i] * (double)acc[i];
+  s = rate_sum<R>(s * probs[r]);
+  const double slnl = log(s) + (double)acc_e * 0.6931471805599453;
   double contrib = 0.0;
-  if (act) {
+  if (act && r == 0) {
     if (site_lnl) site_lnl[p] = slnl;
     contrib = (double)weights[p] * slnl;
   }
-  // one partial sum per wave of 64 patterns: the reduction order does not
-  // depend on the block size
+  // one partial sum per wave: the reduction order does not depend on the
+  // block size
   const double ws = wave_sum_d(contrib);
-  if ((tid & 63) == 0) block_sums[p >> 6] = ws;
+  if ((tid & 63) == 0) wave_sums[((int64_t)blockIdx.x * bd + tid) >> 6] = ws;
+}
+
+template <typename real, int R, bool FUSED>
+__global__ __launch_bounds__(256) void k_prune_down(
+    const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids, int n_leaf_ids,
+    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
+    const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
+    real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
+    double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth) {
+  prune_down_body<real, R, FUSED>(prog, n_ops, leaf_ids, n_leaf_ids, tips, stride, n_pat, pitch, Pm,
+                                  pi, probs, part, ex, weights, site_lnl, wave_sums, N,
+                                  stack_depth);
+}
+
+// Many candidate topologies over the same tips and weights (fused form):
+// blockIdx.y selects candidate c, whose program, leaf order, P matrices and
+// per-wave sums sit at fixed strides.
+template <typename real, int R>
+__global__ __launch_bounds__(256) void k_prune_down_batch(
+    const int4* __restrict__ prog, const int32_t* __restrict__ leaf_ids, int N,
+    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
+    const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
+    const uint32_t* __restrict__ weights, double* __restrict__ wave_sums, int64_t n_waves,
+    int stack_depth) {
+  const int64_t c = blockIdx.y;
+  const int M = 2 * N - 1;
+  prune_down_body<real, R, true>(prog + c * (N - 1), N - 1, leaf_ids + c * N, N, tips, stride,
+                                 n_pat, pitch, Pm + c * (int64_t)M * R * 16, pi, probs, nullptr,
+                                 nullptr, weights, nullptr, wave_sums + c * n_waves, N,
+                                 stack_depth);
 }
 
 // ---------------------------------------------------------------------------
@@ -317,6 +409,8 @@ __global__ __launch_bounds__(256) void k_prune_down(
 //   A_c = M_p (*) (P_s D_s),  M_root = pi,  M_c = A_c^T P_c
 //   L = sum_r prob_r A_c . (P_c D_c),  L' and L'' with P' and P''
 // per-pattern g = L'/L and h = L''/L - g^2, summed per branch over the wave.
+// Same thread mapping as the post-order pass; P, P', P'' of both children
+// are staged in LDS one node ahead.
 // ---------------------------------------------------------------------------
 template <typename real, int R>
 __global__ __launch_bounds__(256) void k_prune_deriv(
@@ -325,92 +419,96 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
     const real* __restrict__ P2m, const double* __restrict__ pi, const double* __restrict__ probs,
     const real* __restrict__ part, const uint32_t* __restrict__ weights,
     double* __restrict__ wave_sums, int N) {
-  constexpr int RK = R * K4;
+  constexpr int NM = 6 * R * 16;
+  constexpr int PF = (NM + 63) / 64;
   extern __shared__ unsigned char lds_raw[];
-  real* stk = reinterpret_cast<real*>(lds_raw);
+  real* Pbuf = reinterpret_cast<real*>(lds_raw);     // [2][NM]
+  real* stk = Pbuf + 2 * NM;                          // [depth][4][bd]
   const int tid = threadIdx.x, bd = blockDim.x;
-  const int64_t p = (int64_t)blockIdx.x * bd + tid;
+  const int r = tid % R;
+  const int64_t p = (int64_t)blockIdx.x * (bd / R) + tid / R;
   const bool act = p < n_pat;
   const int64_t pc = act ? p : 0;
+  const int64_t rowlen = pitch * R;
   const double wp = act ? (double)weights[p] : 0.0;
   const int M = 2 * N - 1;
   const int64_t wave = ((int64_t)blockIdx.x * bd + tid) >> 6;
   const int lane = tid & 63;
+  for (int j = tid; j < NM; j += bd) Pbuf[j] = stage_elem<real, R>(prog[0], j, Pm, P1m, P2m);
+  __syncthreads();
 
-  real accM[RK];
-  int sp = 0;
-#pragma unroll
-  for (int k = 0; k < RK; ++k) accM[k] = (real)0;
-
-  auto load_D = [&](int c, real* D) {
+  auto load_D = [&](int c, real* Dv) {
     if (c < N) {
-      real t[4];
-      tipvec(tip_code(tips, stride, c, pc), t);
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) D[r * 4 + i] = t[i];
+      tipvec(tip_code(tips, stride, c, pc), Dv);
     } else {
-      const real* s = part + (size_t)(c - N) * RK * pitch + pc;
+      const real* s = part + (size_t)(c - N) * 4 * rowlen + (pc * R + r);
 #pragma unroll
-      for (int k = 0; k < RK; ++k) D[k] = s[(size_t)k * pitch];
+      for (int k = 0; k < 4; ++k) Dv[k] = s[(size_t)k * rowlen];
     }
   };
 
+  real accM[4] = {(real)0, (real)0, (real)0, (real)0};
+  int sp = 0;
   for (int it = 0; it < n_ops; ++it) {
     const int4 op = prog[it];
+    const bool more = it + 1 < n_ops;
+    real nxt[PF];
+    if (more) {
+      const int4 on = prog[it + 1];
+#pragma unroll
+      for (int k = 0; k < PF; ++k) {
+        const int j = tid + k * bd;
+        if (j < NM) nxt[k] = stage_elem<real, R>(on, j, Pm, P1m, P2m);
+      }
+    }
+    const real* Pc = Pbuf + (it & 1) * NM;
+    // matrix m = which*2 + child: P_a, P_b, P1_a, P1_b, P2_a, P2_b
+    const real* Pa = Pc + (0 * R + r) * 16;
+    const real* Pb = Pc + (1 * R + r) * 16;
+    const real* P1a = Pc + (2 * R + r) * 16;
+    const real* P1b = Pc + (3 * R + r) * 16;
+    const real* P2a = Pc + (4 * R + r) * 16;
+    const real* P2b = Pc + (5 * R + r) * 16;
     const int fl = op.w, src = fl & 3;
     const int a = op.y, b = op.z;
-    real Mv[RK];
-    if (src == SRC_LEAF) {  // root
+    real Mv[4];
+    if (src == SRC_LEAF) {
 #pragma unroll
-      for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) Mv[r * 4 + i] = (real)pi[i];
+      for (int i = 0; i < 4; ++i) Mv[i] = (real)pi[i];
     } else if (src == SRC_ACC) {
 #pragma unroll
-      for (int k = 0; k < RK; ++k) Mv[k] = accM[k];
+      for (int k = 0; k < 4; ++k) Mv[k] = accM[k];
     } else {
       --sp;
 #pragma unroll
-      for (int k = 0; k < RK; ++k) Mv[k] = stk[((size_t)sp * RK + k) * bd + tid];
+      for (int k = 0; k < 4; ++k) Mv[k] = stk[((size_t)sp * 4 + k) * bd + tid];
     }
-    real Da[RK], Db[RK], ya[RK], yb[RK];
+    real Da[4], Db[4], ya[4], yb[4], x1a[4], x2a[4], x1b[4], x2b[4];
     load_D(a, Da);
     load_D(b, Db);
+    matvec4(Pa, Da, ya);
+    matvec4(Pb, Db, yb);
+    matvec4(P1a, Da, x1a);
+    matvec4(P2a, Da, x2a);
+    matvec4(P1b, Db, x1b);
+    matvec4(P2b, Db, x2b);
+    real s0 = 0, s1a = 0, s2a = 0, s1b = 0, s2b = 0;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      matvec4(Pm + ((size_t)a * R + r) * 16, Da + r * 4, ya + r * 4);
-      matvec4(Pm + ((size_t)b * R + r) * 16, Db + r * 4, yb + r * 4);
+    for (int i = 0; i < 4; ++i) {
+      const real ma = Mv[i] * yb[i];  // A_a
+      const real mb = Mv[i] * ya[i];  // A_b
+      s0 = fma(ma, ya[i], s0);
+      s1a = fma(ma, x1a[i], s1a);
+      s2a = fma(ma, x2a[i], s2a);
+      s1b = fma(mb, x1b[i], s1b);
+      s2b = fma(mb, x2b[i], s2b);
     }
-    double l0 = 0.0, l1a = 0.0, l2a = 0.0, l1b = 0.0, l2b = 0.0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      real x1a[4], x2a[4], x1b[4], x2b[4];
-      matvec4(P1m + ((size_t)a * R + r) * 16, Da + r * 4, x1a);
-      matvec4(P2m + ((size_t)a * R + r) * 16, Da + r * 4, x2a);
-      matvec4(P1m + ((size_t)b * R + r) * 16, Db + r * 4, x1b);
-      matvec4(P2m + ((size_t)b * R + r) * 16, Db + r * 4, x2b);
-      real s0 = 0, s1a = 0, s2a = 0, s1b = 0, s2b = 0;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const real ma = Mv[r * 4 + i] * yb[r * 4 + i];  // A_a
-        const real mb = Mv[r * 4 + i] * ya[r * 4 + i];  // A_b
-        s0 = fma(ma, ya[r * 4 + i], s0);
-        s1a = fma(ma, x1a[i], s1a);
-        s2a = fma(ma, x2a[i], s2a);
-        s1b = fma(mb, x1b[i], s1b);
-        s2b = fma(mb, x2b[i], s2b);
-      }
-      const double pr = probs[r];
-      l0 += pr * (double)s0;
-      l1a += pr * (double)s1a;
-      l2a += pr * (double)s2a;
-      l1b += pr * (double)s1b;
-      l2b += pr * (double)s2b;
-    }
+    const double pr = probs[r];
+    const double l0 = rate_sum<R>(pr * (double)s0);
+    const double l1a = rate_sum<R>(pr * (double)s1a), l2a = rate_sum<R>(pr * (double)s2a);
+    const double l1b = rate_sum<R>(pr * (double)s1b), l2b = rate_sum<R>(pr * (double)s2b);
     double ga = 0, ha = 0, gb = 0, hb = 0;
-    if (act) {
+    if (act && r == 0) {
       const double inv = 1.0 / l0;
       const double g1 = l1a * inv, g2 = l1b * inv;
       ga = wp * g1;
@@ -430,39 +528,41 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
       ws[2 * b + 1] = hb;
     }
     // messages to internal children: M_c = (M (*) P_s D_s)^T P_c
-    real Ma[RK], Mb[RK];
+    real Ma[4], Mb[4];
     int dummy = 0;
     if (!(fl & UP_A_LEAF)) {
+      real Av[4];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        real A[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) A[i] = Mv[r * 4 + i] * yb[r * 4 + i];
-        vecmat4(Pm + ((size_t)a * R + r) * 16, A, Ma + r * 4);
-      }
-      rescale<real, RK>(Ma, &dummy);
+      for (int i = 0; i < 4; ++i) Av[i] = Mv[i] * yb[i];
+      vecmat4(Pa, Av, Ma);
+      rescale_pattern<real, R>(Ma, &dummy);
     }
     if (!(fl & UP_B_LEAF)) {
+      real Av[4];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        real A[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) A[i] = Mv[r * 4 + i] * ya[r * 4 + i];
-        vecmat4(Pm + ((size_t)b * R + r) * 16, A, Mb + r * 4);
-      }
-      rescale<real, RK>(Mb, &dummy);
+      for (int i = 0; i < 4; ++i) Av[i] = Mv[i] * ya[i];
+      vecmat4(Pb, Av, Mb);
+      rescale_pattern<real, R>(Mb, &dummy);
     }
     if (fl & UP_PUSH) {
       const real* o = (fl & UP_CONT_B) ? Ma : Mb;
 #pragma unroll
-      for (int k = 0; k < RK; ++k) stk[((size_t)sp * RK + k) * bd + tid] = o[k];
+      for (int k = 0; k < 4; ++k) stk[((size_t)sp * 4 + k) * bd + tid] = o[k];
       ++sp;
     }
     if (!(fl & UP_A_LEAF) || !(fl & UP_B_LEAF)) {
       const real* c = (fl & UP_CONT_B) ? Mb : Ma;
 #pragma unroll
-      for (int k = 0; k < RK; ++k) accM[k] = c[k];
+      for (int k = 0; k < 4; ++k) accM[k] = c[k];
     }
+    if (more) {
+#pragma unroll
+      for (int k = 0; k < PF; ++k) {
+        const int j = tid + k * bd;
+        if (j < NM) Pbuf[((it + 1) & 1) * NM + j] = nxt[k];
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -492,6 +592,31 @@ __global__ void k_reduce_sum(const double* __restrict__ x, int64_t n, int64_t st
       Cc += cc[k];
     }
     *out = S + Cc;
+  }
+}
+
+// one block per row: out[row] = Neumaier sum of x[row * n .. row * n + n)
+__global__ void k_reduce_rows(const double* __restrict__ x, int64_t n, double* __restrict__ out) {
+  __shared__ double ss[256], cc[256];
+  const double* r = x + (int64_t)blockIdx.x * n;
+  double s = 0.0, c = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = r[i], t = s + v;
+    if (fabs(s) >= fabs(v)) c += (s - t) + v; else c += (v - t) + s;
+    s = t;
+  }
+  ss[threadIdx.x] = s;
+  cc[threadIdx.x] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double S = 0.0, C = 0.0;
+    for (int k = 0; k < (int)blockDim.x; ++k) {
+      const double v = ss[k], t = S + v;
+      if (fabs(S) >= fabs(v)) C += (S - t) + v; else C += (v - t) + S;
+      S = t;
+      C += cc[k];
+    }
+    out[blockIdx.x] = S + C;
   }
 }
 
@@ -618,22 +743,49 @@ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 struct Plan {
   size_t off_P = 0, off_P1 = 0, off_P2 = 0, off_part = 0, off_ex = 0, off_bsum = 0, off_wsum = 0;
   size_t total = 0;
-  int64_t pitch = 0, n_blocks = 0, n_waves = 0;
-  int bs = 0;
-  size_t esz = 8;
+  int64_t pitch = 0, n_blocks = 0, n_waves = 0, dblocks = 0;
+  int bs = 0, dbs = 0;  // block size of the post-order pass / derivative pass
+  size_t esz = 8, lds = 0, dlds = 0;
 };
+
+constexpr size_t kLdsMax = 64 * 1024;
+
+size_t down_lds(int R, size_t esz, int bs, bool fused, int depth) {
+  return (size_t)2 * 2 * R * 16 * esz + (size_t)kTipBatch * bs * 4 +
+         (fused ? (size_t)std::max(depth, 1) * bs * (4 * esz + 4) : 0);
+}
+size_t deriv_lds(int R, size_t esz, int bs, int depth) {
+  return (size_t)2 * 6 * R * 16 * esz + (size_t)std::max(depth, 1) * bs * 4 * esz;
+}
+
+// largest block size <= want whose LDS fits (0 if none)
+template <class F>
+int fit_block(int want, F lds_of) {
+  for (int bs = want; bs >= 64; bs /= 2)
+    if (lds_of(bs) <= kLdsMax) return bs;
+  return 0;
+}
 
 Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t form, int32_t derivs) {
   Plan pl;
   const int N = t->n_leaves, M = 2 * N - 1, R = m->R, K = m->K;
   pl.esz = (form & PHY_F32) ? 4 : 8;
   const bool fused = (form & PHY_FUSED) != 0;
-  pl.bs = env_block(fused ? "PHY_PRUNE_FUSED_BLOCK" : "PHY_PRUNE_BLOCK", fused ? 64 : 128);
-  if (derivs) pl.bs = env_block("PHY_PRUNE_DERIV_BLOCK", 64);
+  const size_t esz = pl.esz;
+  const int depth = t->down_depth;
+  pl.bs = fit_block(env_block(fused ? "PHY_PRUNE_FUSED_BLOCK" : "PHY_PRUNE_BLOCK", 256),
+                    [&](int bs) { return down_lds(R, esz, bs, fused, depth); });
+  pl.lds = pl.bs ? down_lds(R, esz, pl.bs, fused, depth) : 0;
+  if (derivs) {
+    pl.dbs = fit_block(env_block("PHY_PRUNE_DERIV_BLOCK", 256),
+                       [&](int bs) { return deriv_lds(R, esz, bs, t->up_depth); });
+    pl.dlds = pl.dbs ? deriv_lds(R, esz, pl.dbs, t->up_depth) : 0;
+  }
   pl.pitch = ((n_pat + 255) / 256) * 256;
   if (pl.pitch == 0) pl.pitch = 256;
-  pl.n_blocks = pl.pitch / pl.bs;
-  pl.n_waves = pl.pitch / 64;
+  pl.n_blocks = pl.bs ? pl.pitch * R / pl.bs : 0;
+  pl.dblocks = pl.dbs ? pl.pitch * R / pl.dbs : 0;
+  pl.n_waves = pl.pitch * R / 64;
   size_t o = 0;
   const size_t pm = (size_t)M * R * K * K * pl.esz;
   pl.off_P = o; o = align256(o + pm);
@@ -665,6 +817,8 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
   real* P = reinterpret_cast<real*>(work + pl.off_P);
   real* P1 = derivs ? reinterpret_cast<real*>(work + pl.off_P1) : nullptr;
   real* P2 = derivs ? reinterpret_cast<real*>(work + pl.off_P2) : nullptr;
+  if (!pl.bs) return set_err(ctx, PHY_ERANGE, "pruning: LDS stack too deep for this tree");
+  if (derivs && !pl.dbs) return set_err(ctx, PHY_ERANGE, "pruning: derivative LDS stack too deep");
   {
     const int64_t n = (int64_t)M * R * K * K;
     hipLaunchKernelGGL((k_pmats<real>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
@@ -674,28 +828,20 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
   real* part = fused ? nullptr : reinterpret_cast<real*>(work + pl.off_part);
   int32_t* ex = fused ? nullptr : reinterpret_cast<int32_t*>(work + pl.off_ex);
   double* bsum = reinterpret_cast<double*>(work + pl.off_bsum);
-  const int bs = pl.bs;
-  const size_t ring = (size_t)kTipBatch * bs * sizeof(uint32_t);
-  if (fused) {
-    const size_t lds = ring + (size_t)t->down_depth * bs * (R * K * sizeof(real) + sizeof(int32_t));
-    if (lds > 64 * 1024)
-      return set_err(ctx, PHY_ERANGE, "pruning: fused LDS stack exceeds 64 KiB for this tree");
-    hipLaunchKernelGGL((k_prune_down<real, R, true>), dim3((unsigned)pl.n_blocks), dim3(bs), lds,
+  if (fused)
+    hipLaunchKernelGGL((k_prune_down<real, R, true>), dim3((unsigned)pl.n_blocks), dim3(pl.bs), pl.lds,
                        ctx->stream, t->d_down_prog, N - 1, t->d_down_fifo, N, tips, stride, n_pat,
                        pl.pitch, P, d_pi, d_probs, part, ex, weights, site_lnl, bsum, N, t->down_depth);
-  } else {
-    hipLaunchKernelGGL((k_prune_down<real, R, false>), dim3((unsigned)pl.n_blocks), dim3(bs), ring,
+  else
+    hipLaunchKernelGGL((k_prune_down<real, R, false>), dim3((unsigned)pl.n_blocks), dim3(pl.bs), pl.lds,
                        ctx->stream, t->d_down_prog, N - 1, t->d_down_fifo, N, tips, stride, n_pat,
                        pl.pitch, P, d_pi, d_probs, part, ex, weights, site_lnl, bsum, N, 0);
-  }
   PHY_LAUNCHED(ctx);
   hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, ctx->stream, bsum, pl.n_waves, (int64_t)1, lnl);
   PHY_LAUNCHED(ctx);
   if (derivs) {
     double* wsum = reinterpret_cast<double*>(work + pl.off_wsum);
-    const size_t lds = (size_t)t->up_depth * R * K * bs * sizeof(real);
-    if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "pruning: derivative stack too deep");
-    hipLaunchKernelGGL((k_prune_deriv<real, R>), dim3((unsigned)pl.n_blocks), dim3(bs), lds,
+    hipLaunchKernelGGL((k_prune_deriv<real, R>), dim3((unsigned)pl.dblocks), dim3(pl.dbs), pl.dlds,
                        ctx->stream, t->d_up_prog, N - 1, tips, stride, n_pat, pl.pitch, P, P1, P2,
                        d_pi, d_probs, part, weights, wsum, N);
     PHY_LAUNCHED(ctx);
@@ -810,7 +956,7 @@ int phy_lk_tree_partials(const phy_topo* t, const phy_model* m, int64_t n_pat, i
   if (v < N || v > 2 * N - 2) return PHY_EINVAL;
   const Plan pl = make_plan(t, m, n_pat, form, derivs);
   char* w = static_cast<char*>(work);
-  if (d_partial) *d_partial = w + pl.off_part + (size_t)(v - N) * m->R * m->K * pl.pitch * pl.esz;
+  if (d_partial) *d_partial = w + pl.off_part + (size_t)(v - N) * m->R * m->K * pl.pitch * pl.esz;  // [4][pitch*R]
   if (d_exp) *d_exp = reinterpret_cast<int32_t*>(w + pl.off_ex) + (size_t)(v - N) * pl.pitch;
   if (pitch) *pitch = pl.pitch;
   return PHY_OK;
@@ -839,6 +985,148 @@ int phy_lk_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, int32_t for
                                 site_lnl, d1, d2, w, pl);
   return dispatch_tree<double>(ctx, t, m, fused, derivs, n_pat, tips, stride, weights, blen, lnl,
                                site_lnl, d1, d2, w, pl);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched SPR candidate scoring
+// ---------------------------------------------------------------------------
+extern "C" int phy_spr_apply(int32_t N, const int32_t* children, int32_t root, const double* blen,
+                             int32_t s, int32_t v, int32_t* children_out, int32_t* root_out,
+                             double* blen_out);
+
+namespace {
+
+struct SprPlan {
+  size_t off_P, off_bl, off_prog, off_fifo, off_ws, total;
+  int64_t pitch, n_blocks, n_waves;
+  int bs, depth_cap;
+};
+
+SprPlan spr_plan(int32_t N, const phy_model* m, int64_t n_pat, int32_t form, int32_t chunk) {
+  SprPlan pl;
+  const int M = 2 * N - 1;
+  const size_t esz = (form & PHY_F32) ? 4 : 8;
+  pl.bs = env_block("PHY_PRUNE_FUSED_BLOCK", 256);
+  pl.pitch = ((n_pat + 255) / 256) * 256;
+  pl.n_blocks = pl.pitch * m->R / pl.bs;
+  pl.n_waves = pl.pitch * m->R / 64;
+  pl.depth_cap = 12;
+  size_t o = 0;
+  pl.off_P = o; o = align256(o + (size_t)chunk * M * m->R * 16 * esz);
+  pl.off_bl = o; o = align256(o + (size_t)chunk * M * 8);
+  pl.off_prog = o; o = align256(o + (size_t)chunk * (N - 1) * 16);
+  pl.off_fifo = o; o = align256(o + (size_t)chunk * N * 4);
+  pl.off_ws = o; o = align256(o + (size_t)chunk * pl.n_waves * 8);
+  pl.total = o;
+  return pl;
+}
+
+template <typename real, int R>
+int spr_launch(phy_ctx* ctx, const phy_model* m, int N, int64_t n_pat, const uint64_t* tips,
+               int64_t stride, const uint32_t* w, int C, int depth, char* work, const SprPlan& pl,
+               double* out) {
+  const int M = 2 * N - 1, K = 4;
+  const double* cst = m->d_consts;
+  const double* d_pi = cst + 2 * K * K + K + R;
+  real* P = reinterpret_cast<real*>(work + pl.off_P);
+  const double* bl = reinterpret_cast<const double*>(work + pl.off_bl);
+  const int64_t n = (int64_t)C * M * R * 16;
+  hipLaunchKernelGGL((k_pmats<real>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, cst,
+                     K, R, bl, 0.0, 0.0, C * M, P, (real*)nullptr, (real*)nullptr);
+  PHY_LAUNCHED(ctx);
+  const int bs = fit_block(pl.bs, [&](int b) { return down_lds(R, sizeof(real), b, true, depth); });
+  if (!bs) return set_err(ctx, PHY_ERANGE, "spr scores: LDS stack too deep");
+  const size_t lds = down_lds(R, sizeof(real), bs, true, depth);
+  double* ws = reinterpret_cast<double*>(work + pl.off_ws);
+  hipLaunchKernelGGL((k_prune_down_batch<real, R>), dim3((unsigned)(pl.pitch * R / bs), (unsigned)C), dim3(bs),
+                     lds, ctx->stream, reinterpret_cast<const int4*>(work + pl.off_prog),
+                     reinterpret_cast<const int32_t*>(work + pl.off_fifo), N, tips, stride, n_pat,
+                     pl.pitch, P, d_pi, d_pi + K, w, ws, pl.n_waves, depth);
+  PHY_LAUNCHED(ctx);
+  hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)C), dim3(256), 0, ctx->stream, ws, pl.n_waves, out);
+  PHY_LAUNCHED(ctx);
+  return PHY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phy_lk_spr_work_size(int32_t N, const phy_model* m, int64_t n_pat, int32_t form, int32_t chunk,
+                         size_t* bytes) {
+  if (!m || !bytes || N < 3 || n_pat < 1 || chunk < 1) return PHY_EINVAL;
+  *bytes = spr_plan(N, m, n_pat, form, chunk).total;
+  return PHY_OK;
+}
+
+int phy_lk_spr_scores(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat,
+                      const uint64_t* tips, int64_t stride, const uint32_t* weights, int32_t N,
+                      const int32_t* children, int32_t root, const double* blen, int64_t n_moves,
+                      const int32_t* moves, int32_t chunk, double* d_lnl, void* d_work,
+                      size_t work_bytes) {
+  if (!ctx || !m) return PHY_EINVAL;
+  if (m->K != 4) return set_err(ctx, PHY_EINVAL, "spr scores: 4-state models only");
+  if (N < 3 || n_pat < 1 || chunk < 1 || n_moves < 0 || !children || !blen || !tips || !weights ||
+      !d_lnl || !d_work || (n_moves > 0 && !moves))
+    return set_err(ctx, PHY_EINVAL, "spr scores: bad argument");
+  if (stride < phy_words(n_pat)) return set_err(ctx, PHY_EINVAL, "spr scores: tip stride < words");
+  const SprPlan pl = spr_plan(N, m, n_pat, form, chunk);
+  if (work_bytes < pl.total) return set_err(ctx, PHY_EINVAL, "spr scores: workspace too small");
+  const int M = 2 * N - 1;
+  char* work = static_cast<char*>(d_work);
+  std::vector<int32_t> prog, fifo;
+  std::vector<double> bl;
+  for (int64_t base = 0; base < n_moves; base += chunk) {
+    const int C = (int)std::min<int64_t>(chunk, n_moves - base);
+    prog.assign((size_t)C * (N - 1) * 4, 0);
+    fifo.assign((size_t)C * N, 0);
+    bl.assign((size_t)C * M, 0.0);
+    std::vector<int> depth(C, 0), bad(C, 0);
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned k = 0; k < nt; ++k)
+      th.emplace_back([&, k] {
+        std::vector<int32_t> ch((size_t)2 * (N - 1));
+        for (int c = (int)k; c < C; c += (int)nt) {
+          int32_t r = -1;
+          const int32_t s = moves[2 * (base + c)], v = moves[2 * (base + c) + 1];
+          if (phy_spr_apply(N, children, root, blen, s, v, ch.data(), &r, &bl[(size_t)c * M]) != PHY_OK) {
+            bad[c] = 1;
+            continue;
+          }
+          phy_topo* t = nullptr;
+          if (phy_topo_create(nullptr, N, ch.data(), r, &t) != PHY_OK) {
+            bad[c] = 1;
+            continue;
+          }
+          std::copy(t->down_prog.begin(), t->down_prog.end(), prog.begin() + (size_t)c * (N - 1) * 4);
+          std::copy(t->down_fifo.begin(), t->down_fifo.end(), fifo.begin() + (size_t)c * N);
+          depth[c] = t->down_depth;
+          phy_topo_destroy(t);
+        }
+      });
+    for (auto& x : th) x.join();
+    for (int c = 0; c < C; ++c)
+      if (bad[c]) return set_err(ctx, PHY_EINVAL, "spr scores: invalid move " + std::to_string(base + c));
+    const int dmax = *std::max_element(depth.begin(), depth.end());
+    PHY_HIP(ctx, hipMemcpyAsync(work + pl.off_prog, prog.data(), prog.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    PHY_HIP(ctx, hipMemcpyAsync(work + pl.off_fifo, fifo.data(), fifo.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    PHY_HIP(ctx, hipMemcpyAsync(work + pl.off_bl, bl.data(), bl.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    int rc;
+    switch ((form & PHY_F32 ? 10 : 0) + m->R) {
+      case 1: rc = spr_launch<double, 1>(ctx, m, N, n_pat, tips, stride, weights, C, dmax, work, pl, d_lnl + base); break;
+      case 4: rc = spr_launch<double, 4>(ctx, m, N, n_pat, tips, stride, weights, C, dmax, work, pl, d_lnl + base); break;
+      case 11: rc = spr_launch<float, 1>(ctx, m, N, n_pat, tips, stride, weights, C, dmax, work, pl, d_lnl + base); break;
+      case 14: rc = spr_launch<float, 4>(ctx, m, N, n_pat, tips, stride, weights, C, dmax, work, pl, d_lnl + base); break;
+      default: rc = set_err(ctx, PHY_ERANGE, "spr scores: n_rates must be 1 or 4");
+    }
+    if (rc) return rc;
+    // the host buffers are reused by the next chunk: wait for the copies
+    PHY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return PHY_OK;
 }
 
 }  // extern "C"

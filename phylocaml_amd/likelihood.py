@@ -150,7 +150,9 @@ class Evaluator:
             D.dptr(self.work), self.work_bytes), "lk_tree")
 
     def partial(self, v: int):
-        """(partial [R*K, pitch], exps [pitch]) of internal node v after run()."""
+        """(partial [R*K, pitch] rate-major view, exps [pitch]) of internal node v
+        after run().  In the workspace the partial is stored as [K][pitch*R]
+        (column p*R + r), the layout the kernels write coalesced."""
         torch = __import__("torch")
         pp, pe, pitch = C.c_void_p(), C.c_void_p(), C.c_int64()
         check(lib.phy_lk_tree_partials(self.topo.handle, self.dm.handle, self.n_patterns,
@@ -159,10 +161,11 @@ class Evaluator:
         esz = 4 if self.form & PHY_F32 else 8
         base = int(self.work.data_ptr())
         off_p, off_e = pp.value - base, pe.value - base
-        RK = self.dm.R * self.dm.K
-        nbytes = RK * pitch.value * esz
-        part = self.work[off_p: off_p + nbytes].view(_dtype(self.form)).view(RK, pitch.value)
-        ex = self.work[off_e: off_e + 4 * pitch.value].view(torch.int32)
+        R, K, P = self.dm.R, self.dm.K, pitch.value
+        nbytes = R * K * P * esz
+        raw = self.work[off_p: off_p + nbytes].view(_dtype(self.form)).view(K, P, R)
+        part = raw.permute(2, 0, 1).reshape(R * K, P)
+        ex = self.work[off_e: off_e + 4 * P].view(torch.int32)
         return part, ex
 
 

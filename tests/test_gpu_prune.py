@@ -132,7 +132,7 @@ def test_stored_partials_match_oracle_pieces(ctx, rng):
         tb = (tips[1][None, :] >> np.arange(4)[:, None]) & 1
         exp = (Pa @ ta) * (Pb @ tb)
         got = part[4 * r: 4 * r + 4] * np.ldexp(1.0, ex)[None, :]
-        assert np.allclose(got, exp, rtol=1e-13, atol=0)
+        assert np.allclose(got, exp, rtol=1e-12, atol=1e-15 * np.abs(exp).max())
 
 
 def test_node_api_matches_tree(ctx, rng):
