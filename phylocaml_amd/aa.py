@@ -128,3 +128,79 @@ def tips_bitmask(tips: np.ndarray) -> np.ndarray:
     """State indices -> uint32 state bitmasks (>= 20 -> all 20 states)."""
     t = np.asarray(tips, np.uint32)
     return np.where(t < K, np.left_shift(1, np.minimum(t, 31)), (1 << K) - 1).astype(np.uint32)
+
+
+# ---------------------------------------------------------------------------
+# Autotuner: waves per workgroup (tile = 16 * waves patterns) per size bucket
+# ---------------------------------------------------------------------------
+BUCKETS = ((2, 64), (64, 128), (128, 1 << 30))  # taxa ranges [lo, hi)
+WAVE_CHOICES = (1, 2, 4)
+
+
+def bucket_of(n_leaves: int) -> int:
+    for k, (lo, hi) in enumerate(BUCKETS):
+        if lo <= n_leaves < hi:
+            return k
+    return len(BUCKETS) - 1
+
+
+def _time_batch(torch, b: AABatch, reps: int) -> float:
+    b.run()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        b.run()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def autotune(ctx: D.Context, trees: list[AATree], sample: int = 48, reps: int = 2,
+             seed: int = 0) -> dict:
+    """For each size bucket, time every waves choice on a sample of that
+    bucket's trees; returns {bucket: {"waves": w, "ms": {w: ms}, "range": ...}}."""
+    torch = __import__("torch")
+    rng = np.random.default_rng(seed)
+    out = {}
+    for k, (lo, hi) in enumerate(BUCKETS):
+        idx = [i for i, t in enumerate(trees) if lo <= t.n_leaves < hi]
+        if not idx:
+            continue
+        pick = [trees[i] for i in rng.choice(idx, min(sample, len(idx)), replace=False)]
+        ms = {}
+        for w in WAVE_CHOICES:
+            try:
+                b = AABatch(ctx, pick, waves=w)
+            except Exception:
+                continue  # this tile does not fit (LDS)
+            ms[w] = _time_batch(torch, b, reps)
+            b.close()
+        best = min(ms, key=ms.get)
+        out[k] = {"taxa": [lo, hi], "waves": best, "tile_patterns": 16 * best,
+                  "ms": {str(w): v for w, v in ms.items()}, "sample_trees": len(pick)}
+    return out
+
+
+class TunedBatch:
+    """The trees split by size bucket, one persistent launch per bucket with
+    that bucket's autotuned waves; lnl in the original tree order."""
+
+    def __init__(self, ctx: D.Context, trees: list[AATree], choice: dict):
+        torch = __import__("torch")
+        self.ctx = ctx
+        self.parts = []
+        for k in range(len(BUCKETS)):
+            idx = [i for i, t in enumerate(trees) if bucket_of(t.n_leaves) == k]
+            if not idx:
+                continue
+            w = choice.get(k, {"waves": 4})["waves"]
+            self.parts.append((np.array(idx), AABatch(ctx, [trees[i] for i in idx], waves=w)))
+        self.lnl = torch.zeros((len(trees),), dtype=torch.float64, device=f"cuda:{ctx.device}")
+        self.pattern_nodes = sum(b.pattern_nodes for _, b in self.parts)
+        self._idx = [torch.from_numpy(i).to(self.lnl.device) for i, _ in self.parts]
+
+    def run(self) -> None:
+        for (idx, b), di in zip(self.parts, self._idx):
+            b.run()
+            self.lnl.index_copy_(0, di, b.lnl)

@@ -98,6 +98,7 @@ __device__ __forceinline__ uint64_t weigh(uint32_t elo, uint32_t ehi, const uint
 // block may end in padding nibbles, which must not count as empty sites)
 __device__ __forceinline__ uint64_t valid_nibbles(int64_t w, int64_t n_sites) {
   const int64_t r = n_sites - w * 16;
+  if (r <= 0) return 0ull;
   return r >= 16 ? ~0ull : ((1ull << (4 * r)) - 1ull);
 }
 
@@ -178,28 +179,46 @@ __global__ __launch_bounds__(256) void k_fitch_elem(const uint64_t* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Whole-tree down-pass
+// Whole-tree down-pass.  The passes work on 32-bit lanes (8 sites per thread;
+// the uint64 layout read as uint32 pairs, little endian), which gives twice
+// as many waves as 64-bit lanes to hide the per-node latency chain.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t valid_nibbles32(int64_t w, int64_t n_sites) {
+  const int64_t r = n_sites - w * 8;
+  if (r <= 0) return 0u;
+  return r >= 8 ? ~0u : (uint32_t)((1ull << (4 * r)) - 1ull);
+}
+
+__device__ __forceinline__ uint64_t weigh32(uint32_t e, const uint32_t* __restrict__ wt, int64_t w) {
+  uint64_t c = 0;
+  while (e) {
+    const int k = __builtin_ctz(e) >> 2;
+    c += wt[w * 8 + k];
+    e &= e - 1;
+  }
+  return c;
+}
+
 template <int BATCH, bool NODE_COST>
 __global__ __launch_bounds__(256) void k_fitch_down(
     const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ fifo_ids, int n_fifo,
-    uint64_t* __restrict__ sets, int64_t stride, int64_t W, int64_t n_sites, int N,
+    uint32_t* __restrict__ sets, int64_t stride, int64_t W, int64_t n_sites, int N,
     const uint32_t* __restrict__ wt, uint32_t* __restrict__ site_cost,
     unsigned long long* __restrict__ node_cost, unsigned long long* __restrict__ total) {
-  extern __shared__ uint64_t lds[];
+  extern __shared__ uint32_t lds32[];
   const int tid = threadIdx.x, bd = blockDim.x;
-  const int64_t w = (int64_t)blockIdx.x * bd + tid;
-  const bool act = w < W;
-  const int64_t wc = act ? w : 0;  // clamped column for address arithmetic
-  uint64_t* ring = lds;            // BATCH slots
-  uint64_t* stk = lds + (size_t)BATCH * bd;
+  const int64_t w = (int64_t)blockIdx.x * bd + tid;  // 32-bit lane index
+  const bool act = w < W;                              // W = 2 * words
+  const int64_t wc = act ? w : 0;
+  uint32_t* ring = lds32;
+  uint32_t* stk = lds32 + (size_t)BATCH * bd;
 
-  uint64_t pre[BATCH];
+  uint32_t pre[BATCH];
   auto issue = [&](int start) {
 #pragma unroll
     for (int j = 0; j < BATCH; ++j) {
       const int idx = start + j;
-      pre[j] = (act && idx < n_fifo) ? sets[(int64_t)fifo_ids[idx] * stride + wc] : 0ull;
+      pre[j] = (act && idx < n_fifo) ? sets[(int64_t)fifo_ids[idx] * stride + wc] : 0u;
     }
   };
   issue(0);
@@ -208,7 +227,7 @@ __global__ __launch_bounds__(256) void k_fitch_down(
   issue(BATCH);
   int next_batch = 2 * BATCH;
   int fpos = 0;
-  auto next_leaf = [&]() -> uint64_t {
+  auto next_leaf = [&]() -> uint32_t {
     if (fpos == BATCH) {
 #pragma unroll
       for (int j = 0; j < BATCH; ++j) ring[j * bd + tid] = pre[j];
@@ -219,14 +238,14 @@ __global__ __launch_bounds__(256) void k_fitch_down(
     return ring[(fpos++) * bd + tid];
   };
 
-  uint32_t cnt_lo = 0, cnt_hi = 0;  // nibble counters, flushed every 15 nodes
-  uint32_t site[16];
+  uint32_t cnt = 0;  // nibble counters, flushed every 15 nodes
+  uint32_t site[8];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) site[k] = 0;
+  for (int k = 0; k < 8; ++k) site[k] = 0;
   int since = 0;
-  uint64_t acc = 0;
+  uint32_t acc = 0;
   int sp = 0;
-  const uint64_t vm = valid_nibbles(wc, n_sites);
+  const uint32_t vm = valid_nibbles32(wc, n_sites);
   for (int i = 0; i < n_ops; ++i) {
     const int4 op = prog[i];
     const int fl = op.w;
@@ -235,63 +254,55 @@ __global__ __launch_bounds__(256) void k_fitch_down(
       ++sp;
     }
     const int sa = fl & 3, sb = (fl >> 2) & 3;
-    uint64_t x, y;
+    uint32_t x, y;
     if (sa == SRC_LEAF) x = next_leaf();
     else if (sa == SRC_ACC) x = acc;
     else { --sp; x = stk[sp * bd + tid]; }
     if (sb == SRC_LEAF) y = next_leaf();
     else y = acc;  // SRC_ACC (B is never taken from the stack)
-    uint32_t elo, ehi;
-    acc = fitch64(x, y, &elo, &ehi);
+    uint32_t r;
+    const uint32_t e = fitch32(x, y, &r);
+    acc = r;
     if (act) sets[(int64_t)op.x * stride + w] = acc;
-    cnt_lo += elo;
-    cnt_hi += ehi;
+    cnt += e;
     if (NODE_COST) {
+      const uint32_t m = e & vm;
       uint64_t c = 0;
-      const uint32_t ml = elo & (uint32_t)vm, mh = ehi & (uint32_t)(vm >> 32);
-      if (act) c = wt ? weigh(ml, mh, wt, w) : (uint64_t)(__popc(ml) + __popc(mh));
+      if (act) c = wt ? weigh32(m, wt, w) : (uint64_t)__popc(m);
       c = wave_sum(c);
       if ((tid & 63) == 0 && c) atomicAdd(&node_cost[op.x - N], (unsigned long long)c);
     }
     if (++since == 15) {
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        site[k] += (cnt_lo >> (4 * k)) & 15u;
-        site[8 + k] += (cnt_hi >> (4 * k)) & 15u;
-      }
-      cnt_lo = cnt_hi = 0;
+      for (int k = 0; k < 8; ++k) site[k] += (cnt >> (4 * k)) & 15u;
+      cnt = 0;
       since = 0;
     }
   }
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    site[k] += (cnt_lo >> (4 * k)) & 15u;
-    site[8 + k] += (cnt_hi >> (4 * k)) & 15u;
-  }
+  for (int k = 0; k < 8; ++k) site[k] += (cnt >> (4 * k)) & 15u;
   uint64_t tot = 0;
   if (act) {
-    const int64_t s0 = w * 16;
-    if (s0 + 16 <= n_sites) {
+    const int64_t s0 = w * 8;
+    if (s0 + 8 <= n_sites) {
       uint4* dst = reinterpret_cast<uint4*>(site_cost + s0);
       dst[0] = make_uint4(site[0], site[1], site[2], site[3]);
       dst[1] = make_uint4(site[4], site[5], site[6], site[7]);
-      dst[2] = make_uint4(site[8], site[9], site[10], site[11]);
-      dst[3] = make_uint4(site[12], site[13], site[14], site[15]);
       if (wt) {
         const uint4* ws = reinterpret_cast<const uint4*>(wt + s0);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const uint4 x = ws[q];
-          tot += (uint64_t)site[4 * q] * x.x + (uint64_t)site[4 * q + 1] * x.y +
-                 (uint64_t)site[4 * q + 2] * x.z + (uint64_t)site[4 * q + 3] * x.w;
+        for (int q = 0; q < 2; ++q) {
+          const uint4 xw = ws[q];
+          tot += (uint64_t)site[4 * q] * xw.x + (uint64_t)site[4 * q + 1] * xw.y +
+                 (uint64_t)site[4 * q + 2] * xw.z + (uint64_t)site[4 * q + 3] * xw.w;
         }
       } else {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) tot += site[k];
+        for (int k = 0; k < 8; ++k) tot += site[k];
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < 16; ++k)
+      for (int k = 0; k < 8; ++k)
         if (s0 + k < n_sites) {
           site_cost[s0 + k] = site[k];
           tot += (uint64_t)site[k] * (wt ? wt[s0 + k] : 1u);
@@ -302,28 +313,28 @@ __global__ __launch_bounds__(256) void k_fitch_down(
 }
 
 // ---------------------------------------------------------------------------
-// Whole-tree up-pass
+// Whole-tree up-pass (32-bit lanes)
 // ---------------------------------------------------------------------------
 template <int BATCH>
 __global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog, int n_ops,
                                                   const int32_t* __restrict__ fifo_ids, int n_fifo,
-                                                  const uint64_t* __restrict__ sets,
-                                                  uint64_t* __restrict__ fin, int64_t stride,
+                                                  const uint32_t* __restrict__ sets,
+                                                  uint32_t* __restrict__ fin, int64_t stride,
                                                   int64_t W, int N) {
-  extern __shared__ uint64_t lds[];
+  extern __shared__ uint32_t lds32[];
   const int tid = threadIdx.x, bd = blockDim.x;
   const int64_t w = (int64_t)blockIdx.x * bd + tid;
   const bool act = w < W;
   const int64_t wc = act ? w : 0;
-  uint64_t* ring = lds;
-  uint64_t* stk = lds + (size_t)BATCH * bd;  // pairs (P, F)
+  uint32_t* ring = lds32;
+  uint32_t* stk = lds32 + (size_t)BATCH * bd;  // pairs (P, F)
 
-  uint64_t pre[BATCH];
+  uint32_t pre[BATCH];
   auto issue = [&](int start) {
 #pragma unroll
     for (int j = 0; j < BATCH; ++j) {
       const int idx = start + j;
-      pre[j] = (act && idx < n_fifo) ? sets[(int64_t)fifo_ids[idx] * stride + wc] : 0ull;
+      pre[j] = (act && idx < n_fifo) ? sets[(int64_t)fifo_ids[idx] * stride + wc] : 0u;
     }
   };
   issue(0);
@@ -332,7 +343,7 @@ __global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog,
   issue(BATCH);
   int next_batch = 2 * BATCH;
   int fpos = 0;
-  auto next_set = [&]() -> uint64_t {
+  auto next_set = [&]() -> uint32_t {
     if (fpos == BATCH) {
 #pragma unroll
       for (int j = 0; j < BATCH; ++j) ring[j * bd + tid] = pre[j];
@@ -343,13 +354,13 @@ __global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog,
     return ring[(fpos++) * bd + tid];
   };
 
-  uint64_t accP = 0, accF = 0;
+  uint32_t accP = 0, accF = 0;
   int sp = 0;
   for (int i = 0; i < n_ops; ++i) {
     const int4 op = prog[i];
     const int fl = op.w;
     const int src = fl & 3;
-    uint64_t pv, fp = 0;
+    uint32_t pv, fp = 0;
     if (src == SRC_LEAF) pv = next_set();
     else if (src == SRC_ACC) { pv = accP; fp = accF; }
     else {
@@ -357,14 +368,14 @@ __global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog,
       pv = stk[(2 * sp) * bd + tid];
       fp = stk[(2 * sp + 1) * bd + tid];
     }
-    const uint64_t pa = next_set();
-    const uint64_t pb = next_set();
-    const uint64_t fv = (src == SRC_LEAF) ? pv : final64(pv, pa, pb, fp);
+    const uint32_t pa = next_set();
+    const uint32_t pb = next_set();
+    const uint32_t fv = (src == SRC_LEAF) ? pv : final32(pv, pa, pb, fp);
     if (act) fin[(int64_t)op.x * stride + w] = fv;
-    if (fl & UP_A_LEAF) { if (act) fin[(int64_t)op.y * stride + w] = leaf64(fv, pa); }
-    if (fl & UP_B_LEAF) { if (act) fin[(int64_t)op.z * stride + w] = leaf64(fv, pb); }
+    if (fl & UP_A_LEAF) { if (act) fin[(int64_t)op.y * stride + w] = leaf32(fv, pa); }
+    if (fl & UP_B_LEAF) { if (act) fin[(int64_t)op.z * stride + w] = leaf32(fv, pb); }
     if (fl & UP_PUSH) {
-      const uint64_t other = (fl & UP_CONT_B) ? pa : pb;
+      const uint32_t other = (fl & UP_CONT_B) ? pa : pb;
       stk[(2 * sp) * bd + tid] = other;
       stk[(2 * sp + 1) * bd + tid] = fv;
       ++sp;
@@ -489,16 +500,18 @@ int phy_fitch_downpass(phy_ctx* ctx, const phy_topo* topo, uint64_t* sets, int64
   if (node_cost && (rc = phy_zero_u64(ctx, node_cost, N - 1))) return rc;
   if (W == 0) return PHY_OK;
   const int bs = block_size_env("PHY_FITCH_BLOCK", 64);
-  const int64_t g = (W + bs - 1) / bs;
-  const size_t lds = (size_t)(kBatch + topo->down_depth) * bs * sizeof(uint64_t);
-  if (lds > 160 * 1024) return set_err(ctx, PHY_ERANGE, "downpass: LDS stack too deep for this tree");
+  const int64_t W32 = 2 * W;  // 32-bit lanes
+  const int64_t g = (W32 + bs - 1) / bs;
+  const size_t lds = (size_t)(kBatch + topo->down_depth) * bs * sizeof(uint32_t);
+  if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "downpass: LDS stack too deep for this tree");
+  uint32_t* s32 = reinterpret_cast<uint32_t*>(sets);
   if (node_cost)
     hipLaunchKernelGGL((k_fitch_down<kBatch, true>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
-                       topo->d_down_prog, N - 1, topo->d_down_fifo, N, sets, stride, W, n_sites, N,
+                       topo->d_down_prog, N - 1, topo->d_down_fifo, N, s32, 2 * stride, W32, n_sites, N,
                        wt, site_cost, (unsigned long long*)node_cost, (unsigned long long*)total);
   else
     hipLaunchKernelGGL((k_fitch_down<kBatch, false>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
-                       topo->d_down_prog, N - 1, topo->d_down_fifo, N, sets, stride, W, n_sites, N,
+                       topo->d_down_prog, N - 1, topo->d_down_fifo, N, s32, 2 * stride, W32, n_sites, N,
                        wt, site_cost, (unsigned long long*)nullptr, (unsigned long long*)total);
   PHY_LAUNCHED(ctx);
   return PHY_OK;
@@ -514,11 +527,14 @@ int phy_fitch_uppass(phy_ctx* ctx, const phy_topo* topo, const uint64_t* sets, u
   if (W == 0) return PHY_OK;
   const int N = topo->n_leaves;
   const int bs = block_size_env("PHY_FITCH_BLOCK", 64);
-  const int64_t g = (W + bs - 1) / bs;
-  const size_t lds = (size_t)(kBatch + 2 * topo->up_depth) * bs * sizeof(uint64_t);
-  if (lds > 160 * 1024) return set_err(ctx, PHY_ERANGE, "uppass: LDS stack too deep for this tree");
+  const int64_t W32 = 2 * W;
+  const int64_t g = (W32 + bs - 1) / bs;
+  const size_t lds = (size_t)(kBatch + 2 * topo->up_depth) * bs * sizeof(uint32_t);
+  if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "uppass: LDS stack too deep for this tree");
   hipLaunchKernelGGL((k_fitch_up<kBatch>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
-                     topo->d_up_prog, N - 1, topo->d_up_fifo, 2 * N - 1, sets, fin, stride, W, N);
+                     topo->d_up_prog, N - 1, topo->d_up_fifo, 2 * N - 1,
+                     reinterpret_cast<const uint32_t*>(sets), reinterpret_cast<uint32_t*>(fin),
+                     2 * stride, W32, N);
   PHY_LAUNCHED(ctx);
   return PHY_OK;
 }

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <thread>
+#include <type_traits>
 
 #include "common.hpp"
 #include "model_host.hpp"
@@ -149,10 +150,38 @@ __device__ __forceinline__ void rescale(real* v, int* e) {
   }
 }
 
+// Cross-lane data movement through DPP (no LDS round trip).  Controls:
+// quad_perm(1,0,3,2) = lane ^ 1, quad_perm(2,3,0,1) = lane ^ 2,
+// row_half_mirror = lane ^ 7 within 8, row_mirror = lane ^ 15 within 16.
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140;
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL,
+                                                               0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double readlane(double x, int l) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Sum over the 64 lanes, the same fixed order for every wave; the result is
+// wave-uniform.
 __device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp<DPP_XOR1>(v);
+  v += dpp<DPP_XOR2>(v);
+  v += dpp<DPP_HALF_MIRROR>(v);
+  v += dpp<DPP_MIRROR>(v);
+  return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
 }
 
 // Small ring that prefetches the tip codes of the post-order leaf stream.
@@ -211,16 +240,20 @@ constexpr int kTipBatch = 8;
 // p*R + r: part[((v - N) * 4 + i) * (pitch * R) + p * R + r], exponents
 // ex[(v - N) * pitch + p].
 // ---------------------------------------------------------------------------
+// max / sum over the R lanes of one pattern (lanes p*R .. p*R+R-1); every lane
+// of the group gets the same bits (a + b == b + a exactly)
 template <typename real, int R>
 __device__ __forceinline__ real rate_max(real x) {
-#pragma unroll
-  for (int o = 1; o < R; o <<= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  if (R >= 2) x = fmax(x, dpp<DPP_XOR1>(x));
+  if (R >= 4) x = fmax(x, dpp<DPP_XOR2>(x));
+  if (R >= 8) x = fmax(x, dpp<DPP_HALF_MIRROR>(x));
   return x;
 }
 template <int R>
 __device__ __forceinline__ double rate_sum(double x) {
-#pragma unroll
-  for (int o = 1; o < R; o <<= 1) x += __shfl_xor(x, o, 64);
+  if (R >= 2) x += dpp<DPP_XOR1>(x);
+  if (R >= 4) x += dpp<DPP_XOR2>(x);
+  if (R >= 8) x += dpp<DPP_HALF_MIRROR>(x);
   return x;
 }
 
@@ -250,6 +283,21 @@ __device__ __forceinline__ real stage_elem(const int4 op, int j, const real* __r
   return base[(size_t)child * R * 16 + rem];
 }
 
+// compile-time loop: f(integral_constant<int, I>) for I in [B, E)
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// Transition matrices are staged through registers into an LDS double buffer
+// kPD nodes ahead: slot u of the register pipeline holds the node with index
+// = u (mod kPD); a load issued at node i is written to LDS at node i+kPD-1,
+// so kPD-1 node computations cover its latency.
+constexpr int kPD = 8;
+
 template <typename real, int R, bool FUSED>
 __device__ __forceinline__ void prune_down_body(
     const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids, int n_leaf_ids,
@@ -272,7 +320,21 @@ __device__ __forceinline__ void prune_down_body(
   uint32_t* ring = reinterpret_cast<uint32_t*>(Pbuf + 2 * NM);
   real* stk = reinterpret_cast<real*>(ring + (size_t)kTipBatch * bd);
   int32_t* stke = reinterpret_cast<int32_t*>(stk + (size_t)stack_depth * 4 * bd);
+
+  real pf[kPD][PF];
+  auto load_slot = [&](real* dst, int node) {
+    const int4 on = prog[node];
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+      const int j = tid + k * bd;
+      if (j < NM) dst[k] = stage_elem<real, R>(on, j, Pm, Pm, Pm);
+    }
+  };
   for (int j = tid; j < NM; j += bd) Pbuf[j] = stage_elem<real, R>(prog[0], j, Pm, Pm, Pm);
+  static_for<1, kPD>([&](auto U) {
+    constexpr int u = decltype(U)::value;
+    if (u < n_ops) load_slot(pf[u], u);
+  });
   TipStream<kTipBatch> ts;
   ts.ring = ring;
   ts.ids = leaf_ids;
@@ -287,75 +349,69 @@ __device__ __forceinline__ void prune_down_body(
 
   real acc[4] = {(real)0, (real)0, (real)0, (real)0};
   int acc_e = 0, sp = 0;
-  for (int it = 0; it < n_ops; ++it) {
-    const int4 op = prog[it];
-    const bool more = it + 1 < n_ops;
-    real nxt[PF];
-    if (more) {
-      const int4 on = prog[it + 1];
+  for (int base = 0; base < n_ops; base += kPD) {
+    static_for<0, kPD>([&](auto U) {
+      constexpr int u = decltype(U)::value;
+      const int it = base + u;
+      if (it >= n_ops) return;
+      const int4 op = prog[it];
+      if (it + kPD < n_ops) load_slot(pf[u], it + kPD);
+      const real* Pc = Pbuf + (u & 1) * NM;
+      const real* PA = Pc + r * 16;
+      const real* PB = Pc + (R + r) * 16;
+      const int fl = op.w;
+      // the LDS stack holds spilled partials in both forms; the stored form
+      // additionally writes every partial to HBM but never reads it back
+      if (fl & DOWN_SPILL) {
 #pragma unroll
-      for (int k = 0; k < PF; ++k) {
-        const int j = tid + k * bd;
-        if (j < NM) nxt[k] = stage_elem<real, R>(on, j, Pm, Pm, Pm);
+        for (int k = 0; k < 4; ++k) stk[((size_t)sp * 4 + k) * bd + tid] = acc[k];
+        stke[(size_t)sp * bd + tid] = acc_e;
+        ++sp;
       }
-    }
-    const real* Pc = Pbuf + (it & 1) * NM;
-    const real* PA = Pc + r * 16;
-    const real* PB = Pc + (R + r) * 16;
-    const int fl = op.w;
-    if (FUSED && (fl & DOWN_SPILL)) {
+      const int sa = fl & 3, sb = (fl >> 2) & 3;
+      real xa[4], xb[4], src[4];
+      int ea = 0, eb = 0;
+      if (sa == SRC_LEAF) {
+        tipvec(ts.get(), src);
+      } else if (sa == SRC_ACC) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) stk[((size_t)sp * 4 + k) * bd + tid] = acc[k];
-      stke[(size_t)sp * bd + tid] = acc_e;
-      ++sp;
-    }
-    const int sa = fl & 3, sb = (fl >> 2) & 3;
-    real xa[4], xb[4], src[4];
-    int ea = 0, eb = 0;
-    if (sa == SRC_LEAF) {
-      tipvec(ts.get(), src);
-    } else if (sa == SRC_ACC) {
+        for (int k = 0; k < 4; ++k) src[k] = acc[k];
+        ea = acc_e;
+      } else {
+        --sp;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) src[k] = acc[k];
-      ea = acc_e;
-    } else if (FUSED) {
-      --sp;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) src[k] = stk[((size_t)sp * 4 + k) * bd + tid];
-      ea = stke[(size_t)sp * bd + tid];
-    } else {
-      const real* s = part + (size_t)(op.y - N) * 4 * rowlen + (pc * R + r);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) src[k] = s[(size_t)k * rowlen];
-      ea = ex[(size_t)(op.y - N) * pitch + pc];
-    }
-    matvec4(PA, src, xa);
-    if (sb == SRC_LEAF) {
-      tipvec(ts.get(), src);
-    } else {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) src[k] = acc[k];
-      eb = acc_e;
-    }
-    matvec4(PB, src, xb);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) acc[k] = xa[k] * xb[k];
-    acc_e = ea + eb;
-    rescale_pattern<real, R>(acc, &acc_e);
-    if (!FUSED && act) {
-      real* d = part + (size_t)(op.x - N) * 4 * rowlen + col;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) d[(size_t)k * rowlen] = acc[k];
-      if (r == 0) ex[(size_t)(op.x - N) * pitch + p] = acc_e;
-    }
-    if (more) {
-#pragma unroll
-      for (int k = 0; k < PF; ++k) {
-        const int j = tid + k * bd;
-        if (j < NM) Pbuf[((it + 1) & 1) * NM + j] = nxt[k];
+        for (int k = 0; k < 4; ++k) src[k] = stk[((size_t)sp * 4 + k) * bd + tid];
+        ea = stke[(size_t)sp * bd + tid];
       }
-    }
-    __syncthreads();
+      matvec4(PA, src, xa);
+      if (sb == SRC_LEAF) {
+        tipvec(ts.get(), src);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) src[k] = acc[k];
+        eb = acc_e;
+      }
+      matvec4(PB, src, xb);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc[k] = xa[k] * xb[k];
+      acc_e = ea + eb;
+      rescale_pattern<real, R>(acc, &acc_e);
+      if (!FUSED && act) {
+        real* d = part + (size_t)(op.x - N) * 4 * rowlen + col;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) d[(size_t)k * rowlen] = acc[k];
+        if (r == 0) ex[(size_t)(op.x - N) * pitch + p] = acc_e;
+      }
+      if (it + 1 < n_ops) {
+        constexpr int nu = (u + 1) % kPD;
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+          const int j = tid + k * bd;
+          if (j < NM) Pbuf[((u + 1) & 1) * NM + j] = pf[nu][k];
+        }
+      }
+      __syncthreads();
+    });
   }
   // root: ln L_p = ln sum_r prob_r sum_i pi_i L[r][i] + e ln 2
   double s = 0.0;
@@ -412,6 +468,8 @@ __global__ __launch_bounds__(256) void k_prune_down_batch(
 // Same thread mapping as the post-order pass; P, P', P'' of both children
 // are staged in LDS one node ahead.
 // ---------------------------------------------------------------------------
+constexpr int kPDD = 4;  // prefetch distance (nodes) of the derivative pass
+
 template <typename real, int R>
 __global__ __launch_bounds__(256) void k_prune_deriv(
     const int4* __restrict__ prog, int n_ops, const uint64_t* __restrict__ tips, int64_t stride,
@@ -420,7 +478,7 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
     const real* __restrict__ part, const uint32_t* __restrict__ weights,
     double* __restrict__ wave_sums, int N) {
   constexpr int NM = 6 * R * 16;
-  constexpr int PF = (NM + 63) / 64;
+  constexpr int PF = (NM + 127) / 128;  // blocks of >= 128 threads
   extern __shared__ unsigned char lds_raw[];
   real* Pbuf = reinterpret_cast<real*>(lds_raw);     // [2][NM]
   real* stk = Pbuf + 2 * NM;                          // [depth][4][bd]
@@ -434,9 +492,17 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
   const int M = 2 * N - 1;
   const int64_t wave = ((int64_t)blockIdx.x * bd + tid) >> 6;
   const int lane = tid & 63;
-  for (int j = tid; j < NM; j += bd) Pbuf[j] = stage_elem<real, R>(prog[0], j, Pm, P1m, P2m);
-  __syncthreads();
 
+  real pf[kPDD][PF];
+  auto load_slot = [&](real* dst, int node) {
+    const int4 on = prog[node];
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+      const int j = tid + k * bd;
+      if (j < NM) dst[k] = stage_elem<real, R>(on, j, Pm, P1m, P2m);
+    }
+  };
+  // children partials (4 values each) of an entry
   auto load_D = [&](int c, real* Dv) {
     if (c < N) {
       tipvec(tip_code(tips, stride, c, pc), Dv);
@@ -446,123 +512,138 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
       for (int k = 0; k < 4; ++k) Dv[k] = s[(size_t)k * rowlen];
     }
   };
+  for (int j = tid; j < NM; j += bd) Pbuf[j] = stage_elem<real, R>(prog[0], j, Pm, P1m, P2m);
+  static_for<1, kPDD>([&](auto U) {
+    constexpr int u = decltype(U)::value;
+    if (u < n_ops) load_slot(pf[u], u);
+  });
+  real Dn[2][8];  // children partials of the current / next entry
+  {
+    const int4 o = prog[0];
+    load_D(o.y, Dn[0]);
+    load_D(o.z, Dn[0] + 4);
+  }
+  __syncthreads();
 
   real accM[4] = {(real)0, (real)0, (real)0, (real)0};
   int sp = 0;
-  for (int it = 0; it < n_ops; ++it) {
-    const int4 op = prog[it];
-    const bool more = it + 1 < n_ops;
-    real nxt[PF];
-    if (more) {
-      const int4 on = prog[it + 1];
-#pragma unroll
-      for (int k = 0; k < PF; ++k) {
-        const int j = tid + k * bd;
-        if (j < NM) nxt[k] = stage_elem<real, R>(on, j, Pm, P1m, P2m);
+  for (int base = 0; base < n_ops; base += kPDD) {
+    static_for<0, kPDD>([&](auto U) {
+      constexpr int u = decltype(U)::value;
+      constexpr int cur = u & 1, nxt = (u + 1) & 1;
+      const int it = base + u;
+      if (it >= n_ops) return;
+      const int4 op = prog[it];
+      if (it + kPDD < n_ops) load_slot(pf[u], it + kPDD);
+      if (it + 1 < n_ops) {
+        const int4 on = prog[it + 1];
+        load_D(on.y, Dn[nxt]);
+        load_D(on.z, Dn[nxt] + 4);
       }
-    }
-    const real* Pc = Pbuf + (it & 1) * NM;
-    // matrix m = which*2 + child: P_a, P_b, P1_a, P1_b, P2_a, P2_b
-    const real* Pa = Pc + (0 * R + r) * 16;
-    const real* Pb = Pc + (1 * R + r) * 16;
-    const real* P1a = Pc + (2 * R + r) * 16;
-    const real* P1b = Pc + (3 * R + r) * 16;
-    const real* P2a = Pc + (4 * R + r) * 16;
-    const real* P2b = Pc + (5 * R + r) * 16;
-    const int fl = op.w, src = fl & 3;
-    const int a = op.y, b = op.z;
-    real Mv[4];
-    if (src == SRC_LEAF) {
+      const real* Pc = Pbuf + (u & 1) * NM;
+      // matrices (which, child): P_a, P_b, P'_a, P'_b, P''_a, P''_b
+      const real* Pa = Pc + (0 * R + r) * 16;
+      const real* Pb = Pc + (1 * R + r) * 16;
+      const real* P1a = Pc + (2 * R + r) * 16;
+      const real* P1b = Pc + (3 * R + r) * 16;
+      const real* P2a = Pc + (4 * R + r) * 16;
+      const real* P2b = Pc + (5 * R + r) * 16;
+      const int fl = op.w, src = fl & 3;
+      const int a = op.y, b = op.z;
+      real Mv[4];
+      if (src == SRC_LEAF) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) Mv[i] = (real)pi[i];
-    } else if (src == SRC_ACC) {
+        for (int i = 0; i < 4; ++i) Mv[i] = (real)pi[i];
+      } else if (src == SRC_ACC) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) Mv[k] = accM[k];
-    } else {
-      --sp;
+        for (int k = 0; k < 4; ++k) Mv[k] = accM[k];
+      } else {
+        --sp;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) Mv[k] = stk[((size_t)sp * 4 + k) * bd + tid];
-    }
-    real Da[4], Db[4], ya[4], yb[4], x1a[4], x2a[4], x1b[4], x2b[4];
-    load_D(a, Da);
-    load_D(b, Db);
-    matvec4(Pa, Da, ya);
-    matvec4(Pb, Db, yb);
-    matvec4(P1a, Da, x1a);
-    matvec4(P2a, Da, x2a);
-    matvec4(P1b, Db, x1b);
-    matvec4(P2b, Db, x2b);
-    real s0 = 0, s1a = 0, s2a = 0, s1b = 0, s2b = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const real ma = Mv[i] * yb[i];  // A_a
-      const real mb = Mv[i] * ya[i];  // A_b
-      s0 = fma(ma, ya[i], s0);
-      s1a = fma(ma, x1a[i], s1a);
-      s2a = fma(ma, x2a[i], s2a);
-      s1b = fma(mb, x1b[i], s1b);
-      s2b = fma(mb, x2b[i], s2b);
-    }
-    const double pr = probs[r];
-    const double l0 = rate_sum<R>(pr * (double)s0);
-    const double l1a = rate_sum<R>(pr * (double)s1a), l2a = rate_sum<R>(pr * (double)s2a);
-    const double l1b = rate_sum<R>(pr * (double)s1b), l2b = rate_sum<R>(pr * (double)s2b);
-    double ga = 0, ha = 0, gb = 0, hb = 0;
-    if (act && r == 0) {
-      const double inv = 1.0 / l0;
-      const double g1 = l1a * inv, g2 = l1b * inv;
-      ga = wp * g1;
-      ha = wp * (l2a * inv - g1 * g1);
-      gb = wp * g2;
-      hb = wp * (l2b * inv - g2 * g2);
-    }
-    ga = wave_sum_d(ga);
-    ha = wave_sum_d(ha);
-    gb = wave_sum_d(gb);
-    hb = wave_sum_d(hb);
-    if (lane == 0) {
-      double* ws = wave_sums + (size_t)wave * M * 2;
-      ws[2 * a] = ga;
-      ws[2 * a + 1] = ha;
-      ws[2 * b] = gb;
-      ws[2 * b + 1] = hb;
-    }
-    // messages to internal children: M_c = (M (*) P_s D_s)^T P_c
-    real Ma[4], Mb[4];
-    int dummy = 0;
-    if (!(fl & UP_A_LEAF)) {
-      real Av[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) Av[i] = Mv[i] * yb[i];
-      vecmat4(Pa, Av, Ma);
-      rescale_pattern<real, R>(Ma, &dummy);
-    }
-    if (!(fl & UP_B_LEAF)) {
-      real Av[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) Av[i] = Mv[i] * ya[i];
-      vecmat4(Pb, Av, Mb);
-      rescale_pattern<real, R>(Mb, &dummy);
-    }
-    if (fl & UP_PUSH) {
-      const real* o = (fl & UP_CONT_B) ? Ma : Mb;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) stk[((size_t)sp * 4 + k) * bd + tid] = o[k];
-      ++sp;
-    }
-    if (!(fl & UP_A_LEAF) || !(fl & UP_B_LEAF)) {
-      const real* c = (fl & UP_CONT_B) ? Mb : Ma;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) accM[k] = c[k];
-    }
-    if (more) {
-#pragma unroll
-      for (int k = 0; k < PF; ++k) {
-        const int j = tid + k * bd;
-        if (j < NM) Pbuf[((it + 1) & 1) * NM + j] = nxt[k];
+        for (int k = 0; k < 4; ++k) Mv[k] = stk[((size_t)sp * 4 + k) * bd + tid];
       }
-    }
-    __syncthreads();
+      const real* Da = Dn[cur];
+      const real* Db = Dn[cur] + 4;
+      real ya[4], yb[4], x1a[4], x2a[4], x1b[4], x2b[4];
+      matvec4(Pa, Da, ya);
+      matvec4(Pb, Db, yb);
+      matvec4(P1a, Da, x1a);
+      matvec4(P2a, Da, x2a);
+      matvec4(P1b, Db, x1b);
+      matvec4(P2b, Db, x2b);
+      real s0 = 0, s1a = 0, s2a = 0, s1b = 0, s2b = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const real ma = Mv[i] * yb[i];  // A_a
+        const real mb = Mv[i] * ya[i];  // A_b
+        s0 = fma(ma, ya[i], s0);
+        s1a = fma(ma, x1a[i], s1a);
+        s2a = fma(ma, x2a[i], s2a);
+        s1b = fma(mb, x1b[i], s1b);
+        s2b = fma(mb, x2b[i], s2b);
+      }
+      const double pr = probs[r];
+      const double l0 = rate_sum<R>(pr * (double)s0);
+      const double l1a = rate_sum<R>(pr * (double)s1a), l2a = rate_sum<R>(pr * (double)s2a);
+      const double l1b = rate_sum<R>(pr * (double)s1b), l2b = rate_sum<R>(pr * (double)s2b);
+      double ga = 0, ha = 0, gb = 0, hb = 0;
+      if (act && r == 0) {
+        const double inv = 1.0 / l0;
+        const double g1 = l1a * inv, g2 = l1b * inv;
+        ga = wp * g1;
+        ha = wp * (l2a * inv - g1 * g1);
+        gb = wp * g2;
+        hb = wp * (l2b * inv - g2 * g2);
+      }
+      ga = wave_sum_d(ga);
+      ha = wave_sum_d(ha);
+      gb = wave_sum_d(gb);
+      hb = wave_sum_d(hb);
+      if (lane == 0) {
+        double* ws = wave_sums + (size_t)wave * M * 2;
+        ws[2 * a] = ga;
+        ws[2 * a + 1] = ha;
+        ws[2 * b] = gb;
+        ws[2 * b + 1] = hb;
+      }
+      // messages to internal children: M_c = (M (*) P_s D_s)^T P_c
+      real Ma[4], Mb[4];
+      int dummy = 0;
+      if (!(fl & UP_A_LEAF)) {
+        real Av[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Av[i] = Mv[i] * yb[i];
+        vecmat4(Pa, Av, Ma);
+        rescale_pattern<real, R>(Ma, &dummy);
+      }
+      if (!(fl & UP_B_LEAF)) {
+        real Av[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Av[i] = Mv[i] * ya[i];
+        vecmat4(Pb, Av, Mb);
+        rescale_pattern<real, R>(Mb, &dummy);
+      }
+      if (fl & UP_PUSH) {
+        const real* o = (fl & UP_CONT_B) ? Ma : Mb;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) stk[((size_t)sp * 4 + k) * bd + tid] = o[k];
+        ++sp;
+      }
+      if (!(fl & UP_A_LEAF) || !(fl & UP_B_LEAF)) {
+        const real* c = (fl & UP_CONT_B) ? Mb : Ma;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) accM[k] = c[k];
+      }
+      if (it + 1 < n_ops) {
+        constexpr int nu = (u + 1) % kPDD;
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+          const int j = tid + k * bd;
+          if (j < NM) Pbuf[((u + 1) & 1) * NM + j] = pf[nu][k];
+        }
+      }
+      __syncthreads();
+    });
   }
 }
 
@@ -752,7 +833,7 @@ constexpr size_t kLdsMax = 64 * 1024;
 
 size_t down_lds(int R, size_t esz, int bs, bool fused, int depth) {
   return (size_t)2 * 2 * R * 16 * esz + (size_t)kTipBatch * bs * 4 +
-         (fused ? (size_t)std::max(depth, 1) * bs * (4 * esz + 4) : 0);
+         (size_t)std::max(depth, 1) * bs * (4 * esz + 4);
 }
 size_t deriv_lds(int R, size_t esz, int bs, int depth) {
   return (size_t)2 * 6 * R * 16 * esz + (size_t)std::max(depth, 1) * bs * 4 * esz;
@@ -760,8 +841,8 @@ size_t deriv_lds(int R, size_t esz, int bs, int depth) {
 
 // largest block size <= want whose LDS fits (0 if none)
 template <class F>
-int fit_block(int want, F lds_of) {
-  for (int bs = want; bs >= 64; bs /= 2)
+int fit_block(int want, F lds_of, int min_bs = 64) {
+  for (int bs = want; bs >= min_bs; bs /= 2)
     if (lds_of(bs) <= kLdsMax) return bs;
   return 0;
 }
@@ -777,8 +858,8 @@ Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t for
                     [&](int bs) { return down_lds(R, esz, bs, fused, depth); });
   pl.lds = pl.bs ? down_lds(R, esz, pl.bs, fused, depth) : 0;
   if (derivs) {
-    pl.dbs = fit_block(env_block("PHY_PRUNE_DERIV_BLOCK", 256),
-                       [&](int bs) { return deriv_lds(R, esz, bs, t->up_depth); });
+    pl.dbs = fit_block(std::max(128, env_block("PHY_PRUNE_DERIV_BLOCK", 256)),
+                       [&](int bs) { return deriv_lds(R, esz, bs, t->up_depth); }, 128);
     pl.dlds = pl.dbs ? deriv_lds(R, esz, pl.dbs, t->up_depth) : 0;
   }
   pl.pitch = ((n_pat + 255) / 256) * 256;
@@ -835,7 +916,7 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
   else
     hipLaunchKernelGGL((k_prune_down<real, R, false>), dim3((unsigned)pl.n_blocks), dim3(pl.bs), pl.lds,
                        ctx->stream, t->d_down_prog, N - 1, t->d_down_fifo, N, tips, stride, n_pat,
-                       pl.pitch, P, d_pi, d_probs, part, ex, weights, site_lnl, bsum, N, 0);
+                       pl.pitch, P, d_pi, d_probs, part, ex, weights, site_lnl, bsum, N, t->down_depth);
   PHY_LAUNCHED(ctx);
   hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, ctx->stream, bsum, pl.n_waves, (int64_t)1, lnl);
   PHY_LAUNCHED(ctx);
