@@ -53,10 +53,12 @@ enum {
 
 /* Likelihood evaluation forms (bit flags). */
 enum {
-  PHY_F64 = 0,      /* FP64 partials, per-pattern int32 exponents            */
-  PHY_F32 = 1,      /* FP32 partials, per-pattern int32 exponents            */
+  PHY_F64 = 0,      /* FP64 partials, int32 exponent per (pattern, rate)     */
+  PHY_F32 = 1,      /* FP32 partials, int32 exponent per (pattern, rate)     */
   PHY_FUSED = 2     /* keep interior partials on chip (lnL only, no stored
-                       partials, no derivatives)                              */
+                       partials, no derivatives); a tree whose stack does
+                       not fit in LDS runs the stored form instead, with
+                       the same result                                       */
 };
 
 /* Gamma rate-category modes. */
@@ -269,7 +271,8 @@ int phy_lk_tree_work_size(const phy_topo* topo, const phy_model* model,
  *   d_blen     2N-1 branch lengths (FP64); d_blen[v] is the branch above v,
  *              d_blen[root] is ignored.
  *   d_lnl      1 double: sum_p w_p (ln L_p), L_p = sum_r prob_r sum_i pi_i
- *              L_root[p][r][i] with the pattern's power-of-two exponents.
+ *              L_root[r][i][p] 2^e[r][p] (each rate class of a pattern keeps
+ *              its own power-of-two exponent).
  *   d_site_lnl (nullable) n_patterns doubles: ln L_p.
  *   d_d1,d_d2  (nullable, not with PHY_FUSED) 2N-1 doubles: dlnL/dt_v and
  *              d2lnL/dt_v^2 for the branch above every non-root node v.  The
@@ -286,9 +289,10 @@ int phy_lk_tree(phy_ctx* ctx, const phy_topo* topo, const phy_model* model,
 
 /* Pointers into d_work of the stored partials and exponents of internal node
  * v (N <= v <= 2N-2) after phy_lk_tree without PHY_FUSED: the partial is
- * K rows of pitch*R values (FP32 or FP64 by form), row i holding
- * L[r][i][p] at column p*R + r (pitch = *pitch, the padded pattern count);
- * exponents e[p] (int32). */
+ * R*K rows of pitch values (FP32 or FP64 by form), row r*K + i holding
+ * L[r][i][p] at column p (pitch = *pitch, the padded pattern count); the
+ * exponents are R rows of pitch int32, e[r][p], so the value is
+ * L[r][i][p] * 2^e[r][p]. */
 int phy_lk_tree_partials(const phy_topo* topo, const phy_model* model,
                          int64_t n_patterns, int32_t form, int32_t want_derivs,
                          void* d_work, int32_t v, void** d_partial,

@@ -227,19 +227,178 @@ constexpr int kTipBatch = 8;
 // ---------------------------------------------------------------------------
 // Whole-tree post-order pass.
 //
-// One thread per (pattern, rate): lanes p*R .. p*R+R-1 hold the R rate classes
-// of pattern p (4 states each), so a wave carries 64/R patterns and the grid
-// has R times more waves than patterns/64 -- enough to hide memory latency.
-// The transition matrices of the two children of the node being computed are
-// staged in an LDS double buffer by the block (prefetched one node ahead),
-// since every wave of the block walks the same program.  Per-pattern
-// quantities (rescaling maximum, root sum) are reduced across the R lanes
-// with xor shuffles.
-//
-// Stored partial of internal node v: row i (state) of 4 rows, column
-// p*R + r: part[((v - N) * 4 + i) * (pitch * R) + p * R + r], exponents
-// ex[(v - N) * pitch + p].
+// A block is R waves over the same 64 patterns; wave r evaluates rate class r
+// (lane = pattern).  Each (pattern, rate) carries its own power-of-two
+// exponent, so rate classes never interact before the root: the transition
+// matrices of a wave are wave-uniform (scalar loads into SGPRs), waves run
+// without barriers, and only the root combines the R classes through LDS:
+//   L_p = sum_r prob_r S_r 2^(e_r - m) * 2^m,  m = max_r e_r  (exact scaling).
+// Stored partial of internal node v: part[(((v - N) * R + r) * 4 + i) * pitch + p]
+// (pattern-contiguous rows), exponents ex[((v - N) * R + r) * pitch + p].
 // ---------------------------------------------------------------------------
+template <typename real, int R>
+__device__ __forceinline__ void load_mat(const real* __restrict__ P, real* m) {
+#pragma unroll
+  for (int k = 0; k < 16; ++k) m[k] = P[k];
+}
+
+// exact power-of-two rescale of one lane's 4 values
+template <typename real>
+__device__ __forceinline__ void rescale4(real* v, int* e) {
+  const real mx = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+  if (mx < Thr<real>::lo && mx > (real)0) {
+    int ex;
+    (void)frexp(mx, &ex);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = ldexp(v[k], -ex);
+    *e += ex;
+  }
+}
+
+template <typename real, int R, bool FUSED>
+__device__ __forceinline__ void prune_down_body(
+    const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids, int n_leaf_ids,
+    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
+    const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
+    real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
+    double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth) {
+  extern __shared__ unsigned char lds_raw[];
+  const int tid = threadIdx.x, bd = blockDim.x, lane = tid & 63;
+  const int r = __builtin_amdgcn_readfirstlane(tid >> 6);  // rate class of this wave
+  const int64_t p = (int64_t)blockIdx.x * 64 + lane;
+  const bool act = p < n_pat;
+  const int64_t pc = act ? p : 0;
+
+  // LDS: root exchange [R][64] (double S, int e) | tip ring [kTipBatch][bd] |
+  //      (fused) stack [depth][4][bd] real + [depth][bd] int
+  double* rootS = reinterpret_cast<double*>(lds_raw);
+  int* rootE = reinterpret_cast<int*>(rootS + R * 64);
+  uint32_t* ring = reinterpret_cast<uint32_t*>(rootE + R * 64);
+  real* stk = reinterpret_cast<real*>(ring + (size_t)kTipBatch * bd);
+  int32_t* stke = reinterpret_cast<int32_t*>(stk + (size_t)(FUSED ? stack_depth : 0) * 4 * bd);
+
+  TipStream<kTipBatch> ts;
+  ts.ring = ring;
+  ts.ids = leaf_ids;
+  ts.n_ids = n_leaf_ids;
+  ts.tips = tips;
+  ts.stride = stride;
+  ts.pc = pc;
+  ts.bd = bd;
+  ts.tid = tid;
+  ts.init();
+
+  real acc[4] = {(real)0, (real)0, (real)0, (real)0};
+  int acc_e = 0, sp = 0;
+  for (int it = 0; it < n_ops; ++it) {
+    const int4 op = prog[it];
+    real PA[16], PB[16];
+    load_mat<real, R>(Pm + ((size_t)op.y * R + r) * 16, PA);
+    load_mat<real, R>(Pm + ((size_t)op.z * R + r) * 16, PB);
+    const int fl = op.w;
+    if (FUSED && (fl & DOWN_SPILL)) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) stk[((size_t)sp * 4 + k) * bd + tid] = acc[k];
+      stke[(size_t)sp * bd + tid] = acc_e;
+      ++sp;
+    }
+    const int sa = fl & 3, sb = (fl >> 2) & 3;
+    real xa[4], xb[4], src[4];
+    int ea = 0, eb = 0;
+    if (sa == SRC_LEAF) {
+      tipvec(ts.get(), src);
+    } else if (sa == SRC_ACC) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) src[k] = acc[k];
+      ea = acc_e;
+    } else if (FUSED) {
+      --sp;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) src[k] = stk[((size_t)sp * 4 + k) * bd + tid];
+      ea = stke[(size_t)sp * bd + tid];
+    } else {
+      const real* sv = part + (((size_t)(op.y - N) * R + r) * 4) * pitch + pc;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) src[k] = sv[(size_t)k * pitch];
+      ea = ex[((size_t)(op.y - N) * R + r) * pitch + pc];
+    }
+    matvec4(PA, src, xa);
+    if (sb == SRC_LEAF) {
+      tipvec(ts.get(), src);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) src[k] = acc[k];
+      eb = acc_e;
+    }
+    matvec4(PB, src, xb);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[k] = xa[k] * xb[k];
+    acc_e = ea + eb;
+    rescale4(acc, &acc_e);
+    if (!FUSED && act) {
+      real* d = part + (((size_t)(op.x - N) * R + r) * 4) * pitch + p;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) d[(size_t)k * pitch] = acc[k];
+      ex[((size_t)(op.x - N) * R + r) * pitch + p] = acc_e;
+    }
+  }
+  // root: S_r = sum_i pi_i L[r][i]; combine the R classes in wave 0
+  double S = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) S += pi[i] * (double)acc[i];
+  rootS[r * 64 + lane] = S;
+  rootE[r * 64 + lane] = acc_e;
+  __syncthreads();
+  if (r == 0) {
+    int m = rootE[lane];
+#pragma unroll
+    for (int q = 1; q < R; ++q) m = max(m, rootE[q * 64 + lane]);
+    double L = 0.0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) L += probs[q] * ldexp(rootS[q * 64 + lane], rootE[q * 64 + lane] - m);
+    const double slnl = log(L) + (double)m * 0.6931471805599453;
+    double contrib = 0.0;
+    if (act) {
+      if (site_lnl) site_lnl[p] = slnl;
+      contrib = (double)weights[p] * slnl;
+    }
+    // one partial sum per 64 patterns, independent of the launch shape
+    const double ws = wave_sum_d(contrib);
+    if (lane == 0) wave_sums[blockIdx.x] = ws;
+  }
+}
+
+template <typename real, int R, bool FUSED>
+__global__ __launch_bounds__(512) void k_prune_down(
+    const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids, int n_leaf_ids,
+    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
+    const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
+    real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
+    double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth) {
+  prune_down_body<real, R, FUSED>(prog, n_ops, leaf_ids, n_leaf_ids, tips, stride, n_pat, pitch, Pm,
+                                  pi, probs, part, ex, weights, site_lnl, wave_sums, N,
+                                  stack_depth);
+}
+
+// Many candidate topologies over the same tips and weights (fused form):
+// blockIdx.y selects candidate c, whose program, leaf order, P matrices and
+// per-wave sums sit at fixed strides.
+template <typename real, int R>
+__global__ __launch_bounds__(512) void k_prune_down_batch(
+    const int4* __restrict__ prog, const int32_t* __restrict__ leaf_ids, int N,
+    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
+    const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
+    const uint32_t* __restrict__ weights, double* __restrict__ wave_sums, int64_t n_waves,
+    int stack_depth) {
+  const int64_t c = blockIdx.y;
+  const int M = 2 * N - 1;
+  prune_down_body<real, R, true>(prog + c * (N - 1), N - 1, leaf_ids + c * N, N, tips, stride,
+                                 n_pat, pitch, Pm + c * (int64_t)M * R * 16, pi, probs, nullptr,
+                                 nullptr, weights, nullptr, wave_sums + c * n_waves, N,
+                                 stack_depth);
+}
+
+// Helpers of the derivative pass, whose threads are (pattern, rate) quads.
 // max / sum over the R lanes of one pattern (lanes p*R .. p*R+R-1); every lane
 // of the group gets the same bits (a + b == b + a exactly)
 template <typename real, int R>
@@ -292,174 +451,6 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
-// Transition matrices are staged through registers into an LDS double buffer
-// kPD nodes ahead: slot u of the register pipeline holds the node with index
-// = u (mod kPD); a load issued at node i is written to LDS at node i+kPD-1,
-// so kPD-1 node computations cover its latency.
-constexpr int kPD = 8;
-
-template <typename real, int R, bool FUSED>
-__device__ __forceinline__ void prune_down_body(
-    const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids, int n_leaf_ids,
-    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
-    const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
-    real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
-    double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth) {
-  constexpr int NM = 2 * R * 16;
-  constexpr int PF = (NM + 63) / 64;  // prefetch registers per thread (block >= 64)
-  extern __shared__ unsigned char lds_raw[];
-  const int tid = threadIdx.x, bd = blockDim.x;
-  const int r = tid % R;
-  const int64_t p = (int64_t)blockIdx.x * (bd / R) + tid / R;
-  const bool act = p < n_pat;
-  const int64_t pc = act ? p : 0;
-  const int64_t rowlen = pitch * R;
-  const int64_t col = p * R + r;
-
-  real* Pbuf = reinterpret_cast<real*>(lds_raw);
-  uint32_t* ring = reinterpret_cast<uint32_t*>(Pbuf + 2 * NM);
-  real* stk = reinterpret_cast<real*>(ring + (size_t)kTipBatch * bd);
-  int32_t* stke = reinterpret_cast<int32_t*>(stk + (size_t)stack_depth * 4 * bd);
-
-  real pf[kPD][PF];
-  auto load_slot = [&](real* dst, int node) {
-    const int4 on = prog[node];
-#pragma unroll
-    for (int k = 0; k < PF; ++k) {
-      const int j = tid + k * bd;
-      if (j < NM) dst[k] = stage_elem<real, R>(on, j, Pm, Pm, Pm);
-    }
-  };
-  for (int j = tid; j < NM; j += bd) Pbuf[j] = stage_elem<real, R>(prog[0], j, Pm, Pm, Pm);
-  static_for<1, kPD>([&](auto U) {
-    constexpr int u = decltype(U)::value;
-    if (u < n_ops) load_slot(pf[u], u);
-  });
-  TipStream<kTipBatch> ts;
-  ts.ring = ring;
-  ts.ids = leaf_ids;
-  ts.n_ids = n_leaf_ids;
-  ts.tips = tips;
-  ts.stride = stride;
-  ts.pc = pc;
-  ts.bd = bd;
-  ts.tid = tid;
-  ts.init();
-  __syncthreads();
-
-  real acc[4] = {(real)0, (real)0, (real)0, (real)0};
-  int acc_e = 0, sp = 0;
-  for (int base = 0; base < n_ops; base += kPD) {
-    static_for<0, kPD>([&](auto U) {
-      constexpr int u = decltype(U)::value;
-      const int it = base + u;
-      if (it >= n_ops) return;
-      const int4 op = prog[it];
-      if (it + kPD < n_ops) load_slot(pf[u], it + kPD);
-      const real* Pc = Pbuf + (u & 1) * NM;
-      const real* PA = Pc + r * 16;
-      const real* PB = Pc + (R + r) * 16;
-      const int fl = op.w;
-      // the LDS stack holds spilled partials in both forms; the stored form
-      // additionally writes every partial to HBM but never reads it back
-      if (fl & DOWN_SPILL) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) stk[((size_t)sp * 4 + k) * bd + tid] = acc[k];
-        stke[(size_t)sp * bd + tid] = acc_e;
-        ++sp;
-      }
-      const int sa = fl & 3, sb = (fl >> 2) & 3;
-      real xa[4], xb[4], src[4];
-      int ea = 0, eb = 0;
-      if (sa == SRC_LEAF) {
-        tipvec(ts.get(), src);
-      } else if (sa == SRC_ACC) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) src[k] = acc[k];
-        ea = acc_e;
-      } else {
-        --sp;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) src[k] = stk[((size_t)sp * 4 + k) * bd + tid];
-        ea = stke[(size_t)sp * bd + tid];
-      }
-      matvec4(PA, src, xa);
-      if (sb == SRC_LEAF) {
-        tipvec(ts.get(), src);
-      } else {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) src[k] = acc[k];
-        eb = acc_e;
-      }
-      matvec4(PB, src, xb);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) acc[k] = xa[k] * xb[k];
-      acc_e = ea + eb;
-      rescale_pattern<real, R>(acc, &acc_e);
-      if (!FUSED && act) {
-        real* d = part + (size_t)(op.x - N) * 4 * rowlen + col;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) d[(size_t)k * rowlen] = acc[k];
-        if (r == 0) ex[(size_t)(op.x - N) * pitch + p] = acc_e;
-      }
-      if (it + 1 < n_ops) {
-        constexpr int nu = (u + 1) % kPD;
-#pragma unroll
-        for (int k = 0; k < PF; ++k) {
-          const int j = tid + k * bd;
-          if (j < NM) Pbuf[((u + 1) & 1) * NM + j] = pf[nu][k];
-        }
-      }
-      __syncthreads();
-    });
-  }
-  // root: ln L_p = ln sum_r prob_r sum_i pi_i L[r][i] + e ln 2
-  double s = 0.0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) s += pi[i] * (double)acc[i];
-  s = rate_sum<R>(s * probs[r]);
-  const double slnl = log(s) + (double)acc_e * 0.6931471805599453;
-  double contrib = 0.0;
-  if (act && r == 0) {
-    if (site_lnl) site_lnl[p] = slnl;
-    contrib = (double)weights[p] * slnl;
-  }
-  // one partial sum per wave: the reduction order does not depend on the
-  // block size
-  const double ws = wave_sum_d(contrib);
-  if ((tid & 63) == 0) wave_sums[((int64_t)blockIdx.x * bd + tid) >> 6] = ws;
-}
-
-template <typename real, int R, bool FUSED>
-__global__ __launch_bounds__(256) void k_prune_down(
-    const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids, int n_leaf_ids,
-    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
-    const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
-    real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
-    double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth) {
-  prune_down_body<real, R, FUSED>(prog, n_ops, leaf_ids, n_leaf_ids, tips, stride, n_pat, pitch, Pm,
-                                  pi, probs, part, ex, weights, site_lnl, wave_sums, N,
-                                  stack_depth);
-}
-
-// Many candidate topologies over the same tips and weights (fused form):
-// blockIdx.y selects candidate c, whose program, leaf order, P matrices and
-// per-wave sums sit at fixed strides.
-template <typename real, int R>
-__global__ __launch_bounds__(256) void k_prune_down_batch(
-    const int4* __restrict__ prog, const int32_t* __restrict__ leaf_ids, int N,
-    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
-    const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
-    const uint32_t* __restrict__ weights, double* __restrict__ wave_sums, int64_t n_waves,
-    int stack_depth) {
-  const int64_t c = blockIdx.y;
-  const int M = 2 * N - 1;
-  prune_down_body<real, R, true>(prog + c * (N - 1), N - 1, leaf_ids + c * N, N, tips, stride,
-                                 n_pat, pitch, Pm + c * (int64_t)M * R * 16, pi, probs, nullptr,
-                                 nullptr, weights, nullptr, wave_sums + c * n_waves, N,
-                                 stack_depth);
-}
-
 // ---------------------------------------------------------------------------
 // Pre-order derivative pass: for every branch c (parent p, sibling s)
 //   A_c = M_p (*) (P_s D_s),  M_root = pi,  M_c = A_c^T P_c
@@ -475,8 +466,8 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
     const int4* __restrict__ prog, int n_ops, const uint64_t* __restrict__ tips, int64_t stride,
     int64_t n_pat, int64_t pitch, const real* __restrict__ Pm, const real* __restrict__ P1m,
     const real* __restrict__ P2m, const double* __restrict__ pi, const double* __restrict__ probs,
-    const real* __restrict__ part, const uint32_t* __restrict__ weights,
-    double* __restrict__ wave_sums, int N) {
+    const real* __restrict__ part, const int32_t* __restrict__ ex,
+    const uint32_t* __restrict__ weights, double* __restrict__ wave_sums, int N) {
   constexpr int NM = 6 * R * 16;
   constexpr int PF = (NM + 127) / 128;  // blocks of >= 128 threads
   extern __shared__ unsigned char lds_raw[];
@@ -502,14 +493,29 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
       if (j < NM) dst[k] = stage_elem<real, R>(on, j, Pm, P1m, P2m);
     }
   };
-  // children partials (4 values each) of an entry
-  auto load_D = [&](int c, real* Dv) {
+  // children partials (4 values each) of an entry, fetched raw one node ahead
+  // (each rate class has its own exponent in the post-order pass) ...
+  auto fetch_D = [&](int c, real* Dv, int* e) {
     if (c < N) {
       tipvec(tip_code(tips, stride, c, pc), Dv);
+      *e = 0;
     } else {
-      const real* s = part + (size_t)(c - N) * 4 * rowlen + (pc * R + r);
+      const real* sv = part + (((size_t)(c - N) * R + r) * 4) * pitch + pc;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) Dv[k] = s[(size_t)k * rowlen];
+      for (int k = 0; k < 4; ++k) Dv[k] = sv[(size_t)k * pitch];
+      *e = ex[((size_t)(c - N) * R + r) * pitch + pc];
+    }
+  };
+  // ... and aligned at use to the largest exponent of the pattern's classes
+  // (exact power-of-two scaling), so the R classes share one scale here
+  auto align_D = [&](real* Dv, int e) {
+    int m = e;
+    if (R >= 2) m = max(m, __builtin_amdgcn_update_dpp(0, m, DPP_XOR1, 0xF, 0xF, false));
+    if (R >= 4) m = max(m, __builtin_amdgcn_update_dpp(0, m, DPP_XOR2, 0xF, 0xF, false));
+    if (R >= 8) m = max(m, __builtin_amdgcn_update_dpp(0, m, DPP_HALF_MIRROR, 0xF, 0xF, false));
+    if (e != m) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) Dv[k] = ldexp(Dv[k], e - m);
     }
   };
   for (int j = tid; j < NM; j += bd) Pbuf[j] = stage_elem<real, R>(prog[0], j, Pm, P1m, P2m);
@@ -518,10 +524,11 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
     if (u < n_ops) load_slot(pf[u], u);
   });
   real Dn[2][8];  // children partials of the current / next entry
+  int En[2][2];
   {
     const int4 o = prog[0];
-    load_D(o.y, Dn[0]);
-    load_D(o.z, Dn[0] + 4);
+    fetch_D(o.y, Dn[0], &En[0][0]);
+    fetch_D(o.z, Dn[0] + 4, &En[0][1]);
   }
   __syncthreads();
 
@@ -537,8 +544,8 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
       if (it + kPDD < n_ops) load_slot(pf[u], it + kPDD);
       if (it + 1 < n_ops) {
         const int4 on = prog[it + 1];
-        load_D(on.y, Dn[nxt]);
-        load_D(on.z, Dn[nxt] + 4);
+        fetch_D(on.y, Dn[nxt], &En[nxt][0]);
+        fetch_D(on.z, Dn[nxt] + 4, &En[nxt][1]);
       }
       const real* Pc = Pbuf + (u & 1) * NM;
       // matrices (which, child): P_a, P_b, P'_a, P'_b, P''_a, P''_b
@@ -562,8 +569,10 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
 #pragma unroll
         for (int k = 0; k < 4; ++k) Mv[k] = stk[((size_t)sp * 4 + k) * bd + tid];
       }
-      const real* Da = Dn[cur];
-      const real* Db = Dn[cur] + 4;
+      real* Da = Dn[cur];
+      real* Db = Dn[cur] + 4;
+      align_D(Da, En[cur][0]);
+      align_D(Db, En[cur][1]);
       real ya[4], yb[4], x1a[4], x2a[4], x1b[4], x2b[4];
       matvec4(Pa, Da, ya);
       matvec4(Pb, Db, yb);
@@ -824,16 +833,19 @@ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 struct Plan {
   size_t off_P = 0, off_P1 = 0, off_P2 = 0, off_part = 0, off_ex = 0, off_bsum = 0, off_wsum = 0;
   size_t total = 0;
-  int64_t pitch = 0, n_blocks = 0, n_waves = 0, dblocks = 0;
+  int64_t pitch = 0, n_blocks = 0, n_waves = 0, dblocks = 0, n_dwaves = 0;
   int bs = 0, dbs = 0;  // block size of the post-order pass / derivative pass
   size_t esz = 8, lds = 0, dlds = 0;
+  bool fused = false;
 };
 
 constexpr size_t kLdsMax = 64 * 1024;
 
-size_t down_lds(int R, size_t esz, int bs, bool fused, int depth) {
-  return (size_t)2 * 2 * R * 16 * esz + (size_t)kTipBatch * bs * 4 +
-         (size_t)std::max(depth, 1) * bs * (4 * esz + 4);
+// LDS of the post-order pass: root exchange, tip ring, (fused) stack
+size_t down_lds(int R, size_t esz, bool fused, int depth) {
+  const size_t bs = 64 * (size_t)R;
+  return (size_t)R * 64 * 12 + (size_t)kTipBatch * bs * 4 +
+         (fused ? (size_t)std::max(depth, 1) * bs * (4 * esz + 4) : 0);
 }
 size_t deriv_lds(int R, size_t esz, int bs, int depth) {
   return (size_t)2 * 6 * R * 16 * esz + (size_t)std::max(depth, 1) * bs * 4 * esz;
@@ -854,9 +866,11 @@ Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t for
   const bool fused = (form & PHY_FUSED) != 0;
   const size_t esz = pl.esz;
   const int depth = t->down_depth;
-  pl.bs = fit_block(env_block(fused ? "PHY_PRUNE_FUSED_BLOCK" : "PHY_PRUNE_BLOCK", 256),
-                    [&](int bs) { return down_lds(R, esz, bs, fused, depth); });
-  pl.lds = pl.bs ? down_lds(R, esz, pl.bs, fused, depth) : 0;
+  pl.bs = 64 * R;  // one wave per rate class over 64 patterns
+  // a fused stack that does not fit in LDS falls back to stored partials
+  // (same arithmetic and result, more HBM traffic)
+  pl.fused = fused && down_lds(R, esz, true, depth) <= kLdsMax;
+  pl.lds = down_lds(R, esz, pl.fused, depth);
   if (derivs) {
     pl.dbs = fit_block(std::max(128, env_block("PHY_PRUNE_DERIV_BLOCK", 256)),
                        [&](int bs) { return deriv_lds(R, esz, bs, t->up_depth); }, 128);
@@ -864,9 +878,10 @@ Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t for
   }
   pl.pitch = ((n_pat + 255) / 256) * 256;
   if (pl.pitch == 0) pl.pitch = 256;
-  pl.n_blocks = pl.bs ? pl.pitch * R / pl.bs : 0;
+  pl.n_blocks = pl.pitch / 64;
   pl.dblocks = pl.dbs ? pl.pitch * R / pl.dbs : 0;
-  pl.n_waves = pl.pitch * R / 64;
+  pl.n_waves = pl.pitch / 64;          // partial lnL sums: one per 64 patterns
+  pl.n_dwaves = pl.pitch * R / 64;     // derivative sums: one per wave
   size_t o = 0;
   const size_t pm = (size_t)M * R * K * K * pl.esz;
   pl.off_P = o; o = align256(o + pm);
@@ -874,13 +889,13 @@ Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t for
     pl.off_P1 = o; o = align256(o + pm);
     pl.off_P2 = o; o = align256(o + pm);
   }
-  if (!fused) {
+  if (!pl.fused) {
     pl.off_part = o; o = align256(o + (size_t)(N - 1) * R * K * pl.pitch * pl.esz);
-    pl.off_ex = o; o = align256(o + (size_t)(N - 1) * pl.pitch * 4);
+    pl.off_ex = o; o = align256(o + (size_t)(N - 1) * R * pl.pitch * 4);
   }
   pl.off_bsum = o; o = align256(o + (size_t)pl.n_waves * 8);
   if (derivs) {
-    pl.off_wsum = o; o = align256(o + (size_t)pl.n_waves * M * 2 * 8);
+    pl.off_wsum = o; o = align256(o + (size_t)pl.n_dwaves * M * 2 * 8);
   }
   pl.total = o;
   return pl;
@@ -898,8 +913,8 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
   real* P = reinterpret_cast<real*>(work + pl.off_P);
   real* P1 = derivs ? reinterpret_cast<real*>(work + pl.off_P1) : nullptr;
   real* P2 = derivs ? reinterpret_cast<real*>(work + pl.off_P2) : nullptr;
-  if (!pl.bs) return set_err(ctx, PHY_ERANGE, "pruning: LDS stack too deep for this tree");
   if (derivs && !pl.dbs) return set_err(ctx, PHY_ERANGE, "pruning: derivative LDS stack too deep");
+  fused = pl.fused;
   {
     const int64_t n = (int64_t)M * R * K * K;
     hipLaunchKernelGGL((k_pmats<real>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
@@ -924,11 +939,11 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
     double* wsum = reinterpret_cast<double*>(work + pl.off_wsum);
     hipLaunchKernelGGL((k_prune_deriv<real, R>), dim3((unsigned)pl.dblocks), dim3(pl.dbs), pl.dlds,
                        ctx->stream, t->d_up_prog, N - 1, tips, stride, n_pat, pl.pitch, P, P1, P2,
-                       d_pi, d_probs, part, weights, wsum, N);
+                       d_pi, d_probs, part, ex, weights, wsum, N);
     PHY_LAUNCHED(ctx);
     const int ra = t->children[2 * (t->root - N)], rb = t->children[2 * (t->root - N) + 1];
     hipLaunchKernelGGL(k_reduce_derivs, dim3((unsigned)M), dim3(256), 0, ctx->stream, wsum,
-                       pl.n_waves, M, t->root, ra, rb, d1, d2);
+                       pl.n_dwaves, M, t->root, ra, rb, d1, d2);
     PHY_LAUNCHED(ctx);
   }
   return PHY_OK;
@@ -1037,8 +1052,8 @@ int phy_lk_tree_partials(const phy_topo* t, const phy_model* m, int64_t n_pat, i
   if (v < N || v > 2 * N - 2) return PHY_EINVAL;
   const Plan pl = make_plan(t, m, n_pat, form, derivs);
   char* w = static_cast<char*>(work);
-  if (d_partial) *d_partial = w + pl.off_part + (size_t)(v - N) * m->R * m->K * pl.pitch * pl.esz;  // [4][pitch*R]
-  if (d_exp) *d_exp = reinterpret_cast<int32_t*>(w + pl.off_ex) + (size_t)(v - N) * pl.pitch;
+  if (d_partial) *d_partial = w + pl.off_part + (size_t)(v - N) * m->R * m->K * pl.pitch * pl.esz;
+  if (d_exp) *d_exp = reinterpret_cast<int32_t*>(w + pl.off_ex) + (size_t)(v - N) * m->R * pl.pitch;
   if (pitch) *pitch = pl.pitch;
   return PHY_OK;
 }
@@ -1089,10 +1104,10 @@ SprPlan spr_plan(int32_t N, const phy_model* m, int64_t n_pat, int32_t form, int
   SprPlan pl;
   const int M = 2 * N - 1;
   const size_t esz = (form & PHY_F32) ? 4 : 8;
-  pl.bs = env_block("PHY_PRUNE_FUSED_BLOCK", 256);
+  pl.bs = 64 * m->R;
   pl.pitch = ((n_pat + 255) / 256) * 256;
-  pl.n_blocks = pl.pitch * m->R / pl.bs;
-  pl.n_waves = pl.pitch * m->R / 64;
+  pl.n_blocks = pl.pitch / 64;
+  pl.n_waves = pl.pitch / 64;
   pl.depth_cap = 12;
   size_t o = 0;
   pl.off_P = o; o = align256(o + (size_t)chunk * M * m->R * 16 * esz);
@@ -1117,11 +1132,10 @@ int spr_launch(phy_ctx* ctx, const phy_model* m, int N, int64_t n_pat, const uin
   hipLaunchKernelGGL((k_pmats<real>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, cst,
                      K, R, bl, 0.0, 0.0, C * M, P, (real*)nullptr, (real*)nullptr);
   PHY_LAUNCHED(ctx);
-  const int bs = fit_block(pl.bs, [&](int b) { return down_lds(R, sizeof(real), b, true, depth); });
-  if (!bs) return set_err(ctx, PHY_ERANGE, "spr scores: LDS stack too deep");
-  const size_t lds = down_lds(R, sizeof(real), bs, true, depth);
+  const size_t lds = down_lds(R, sizeof(real), true, depth);
+  if (lds > kLdsMax) return set_err(ctx, PHY_ERANGE, "spr scores: LDS stack too deep");
   double* ws = reinterpret_cast<double*>(work + pl.off_ws);
-  hipLaunchKernelGGL((k_prune_down_batch<real, R>), dim3((unsigned)(pl.pitch * R / bs), (unsigned)C), dim3(bs),
+  hipLaunchKernelGGL((k_prune_down_batch<real, R>), dim3((unsigned)(pl.pitch / 64), (unsigned)C), dim3(64 * R),
                      lds, ctx->stream, reinterpret_cast<const int4*>(work + pl.off_prog),
                      reinterpret_cast<const int32_t*>(work + pl.off_fifo), N, tips, stride, n_pat,
                      pl.pitch, P, d_pi, d_pi + K, w, ws, pl.n_waves, depth);

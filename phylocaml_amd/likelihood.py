@@ -150,9 +150,9 @@ class Evaluator:
             D.dptr(self.work), self.work_bytes), "lk_tree")
 
     def partial(self, v: int):
-        """(partial [R*K, pitch] rate-major view, exps [pitch]) of internal node v
-        after run().  In the workspace the partial is stored as [K][pitch*R]
-        (column p*R + r), the layout the kernels write coalesced."""
+        """(partial [R*K, pitch], exps [R, pitch]) views of internal node v after
+        run(): row r*K + i holds L[r][i][p], and the value is
+        L[r][i][p] * 2**exps[r][p] (one exponent per pattern and rate class)."""
         torch = __import__("torch")
         pp, pe, pitch = C.c_void_p(), C.c_void_p(), C.c_int64()
         check(lib.phy_lk_tree_partials(self.topo.handle, self.dm.handle, self.n_patterns,
@@ -163,9 +163,8 @@ class Evaluator:
         off_p, off_e = pp.value - base, pe.value - base
         R, K, P = self.dm.R, self.dm.K, pitch.value
         nbytes = R * K * P * esz
-        raw = self.work[off_p: off_p + nbytes].view(_dtype(self.form)).view(K, P, R)
-        part = raw.permute(2, 0, 1).reshape(R * K, P)
-        ex = self.work[off_e: off_e + 4 * P].view(torch.int32)
+        part = self.work[off_p: off_p + nbytes].view(_dtype(self.form)).view(R * K, P)
+        ex = self.work[off_e: off_e + 4 * R * P].view(torch.int32).view(R, P)
         return part, ex
 
 

@@ -176,7 +176,7 @@ def features(ctx, info: TreeInfo, ft, n_sites, weights_dev, mv: np.ndarray,
     x = np.where(chp[:, 0] == s, chp[:, 1], chp[:, 0])
     g = info.parent[p]
     u = info.parent[v]
-    D = max(1, int(info.depth.max()))
+    Dmax = max(1, int(info.depth.max()))
     bl = info.blen
     tu = np.where(u >= 0, bl[np.maximum(u, 0)], 0.0)
     l = info.lca(s, v)
@@ -209,13 +209,13 @@ def features(ctx, info: TreeInfo, ft, n_sites, weights_dev, mv: np.ndarray,
         np.log1p(info.size[s]), np.log1p(info.size[v]), np.log1p(N - info.size[s]),
         info.size[s] / N, info.size[v] / N,
         bl[s], bl[p], bl[x], bl[v], tu, bl[x] + bl[p], np.log(bl[s] + 1e-6), np.log(bl[v] + 1e-6),
-        info.depth[s] / D, info.depth[v] / D, path / D, np.log1p(path), pat,
+        info.depth[s] / Dmax, info.depth[v] / Dmax, path / Dmax, np.log1p(path), pat,
         info.is_anc(v, s).astype(np.float64), info.is_anc(x, v).astype(np.float64),
         (u == g).astype(np.float64),
         100 * c_new / W, 100 * c_old / W, 100 * dc / W, dc / (c_old + 1.0),
         np.sign(dc) * np.log1p(np.abs(dc)), c_new / (c_old + 1.0),
         (p == info.root).astype(np.float64), (g < 0).astype(np.float64),
-        (info.depth[v] - info.depth[s]) / D, bl[s] * info.size[s] / N, bl[v] / 2 + bl[s],
+        (info.depth[v] - info.depth[s]) / Dmax, bl[s] * info.size[s] / N, bl[v] / 2 + bl[s],
     ]
     for k, col in enumerate(cols):
         F[:, k] = col
