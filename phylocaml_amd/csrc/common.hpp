@@ -71,5 +71,13 @@ inline int set_err(phy_ctx* ctx, int code, const std::string& msg) {
 // set); returns 0 on success.  Host only.
 int phy_compile_topo(phy_topo* t);
 
+// Device copies of leaf / node id streams are padded past their end with a
+// valid id (the stream's first), so kernels prefetch whole batches of ids and
+// rows with no bounds check: two batches of at most 64 ids may be read past
+// the last one consumed.  The padded length is a multiple of 64, which keeps
+// every batch start 16-byte aligned for vector id loads.
+constexpr size_t kFifoSlack = 128;
+__host__ __device__ constexpr size_t phy_fifo_padded(size_t n) { return ((n + kFifoSlack + 63) / 64) * 64; }
+
 // Zero n uint64 values on the stream with a kernel (graph-capture safe).
 int phy_zero_u64(phy_ctx* ctx, uint64_t* d, int64_t n);

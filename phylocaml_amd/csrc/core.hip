@@ -314,16 +314,20 @@ int phy_topo_create(phy_ctx* ctx, int32_t N, const int32_t* children, int32_t ro
     return PHY_OK;
   }
   hipError_t e = hipSuccess;
-  const size_t pd = t->down_prog.size() * sizeof(int32_t), fd = t->down_fifo.size() * sizeof(int32_t);
-  const size_t pu = t->up_prog.size() * sizeof(int32_t), fu = t->up_fifo.size() * sizeof(int32_t);
+  std::vector<int32_t> dfifo(phy_fifo_padded(t->down_fifo.size()), t->down_fifo[0]);
+  std::vector<int32_t> ufifo(phy_fifo_padded(t->up_fifo.size()), t->up_fifo[0]);
+  std::copy(t->down_fifo.begin(), t->down_fifo.end(), dfifo.begin());
+  std::copy(t->up_fifo.begin(), t->up_fifo.end(), ufifo.begin());
+  const size_t pd = t->down_prog.size() * sizeof(int32_t), fd = dfifo.size() * sizeof(int32_t);
+  const size_t pu = t->up_prog.size() * sizeof(int32_t), fu = ufifo.size() * sizeof(int32_t);
   if (e == hipSuccess) e = hipMalloc((void**)&t->d_down_prog, pd);
   if (e == hipSuccess) e = hipMalloc((void**)&t->d_down_fifo, fd);
   if (e == hipSuccess) e = hipMalloc((void**)&t->d_up_prog, pu);
   if (e == hipSuccess) e = hipMalloc((void**)&t->d_up_fifo, fu);
   if (e == hipSuccess) e = hipMemcpy(t->d_down_prog, t->down_prog.data(), pd, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(t->d_down_fifo, t->down_fifo.data(), fd, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(t->d_down_fifo, dfifo.data(), fd, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(t->d_up_prog, t->up_prog.data(), pu, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(t->d_up_fifo, t->up_fifo.data(), fu, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(t->d_up_fifo, ufifo.data(), fu, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     phy_topo_destroy(t);
     return set_err(ctx, PHY_EDEVICE, std::string("topology upload: ") + hipGetErrorString(e));

@@ -213,12 +213,18 @@ __global__ __launch_bounds__(256) void k_fitch_down(
   uint32_t* ring = lds32;
   uint32_t* stk = lds32 + (size_t)BATCH * bd;
 
+  // the id stream is padded (phy_fifo_padded) and wc is in bounds: batches
+  // load with no predicate, so the compiler keeps them in flight
   uint32_t pre[BATCH];
   auto issue = [&](int start) {
+    const int4* iv = reinterpret_cast<const int4*>(fifo_ids + start);
 #pragma unroll
-    for (int j = 0; j < BATCH; ++j) {
-      const int idx = start + j;
-      pre[j] = (act && idx < n_fifo) ? sets[(int64_t)fifo_ids[idx] * stride + wc] : 0u;
+    for (int q = 0; q < BATCH / 4; ++q) {
+      const int4 id = iv[q];
+      pre[4 * q + 0] = sets[(int64_t)id.x * stride + wc];
+      pre[4 * q + 1] = sets[(int64_t)id.y * stride + wc];
+      pre[4 * q + 2] = sets[(int64_t)id.z * stride + wc];
+      pre[4 * q + 3] = sets[(int64_t)id.w * stride + wc];
     }
   };
   issue(0);
@@ -329,12 +335,18 @@ __global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog,
   uint32_t* ring = lds32;
   uint32_t* stk = lds32 + (size_t)BATCH * bd;  // pairs (P, F)
 
+  // the id stream is padded (phy_fifo_padded) and wc is in bounds: batches
+  // load with no predicate, so the compiler keeps them in flight
   uint32_t pre[BATCH];
   auto issue = [&](int start) {
+    const int4* iv = reinterpret_cast<const int4*>(fifo_ids + start);
 #pragma unroll
-    for (int j = 0; j < BATCH; ++j) {
-      const int idx = start + j;
-      pre[j] = (act && idx < n_fifo) ? sets[(int64_t)fifo_ids[idx] * stride + wc] : 0u;
+    for (int q = 0; q < BATCH / 4; ++q) {
+      const int4 id = iv[q];
+      pre[4 * q + 0] = sets[(int64_t)id.x * stride + wc];
+      pre[4 * q + 1] = sets[(int64_t)id.y * stride + wc];
+      pre[4 * q + 2] = sets[(int64_t)id.z * stride + wc];
+      pre[4 * q + 3] = sets[(int64_t)id.w * stride + wc];
     }
   };
   issue(0);

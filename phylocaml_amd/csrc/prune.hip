@@ -185,21 +185,27 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 }
 
 // Small ring that prefetches the tip codes of the post-order leaf stream.
+// The id stream is padded (phy_fifo_padded) and pc is an in-bounds column,
+// so a batch is BATCH/4 vector id loads and BATCH unconditional tip loads.
 template <int BATCH>
 struct TipStream {
+  static_assert(BATCH % 4 == 0 && BATCH <= 64, "batch of whole int4 id groups");
   uint32_t pre[BATCH];
   uint32_t* ring;
   const int32_t* ids;
-  int n_ids;
   const uint64_t* tips;
   int64_t stride;
   int64_t pc;
   int next, pos, bd, tid;
   __device__ __forceinline__ void issue(int start) {
+    const int4* iv = reinterpret_cast<const int4*>(ids + start);
 #pragma unroll
-    for (int j = 0; j < BATCH; ++j) {
-      const int idx = start + j;
-      pre[j] = idx < n_ids ? tip_code(tips, stride, ids[idx], pc) : 0u;
+    for (int q = 0; q < BATCH / 4; ++q) {
+      const int4 id = iv[q];
+      pre[4 * q + 0] = tip_code(tips, stride, id.x, pc);
+      pre[4 * q + 1] = tip_code(tips, stride, id.y, pc);
+      pre[4 * q + 2] = tip_code(tips, stride, id.z, pc);
+      pre[4 * q + 3] = tip_code(tips, stride, id.w, pc);
     }
   }
   __device__ __forceinline__ void init() {
@@ -280,7 +286,6 @@ __device__ __forceinline__ void prune_down_body(
   TipStream<kTipBatch> ts;
   ts.ring = ring;
   ts.ids = leaf_ids;
-  ts.n_ids = n_leaf_ids;
   ts.tips = tips;
   ts.stride = stride;
   ts.pc = pc;
@@ -392,7 +397,7 @@ __global__ __launch_bounds__(512) void k_prune_down_batch(
     int stack_depth) {
   const int64_t c = blockIdx.y;
   const int M = 2 * N - 1;
-  prune_down_body<real, R, true>(prog + c * (N - 1), N - 1, leaf_ids + c * N, N, tips, stride,
+  prune_down_body<real, R, true>(prog + c * (N - 1), N - 1, leaf_ids + c * (int64_t)phy_fifo_padded(N), N, tips, stride,
                                  n_pat, pitch, Pm + c * (int64_t)M * R * 16, pi, probs, nullptr,
                                  nullptr, weights, nullptr, wave_sums + c * n_waves, N,
                                  stack_depth);
@@ -1113,7 +1118,7 @@ SprPlan spr_plan(int32_t N, const phy_model* m, int64_t n_pat, int32_t form, int
   pl.off_P = o; o = align256(o + (size_t)chunk * M * m->R * 16 * esz);
   pl.off_bl = o; o = align256(o + (size_t)chunk * M * 8);
   pl.off_prog = o; o = align256(o + (size_t)chunk * (N - 1) * 16);
-  pl.off_fifo = o; o = align256(o + (size_t)chunk * N * 4);
+  pl.off_fifo = o; o = align256(o + (size_t)chunk * phy_fifo_padded(N) * 4);
   pl.off_ws = o; o = align256(o + (size_t)chunk * pl.n_waves * 8);
   pl.total = o;
   return pl;
@@ -1176,7 +1181,8 @@ int phy_lk_spr_scores(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_
   for (int64_t base = 0; base < n_moves; base += chunk) {
     const int C = (int)std::min<int64_t>(chunk, n_moves - base);
     prog.assign((size_t)C * (N - 1) * 4, 0);
-    fifo.assign((size_t)C * N, 0);
+    const size_t Fp = phy_fifo_padded(N);
+    fifo.assign((size_t)C * Fp, 0);
     bl.assign((size_t)C * M, 0.0);
     std::vector<int> depth(C, 0), bad(C, 0);
     const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -1197,7 +1203,8 @@ int phy_lk_spr_scores(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_
             continue;
           }
           std::copy(t->down_prog.begin(), t->down_prog.end(), prog.begin() + (size_t)c * (N - 1) * 4);
-          std::copy(t->down_fifo.begin(), t->down_fifo.end(), fifo.begin() + (size_t)c * N);
+          std::fill(fifo.begin() + (size_t)c * Fp, fifo.begin() + (size_t)(c + 1) * Fp, t->down_fifo[0]);
+          std::copy(t->down_fifo.begin(), t->down_fifo.end(), fifo.begin() + (size_t)c * Fp);
           depth[c] = t->down_depth;
           phy_topo_destroy(t);
         }
