@@ -18,6 +18,7 @@
 // (32) loads ahead through registers into an LDS ring, which keeps enough bytes in
 // flight to cover HBM latency with one wave per SIMD.
 #include "common.hpp"
+#include "stream.hpp"
 
 namespace {
 
@@ -199,9 +200,15 @@ __device__ __forceinline__ uint64_t weigh32(uint32_t e, const uint32_t* __restri
   return c;
 }
 
+struct SetLoad {
+  const uint32_t* sets;
+  int64_t stride, wc;
+  __device__ __forceinline__ uint32_t operator()(int id) const { return sets[(int64_t)id * stride + wc]; }
+};
+
 template <int BATCH, bool NODE_COST>
 __global__ __launch_bounds__(256) void k_fitch_down(
-    const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ fifo_ids, int n_fifo,
+    const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ fifo_ids,
     uint32_t* __restrict__ sets, int64_t stride, int64_t W, int64_t n_sites, int N,
     const uint32_t* __restrict__ wt, uint32_t* __restrict__ site_cost,
     unsigned long long* __restrict__ node_cost, unsigned long long* __restrict__ total) {
@@ -210,39 +217,9 @@ __global__ __launch_bounds__(256) void k_fitch_down(
   const int64_t w = (int64_t)blockIdx.x * bd + tid;  // 32-bit lane index
   const bool act = w < W;                              // W = 2 * words
   const int64_t wc = act ? w : 0;
-  uint32_t* ring = lds32;
-  uint32_t* stk = lds32 + (size_t)BATCH * bd;
-
-  // the id stream is padded (phy_fifo_padded) and wc is in bounds: batches
-  // load with no predicate, so the compiler keeps them in flight
-  uint32_t pre[BATCH];
-  auto issue = [&](int start) {
-    const int4* iv = reinterpret_cast<const int4*>(fifo_ids + start);
-#pragma unroll
-    for (int q = 0; q < BATCH / 4; ++q) {
-      const int4 id = iv[q];
-      pre[4 * q + 0] = sets[(int64_t)id.x * stride + wc];
-      pre[4 * q + 1] = sets[(int64_t)id.y * stride + wc];
-      pre[4 * q + 2] = sets[(int64_t)id.z * stride + wc];
-      pre[4 * q + 3] = sets[(int64_t)id.w * stride + wc];
-    }
-  };
-  issue(0);
-#pragma unroll
-  for (int j = 0; j < BATCH; ++j) ring[j * bd + tid] = pre[j];
-  issue(BATCH);
-  int next_batch = 2 * BATCH;
-  int fpos = 0;
-  auto next_leaf = [&]() -> uint32_t {
-    if (fpos == BATCH) {
-#pragma unroll
-      for (int j = 0; j < BATCH; ++j) ring[j * bd + tid] = pre[j];
-      issue(next_batch);
-      next_batch += BATCH;
-      fpos = 0;
-    }
-    return ring[(fpos++) * bd + tid];
-  };
+  uint32_t* stk = lds32 + (size_t)2 * BATCH * bd;
+  OperandRing<BATCH, SetLoad> leaves;
+  leaves.init(lds32, fifo_ids, SetLoad{sets, stride, wc}, bd, tid);
 
   uint32_t cnt = 0;  // nibble counters, flushed every 15 nodes
   uint32_t site[8];
@@ -252,8 +229,7 @@ __global__ __launch_bounds__(256) void k_fitch_down(
   uint32_t acc = 0;
   int sp = 0;
   const uint32_t vm = valid_nibbles32(wc, n_sites);
-  for (int i = 0; i < n_ops; ++i) {
-    const int4 op = prog[i];
+  for_each_entry(prog, n_ops, [&](int, const int4 op) {
     const int fl = op.w;
     if (fl & DOWN_SPILL) {
       stk[sp * bd + tid] = acc;
@@ -261,10 +237,10 @@ __global__ __launch_bounds__(256) void k_fitch_down(
     }
     const int sa = fl & 3, sb = (fl >> 2) & 3;
     uint32_t x, y;
-    if (sa == SRC_LEAF) x = next_leaf();
+    if (sa == SRC_LEAF) x = leaves.get();
     else if (sa == SRC_ACC) x = acc;
     else { --sp; x = stk[sp * bd + tid]; }
-    if (sb == SRC_LEAF) y = next_leaf();
+    if (sb == SRC_LEAF) y = leaves.get();
     else y = acc;  // SRC_ACC (B is never taken from the stack)
     uint32_t r;
     const uint32_t e = fitch32(x, y, &r);
@@ -284,7 +260,7 @@ __global__ __launch_bounds__(256) void k_fitch_down(
       cnt = 0;
       since = 0;
     }
-  }
+  });
 #pragma unroll
   for (int k = 0; k < 8; ++k) site[k] += (cnt >> (4 * k)) & 15u;
   uint64_t tot = 0;
@@ -323,7 +299,7 @@ __global__ __launch_bounds__(256) void k_fitch_down(
 // ---------------------------------------------------------------------------
 template <int BATCH>
 __global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog, int n_ops,
-                                                  const int32_t* __restrict__ fifo_ids, int n_fifo,
+                                                  const int32_t* __restrict__ fifo_ids,
                                                   const uint32_t* __restrict__ sets,
                                                   uint32_t* __restrict__ fin, int64_t stride,
                                                   int64_t W, int N) {
@@ -332,56 +308,25 @@ __global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog,
   const int64_t w = (int64_t)blockIdx.x * bd + tid;
   const bool act = w < W;
   const int64_t wc = act ? w : 0;
-  uint32_t* ring = lds32;
-  uint32_t* stk = lds32 + (size_t)BATCH * bd;  // pairs (P, F)
-
-  // the id stream is padded (phy_fifo_padded) and wc is in bounds: batches
-  // load with no predicate, so the compiler keeps them in flight
-  uint32_t pre[BATCH];
-  auto issue = [&](int start) {
-    const int4* iv = reinterpret_cast<const int4*>(fifo_ids + start);
-#pragma unroll
-    for (int q = 0; q < BATCH / 4; ++q) {
-      const int4 id = iv[q];
-      pre[4 * q + 0] = sets[(int64_t)id.x * stride + wc];
-      pre[4 * q + 1] = sets[(int64_t)id.y * stride + wc];
-      pre[4 * q + 2] = sets[(int64_t)id.z * stride + wc];
-      pre[4 * q + 3] = sets[(int64_t)id.w * stride + wc];
-    }
-  };
-  issue(0);
-#pragma unroll
-  for (int j = 0; j < BATCH; ++j) ring[j * bd + tid] = pre[j];
-  issue(BATCH);
-  int next_batch = 2 * BATCH;
-  int fpos = 0;
-  auto next_set = [&]() -> uint32_t {
-    if (fpos == BATCH) {
-#pragma unroll
-      for (int j = 0; j < BATCH; ++j) ring[j * bd + tid] = pre[j];
-      issue(next_batch);
-      next_batch += BATCH;
-      fpos = 0;
-    }
-    return ring[(fpos++) * bd + tid];
-  };
+  uint32_t* stk = lds32 + (size_t)2 * BATCH * bd;  // pairs (P, F)
+  OperandRing<BATCH, SetLoad> nodes;
+  nodes.init(lds32, fifo_ids, SetLoad{sets, stride, wc}, bd, tid);
 
   uint32_t accP = 0, accF = 0;
   int sp = 0;
-  for (int i = 0; i < n_ops; ++i) {
-    const int4 op = prog[i];
+  for_each_entry(prog, n_ops, [&](int, const int4 op) {
     const int fl = op.w;
     const int src = fl & 3;
     uint32_t pv, fp = 0;
-    if (src == SRC_LEAF) pv = next_set();
+    if (src == SRC_LEAF) pv = nodes.get();
     else if (src == SRC_ACC) { pv = accP; fp = accF; }
     else {
       --sp;
       pv = stk[(2 * sp) * bd + tid];
       fp = stk[(2 * sp + 1) * bd + tid];
     }
-    const uint32_t pa = next_set();
-    const uint32_t pb = next_set();
+    const uint32_t pa = nodes.get();
+    const uint32_t pb = nodes.get();
     const uint32_t fv = (src == SRC_LEAF) ? pv : final32(pv, pa, pb, fp);
     if (act) fin[(int64_t)op.x * stride + w] = fv;
     if (fl & UP_A_LEAF) { if (act) fin[(int64_t)op.y * stride + w] = leaf32(fv, pa); }
@@ -394,7 +339,7 @@ __global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog,
     }
     accP = (fl & UP_CONT_B) ? pb : pa;
     accF = fv;
-  }
+  });
 }
 
 // Insertion costs of many (c, a, b) queries: block q sums, over the words of
@@ -514,16 +459,16 @@ int phy_fitch_downpass(phy_ctx* ctx, const phy_topo* topo, uint64_t* sets, int64
   const int bs = block_size_env("PHY_FITCH_BLOCK", 64);
   const int64_t W32 = 2 * W;  // 32-bit lanes
   const int64_t g = (W32 + bs - 1) / bs;
-  const size_t lds = (size_t)(kBatch + topo->down_depth) * bs * sizeof(uint32_t);
+  const size_t lds = (size_t)(2 * kBatch + topo->down_depth) * bs * sizeof(uint32_t);
   if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "downpass: LDS stack too deep for this tree");
   uint32_t* s32 = reinterpret_cast<uint32_t*>(sets);
   if (node_cost)
     hipLaunchKernelGGL((k_fitch_down<kBatch, true>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
-                       topo->d_down_prog, N - 1, topo->d_down_fifo, N, s32, 2 * stride, W32, n_sites, N,
+                       topo->d_down_prog, N - 1, topo->d_down_fifo, s32, 2 * stride, W32, n_sites, N,
                        wt, site_cost, (unsigned long long*)node_cost, (unsigned long long*)total);
   else
     hipLaunchKernelGGL((k_fitch_down<kBatch, false>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
-                       topo->d_down_prog, N - 1, topo->d_down_fifo, N, s32, 2 * stride, W32, n_sites, N,
+                       topo->d_down_prog, N - 1, topo->d_down_fifo, s32, 2 * stride, W32, n_sites, N,
                        wt, site_cost, (unsigned long long*)nullptr, (unsigned long long*)total);
   PHY_LAUNCHED(ctx);
   return PHY_OK;
@@ -541,10 +486,10 @@ int phy_fitch_uppass(phy_ctx* ctx, const phy_topo* topo, const uint64_t* sets, u
   const int bs = block_size_env("PHY_FITCH_BLOCK", 64);
   const int64_t W32 = 2 * W;
   const int64_t g = (W32 + bs - 1) / bs;
-  const size_t lds = (size_t)(kBatch + 2 * topo->up_depth) * bs * sizeof(uint32_t);
+  const size_t lds = (size_t)(2 * kBatch + 2 * topo->up_depth) * bs * sizeof(uint32_t);
   if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "uppass: LDS stack too deep for this tree");
   hipLaunchKernelGGL((k_fitch_up<kBatch>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
-                     topo->d_up_prog, N - 1, topo->d_up_fifo, 2 * N - 1,
+                     topo->d_up_prog, N - 1, topo->d_up_fifo,
                      reinterpret_cast<const uint32_t*>(sets), reinterpret_cast<uint32_t*>(fin),
                      2 * stride, W32, N);
   PHY_LAUNCHED(ctx);

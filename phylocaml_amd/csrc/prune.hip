@@ -25,6 +25,7 @@
 #include <type_traits>
 
 #include "common.hpp"
+#include "stream.hpp"
 #include "model_host.hpp"
 
 struct phy_model {
@@ -184,48 +185,10 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
 }
 
-// Small ring that prefetches the tip codes of the post-order leaf stream.
-// The id stream is padded (phy_fifo_padded) and pc is an in-bounds column,
-// so a batch is BATCH/4 vector id loads and BATCH unconditional tip loads.
-template <int BATCH>
-struct TipStream {
-  static_assert(BATCH % 4 == 0 && BATCH <= 64, "batch of whole int4 id groups");
-  uint32_t pre[BATCH];
-  uint32_t* ring;
-  const int32_t* ids;
+struct TipLoad {
   const uint64_t* tips;
-  int64_t stride;
-  int64_t pc;
-  int next, pos, bd, tid;
-  __device__ __forceinline__ void issue(int start) {
-    const int4* iv = reinterpret_cast<const int4*>(ids + start);
-#pragma unroll
-    for (int q = 0; q < BATCH / 4; ++q) {
-      const int4 id = iv[q];
-      pre[4 * q + 0] = tip_code(tips, stride, id.x, pc);
-      pre[4 * q + 1] = tip_code(tips, stride, id.y, pc);
-      pre[4 * q + 2] = tip_code(tips, stride, id.z, pc);
-      pre[4 * q + 3] = tip_code(tips, stride, id.w, pc);
-    }
-  }
-  __device__ __forceinline__ void init() {
-    issue(0);
-#pragma unroll
-    for (int j = 0; j < BATCH; ++j) ring[j * bd + tid] = pre[j];
-    issue(BATCH);
-    next = 2 * BATCH;
-    pos = 0;
-  }
-  __device__ __forceinline__ uint32_t get() {
-    if (pos == BATCH) {
-#pragma unroll
-      for (int j = 0; j < BATCH; ++j) ring[j * bd + tid] = pre[j];
-      issue(next);
-      next += BATCH;
-      pos = 0;
-    }
-    return ring[(pos++) * bd + tid];
-  }
+  int64_t stride, pc;
+  __device__ __forceinline__ uint32_t operator()(int id) const { return tip_code(tips, stride, id, pc); }
 };
 
 constexpr int kTipBatch = 8;
@@ -275,28 +238,20 @@ __device__ __forceinline__ void prune_down_body(
   const bool act = p < n_pat;
   const int64_t pc = act ? p : 0;
 
-  // LDS: root exchange [R][64] (double S, int e) | tip ring [kTipBatch][bd] |
+  // LDS: root exchange [R][64] (double S, int e) | tip ring [2 kTipBatch][bd] |
   //      (fused) stack [depth][4][bd] real + [depth][bd] int
   double* rootS = reinterpret_cast<double*>(lds_raw);
   int* rootE = reinterpret_cast<int*>(rootS + R * 64);
   uint32_t* ring = reinterpret_cast<uint32_t*>(rootE + R * 64);
-  real* stk = reinterpret_cast<real*>(ring + (size_t)kTipBatch * bd);
+  real* stk = reinterpret_cast<real*>(ring + (size_t)2 * kTipBatch * bd);
   int32_t* stke = reinterpret_cast<int32_t*>(stk + (size_t)(FUSED ? stack_depth : 0) * 4 * bd);
 
-  TipStream<kTipBatch> ts;
-  ts.ring = ring;
-  ts.ids = leaf_ids;
-  ts.tips = tips;
-  ts.stride = stride;
-  ts.pc = pc;
-  ts.bd = bd;
-  ts.tid = tid;
-  ts.init();
+  OperandRing<kTipBatch, TipLoad> ts;
+  ts.init(ring, leaf_ids, TipLoad{tips, stride, pc}, bd, tid);
 
   real acc[4] = {(real)0, (real)0, (real)0, (real)0};
   int acc_e = 0, sp = 0;
-  for (int it = 0; it < n_ops; ++it) {
-    const int4 op = prog[it];
+  for_each_entry(prog, n_ops, [&](int, const int4 op) {
     real PA[16], PB[16];
     load_mat<real, R>(Pm + ((size_t)op.y * R + r) * 16, PA);
     load_mat<real, R>(Pm + ((size_t)op.z * R + r) * 16, PB);
@@ -346,7 +301,7 @@ __device__ __forceinline__ void prune_down_body(
       for (int k = 0; k < 4; ++k) d[(size_t)k * pitch] = acc[k];
       ex[((size_t)(op.x - N) * R + r) * pitch + p] = acc_e;
     }
-  }
+  });
   // root: S_r = sum_i pi_i L[r][i]; combine the R classes in wave 0
   double S = 0.0;
 #pragma unroll
@@ -435,18 +390,6 @@ __device__ __forceinline__ void rescale_pattern(real* v, int* e) {
   }
 }
 
-// element j of the matrix block staged for node entry op: matrices ordered
-// (which, child) = P_a, P_b, P'_a, P'_b, P''_a, P''_b, R x 16 reals each
-template <typename real, int R>
-__device__ __forceinline__ real stage_elem(const int4 op, int j, const real* __restrict__ m0,
-                                           const real* __restrict__ m1, const real* __restrict__ m2) {
-  const int mat = j / (R * 16), rem = j % (R * 16);
-  const int which = mat >> 1, c = mat & 1;
-  const int child = c ? op.z : op.y;
-  const real* base = which == 0 ? m0 : (which == 1 ? m1 : m2);
-  return base[(size_t)child * R * 16 + rem];
-}
-
 // compile-time loop: f(integral_constant<int, I>) for I in [B, E)
 template <int B, int E, class F>
 __device__ __forceinline__ void static_for(F&& f) {
@@ -461,46 +404,85 @@ __device__ __forceinline__ void static_for(F&& f) {
 //   A_c = M_p (*) (P_s D_s),  M_root = pi,  M_c = A_c^T P_c
 //   L = sum_r prob_r A_c . (P_c D_c),  L' and L'' with P' and P''
 // per-pattern g = L'/L and h = L''/L - g^2, summed per branch over the wave.
-// Same thread mapping as the post-order pass; P, P', P'' of both children
-// are staged in LDS one node ahead.
+//
+// Everything runs in the eigenbasis of Q = U diag(lam) Ui: with
+// e_k = exp(lam_k rate_r t_c), u = Ui D_c, w = e (*) u and g = U^T A_c,
+//   P_c D_c = U w,  A_c . (P^(n) D_c) = sum_k g_k w_k (lam_k rate_r)^n,
+//   M_c = Ui^T (e (*) g),
+// so a lane needs only the 4 factors e of each child branch per node; U and
+// Ui are wave-uniform.  Threads are (pattern, rate) quads as in the node
+// kernels: lane p*R + r, rate sums by DPP, one exponent per pattern.
 // ---------------------------------------------------------------------------
-constexpr int kPDD = 4;  // prefetch distance (nodes) of the derivative pass
+
+// exp(lam_k rate_r t_b) of nb branches (1 where rate_r t_b < 1e-10, the
+// identity rule of lib/mlmodel.c:332-341), plus real copies of U, Ui and
+// lam_k rate_r: tab = [U 16 | Ui 16 | lr R*4]
+template <typename real>
+__global__ void k_eigen_exp(const double* __restrict__ cst, int R, const double* __restrict__ d_t,
+                            int nb, real* __restrict__ E, real* __restrict__ tab) {
+  constexpr int K = 4;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double* lam = cst + cst_lam(K);
+  if (idx < 32 + 4 * R) {
+    double v;
+    if (idx < 16) v = cst[cst_U(K) + idx];
+    else if (idx < 32) v = cst[cst_Ui(K) + idx - 16];
+    else v = lam[(idx - 32) % 4] * cst[cst_rates(K) + (idx - 32) / 4];
+    tab[idx] = (real)v;
+  }
+  if (idx >= (int64_t)nb * R * K) return;
+  const int b = (int)(idx / (R * K)), r = (int)(idx / K) % R, k = (int)(idx % K);
+  const double rt = cst[cst_rates(K) + r] * d_t[b];
+  E[idx] = (real)((rt >= 1e-10) ? exp(lam[k] * rt) : 1.0);
+}
+
+template <typename real>
+__device__ __forceinline__ void matvec_u(const real* __restrict__ A, const real* x, real* y) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    y[i] = fma(A[4 * i], x[0], fma(A[4 * i + 1], x[1], fma(A[4 * i + 2], x[2], A[4 * i + 3] * x[3])));
+}
+template <typename real>
+__device__ __forceinline__ void vecmat_u(const real* __restrict__ A, const real* x, real* y) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    y[j] = fma(A[j], x[0], fma(A[4 + j], x[1], fma(A[8 + j], x[2], A[12 + j] * x[3])));
+}
 
 template <typename real, int R>
 __global__ __launch_bounds__(256) void k_prune_deriv(
     const int4* __restrict__ prog, int n_ops, const uint64_t* __restrict__ tips, int64_t stride,
-    int64_t n_pat, int64_t pitch, const real* __restrict__ Pm, const real* __restrict__ P1m,
-    const real* __restrict__ P2m, const double* __restrict__ pi, const double* __restrict__ probs,
-    const real* __restrict__ part, const int32_t* __restrict__ ex,
-    const uint32_t* __restrict__ weights, double* __restrict__ wave_sums, int N) {
-  constexpr int NM = 6 * R * 16;
-  constexpr int PF = (NM + 127) / 128;  // blocks of >= 128 threads
+    int64_t n_pat, int64_t pitch, const real* __restrict__ tab, const real* __restrict__ E,
+    const double* __restrict__ pi, const double* __restrict__ probs, const real* __restrict__ part,
+    const int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
+    double* __restrict__ wsum, int64_t n_waves, int N) {
   extern __shared__ unsigned char lds_raw[];
-  real* Pbuf = reinterpret_cast<real*>(lds_raw);     // [2][NM]
-  real* stk = Pbuf + 2 * NM;                          // [depth][4][bd]
+  real* stk = reinterpret_cast<real*>(lds_raw);  // [depth][4][bd]
   const int tid = threadIdx.x, bd = blockDim.x;
   const int r = tid % R;
   const int64_t p = (int64_t)blockIdx.x * (bd / R) + tid / R;
   const bool act = p < n_pat;
   const int64_t pc = act ? p : 0;
-  const int64_t rowlen = pitch * R;
   const double wp = act ? (double)weights[p] : 0.0;
-  const int M = 2 * N - 1;
   const int64_t wave = ((int64_t)blockIdx.x * bd + tid) >> 6;
   const int lane = tid & 63;
 
-  real pf[kPDD][PF];
-  auto load_slot = [&](real* dst, int node) {
-    const int4 on = prog[node];
+  real U[16], Ui[16], l1[4], l2[4];
 #pragma unroll
-    for (int k = 0; k < PF; ++k) {
-      const int j = tid + k * bd;
-      if (j < NM) dst[k] = stage_elem<real, R>(on, j, Pm, P1m, P2m);
-    }
-  };
-  // children partials (4 values each) of an entry, fetched raw one node ahead
-  // (each rate class has its own exponent in the post-order pass) ...
-  auto fetch_D = [&](int c, real* Dv, int* e) {
+  for (int k = 0; k < 16; ++k) {
+    U[k] = tab[k];
+    Ui[k] = tab[16 + k];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    l1[k] = tab[32 + 4 * r + k];
+    l2[k] = l1[k] * l1[k];
+  }
+  const double pr = probs[r];
+
+  // children partials (4 values + exponent each) and eigen factors of an
+  // entry, fetched raw one node ahead
+  auto fetch = [&](int c, real* Dv, int* e, real* ev) {
     if (c < N) {
       tipvec(tip_code(tips, stride, c, pc), Dv);
       *e = 0;
@@ -510,9 +492,12 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
       for (int k = 0; k < 4; ++k) Dv[k] = sv[(size_t)k * pitch];
       *e = ex[((size_t)(c - N) * R + r) * pitch + pc];
     }
+    const real* es = E + ((size_t)c * R + r) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ev[k] = es[k];
   };
-  // ... and aligned at use to the largest exponent of the pattern's classes
-  // (exact power-of-two scaling), so the R classes share one scale here
+  // the R classes of a pattern were scaled separately in the post-order
+  // pass: bring them to the largest exponent (exact power-of-two scaling)
   auto align_D = [&](real* Dv, int e) {
     int m = e;
     if (R >= 2) m = max(m, __builtin_amdgcn_update_dpp(0, m, DPP_XOR1, 0xF, 0xF, false));
@@ -523,49 +508,33 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
       for (int k = 0; k < 4; ++k) Dv[k] = ldexp(Dv[k], e - m);
     }
   };
-  for (int j = tid; j < NM; j += bd) Pbuf[j] = stage_elem<real, R>(prog[0], j, Pm, P1m, P2m);
-  static_for<1, kPDD>([&](auto U) {
-    constexpr int u = decltype(U)::value;
-    if (u < n_ops) load_slot(pf[u], u);
-  });
-  real Dn[2][8];  // children partials of the current / next entry
-  int En[2][2];
+
+  real Dn[2][8], En[2][8];
+  int Xn[2][2];
   {
     const int4 o = prog[0];
-    fetch_D(o.y, Dn[0], &En[0][0]);
-    fetch_D(o.z, Dn[0] + 4, &En[0][1]);
+    fetch(o.y, Dn[0], &Xn[0][0], En[0]);
+    fetch(o.z, Dn[0] + 4, &Xn[0][1], En[0] + 4);
   }
-  __syncthreads();
-
   real accM[4] = {(real)0, (real)0, (real)0, (real)0};
   int sp = 0;
-  for (int base = 0; base < n_ops; base += kPDD) {
-    static_for<0, kPDD>([&](auto U) {
-      constexpr int u = decltype(U)::value;
-      constexpr int cur = u & 1, nxt = (u + 1) & 1;
-      const int it = base + u;
-      if (it >= n_ops) return;
-      const int4 op = prog[it];
-      if (it + kPDD < n_ops) load_slot(pf[u], it + kPDD);
-      if (it + 1 < n_ops) {
-        const int4 on = prog[it + 1];
-        fetch_D(on.y, Dn[nxt], &En[nxt][0]);
-        fetch_D(on.z, Dn[nxt] + 4, &En[nxt][1]);
+  for (int it = 0; it < n_ops; it += 2) {
+    static_for<0, 2>([&](auto U_) {
+      constexpr int cur = decltype(U_)::value, nxt = cur ^ 1;
+      const int i = it + cur;
+      if (i >= n_ops) return;
+      const int4 op = prog[i];
+      if (i + 1 < n_ops) {
+        const int4 on = prog[i + 1];
+        fetch(on.y, Dn[nxt], &Xn[nxt][0], En[nxt]);
+        fetch(on.z, Dn[nxt] + 4, &Xn[nxt][1], En[nxt] + 4);
       }
-      const real* Pc = Pbuf + (u & 1) * NM;
-      // matrices (which, child): P_a, P_b, P'_a, P'_b, P''_a, P''_b
-      const real* Pa = Pc + (0 * R + r) * 16;
-      const real* Pb = Pc + (1 * R + r) * 16;
-      const real* P1a = Pc + (2 * R + r) * 16;
-      const real* P1b = Pc + (3 * R + r) * 16;
-      const real* P2a = Pc + (4 * R + r) * 16;
-      const real* P2b = Pc + (5 * R + r) * 16;
       const int fl = op.w, src = fl & 3;
       const int a = op.y, b = op.z;
       real Mv[4];
       if (src == SRC_LEAF) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) Mv[i] = (real)pi[i];
+        for (int k = 0; k < 4; ++k) Mv[k] = (real)pi[k];
       } else if (src == SRC_ACC) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) Mv[k] = accM[k];
@@ -576,65 +545,75 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
       }
       real* Da = Dn[cur];
       real* Db = Dn[cur] + 4;
-      align_D(Da, En[cur][0]);
-      align_D(Db, En[cur][1]);
-      real ya[4], yb[4], x1a[4], x2a[4], x1b[4], x2b[4];
-      matvec4(Pa, Da, ya);
-      matvec4(Pb, Db, yb);
-      matvec4(P1a, Da, x1a);
-      matvec4(P2a, Da, x2a);
-      matvec4(P1b, Db, x1b);
-      matvec4(P2b, Db, x2b);
-      real s0 = 0, s1a = 0, s2a = 0, s1b = 0, s2b = 0;
+      const real* ea = En[cur];
+      const real* eb = En[cur] + 4;
+      align_D(Da, Xn[cur][0]);
+      align_D(Db, Xn[cur][1]);
+      real ua[4], ub[4], wa[4], wb[4], ya[4], yb[4], Aa[4], Ab[4], ga[4], gb[4];
+      matvec_u(Ui, Da, ua);
+      matvec_u(Ui, Db, ub);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const real ma = Mv[i] * yb[i];  // A_a
-        const real mb = Mv[i] * ya[i];  // A_b
-        s0 = fma(ma, ya[i], s0);
-        s1a = fma(ma, x1a[i], s1a);
-        s2a = fma(ma, x2a[i], s2a);
-        s1b = fma(mb, x1b[i], s1b);
-        s2b = fma(mb, x2b[i], s2b);
+      for (int k = 0; k < 4; ++k) {
+        wa[k] = ea[k] * ua[k];
+        wb[k] = eb[k] * ub[k];
       }
-      const double pr = probs[r];
-      const double l0 = rate_sum<R>(pr * (double)s0);
-      const double l1a = rate_sum<R>(pr * (double)s1a), l2a = rate_sum<R>(pr * (double)s2a);
-      const double l1b = rate_sum<R>(pr * (double)s1b), l2b = rate_sum<R>(pr * (double)s2b);
-      double ga = 0, ha = 0, gb = 0, hb = 0;
+      matvec_u(U, wa, ya);
+      matvec_u(U, wb, yb);
+      real s0 = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        Aa[k] = Mv[k] * yb[k];
+        Ab[k] = Mv[k] * ya[k];
+        s0 = fma(Aa[k], ya[k], s0);
+      }
+      vecmat_u(U, Aa, ga);
+      vecmat_u(U, Ab, gb);
+      real s1a = 0, s2a = 0, s1b = 0, s2b = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const real qa = ga[k] * wa[k], qb = gb[k] * wb[k];
+        s1a = fma(qa, l1[k], s1a);
+        s2a = fma(qa, l2[k], s2a);
+        s1b = fma(qb, l1[k], s1b);
+        s2b = fma(qb, l2[k], s2b);
+      }
+      const double L0 = rate_sum<R>(pr * (double)s0);
+      const double L1a = rate_sum<R>(pr * (double)s1a), L2a = rate_sum<R>(pr * (double)s2a);
+      const double L1b = rate_sum<R>(pr * (double)s1b), L2b = rate_sum<R>(pr * (double)s2b);
+      double dga = 0, dha = 0, dgb = 0, dhb = 0;
       if (act && r == 0) {
-        const double inv = 1.0 / l0;
-        const double g1 = l1a * inv, g2 = l1b * inv;
-        ga = wp * g1;
-        ha = wp * (l2a * inv - g1 * g1);
-        gb = wp * g2;
-        hb = wp * (l2b * inv - g2 * g2);
+        const double inv = 1.0 / L0;
+        const double g1 = L1a * inv, g2 = L1b * inv;
+        dga = wp * g1;
+        dha = wp * (L2a * inv - g1 * g1);
+        dgb = wp * g2;
+        dhb = wp * (L2b * inv - g2 * g2);
       }
-      ga = wave_sum_d(ga);
-      ha = wave_sum_d(ha);
-      gb = wave_sum_d(gb);
-      hb = wave_sum_d(hb);
+      dga = wave_sum_d(dga);
+      dha = wave_sum_d(dha);
+      dgb = wave_sum_d(dgb);
+      dhb = wave_sum_d(dhb);
       if (lane == 0) {
-        double* ws = wave_sums + (size_t)wave * M * 2;
-        ws[2 * a] = ga;
-        ws[2 * a + 1] = ha;
-        ws[2 * b] = gb;
-        ws[2 * b + 1] = hb;
+        wsum[((size_t)a * 2) * n_waves + wave] = dga;
+        wsum[((size_t)a * 2 + 1) * n_waves + wave] = dha;
+        wsum[((size_t)b * 2) * n_waves + wave] = dgb;
+        wsum[((size_t)b * 2 + 1) * n_waves + wave] = dhb;
       }
-      // messages to internal children: M_c = (M (*) P_s D_s)^T P_c
+      // messages to internal children: M_c = Ui^T (e_c (*) g_c)
       real Ma[4], Mb[4];
       int dummy = 0;
       if (!(fl & UP_A_LEAF)) {
-        real Av[4];
+        real t[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) Av[i] = Mv[i] * yb[i];
-        vecmat4(Pa, Av, Ma);
+        for (int k = 0; k < 4; ++k) t[k] = ea[k] * ga[k];
+        vecmat_u(Ui, t, Ma);
         rescale_pattern<real, R>(Ma, &dummy);
       }
       if (!(fl & UP_B_LEAF)) {
-        real Av[4];
+        real t[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) Av[i] = Mv[i] * ya[i];
-        vecmat4(Pb, Av, Mb);
+        for (int k = 0; k < 4; ++k) t[k] = eb[k] * gb[k];
+        vecmat_u(Ui, t, Mb);
         rescale_pattern<real, R>(Mb, &dummy);
       }
       if (fl & UP_PUSH) {
@@ -648,15 +627,6 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
 #pragma unroll
         for (int k = 0; k < 4; ++k) accM[k] = c[k];
       }
-      if (it + 1 < n_ops) {
-        constexpr int nu = (u + 1) % kPDD;
-#pragma unroll
-        for (int k = 0; k < PF; ++k) {
-          const int j = tid + k * bd;
-          if (j < NM) Pbuf[((u + 1) & 1) * NM + j] = pf[nu][k];
-        }
-      }
-      __syncthreads();
     });
   }
 }
@@ -719,13 +689,16 @@ __global__ void k_reduce_rows(const double* __restrict__ x, int64_t n, double* _
 // root child takes the first one's values (one unrooted edge).
 __global__ void k_reduce_derivs(const double* __restrict__ ws, int64_t n_waves, int M, int root,
                                 int ra, int rb, double* __restrict__ d1, double* __restrict__ d2) {
+  // ws[(edge * 2 + q) * n_waves + wave]: q = 0 first, 1 second derivative
   const int c = blockIdx.x;
   __shared__ double s1[256], c1[256], s2[256], c2[256];
   const int src = (c == rb) ? ra : c;
   double a = 0, ac = 0, b = 0, bc = 0;
   if (c != root) {
+    const double* w1 = ws + (size_t)src * 2 * n_waves;
+    const double* w2 = w1 + n_waves;
     for (int64_t w = threadIdx.x; w < n_waves; w += blockDim.x) {
-      const double v1 = ws[((size_t)w * M + src) * 2], v2 = ws[((size_t)w * M + src) * 2 + 1];
+      const double v1 = w1[w], v2 = w2[w];
       double t = a + v1;
       if (fabs(a) >= fabs(v1)) ac += (a - t) + v1; else ac += (v1 - t) + a;
       a = t;
@@ -836,7 +809,7 @@ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Workspace plan of phy_lk_tree
 struct Plan {
-  size_t off_P = 0, off_P1 = 0, off_P2 = 0, off_part = 0, off_ex = 0, off_bsum = 0, off_wsum = 0;
+  size_t off_P = 0, off_E = 0, off_tab = 0, off_part = 0, off_ex = 0, off_bsum = 0, off_wsum = 0;
   size_t total = 0;
   int64_t pitch = 0, n_blocks = 0, n_waves = 0, dblocks = 0, n_dwaves = 0;
   int bs = 0, dbs = 0;  // block size of the post-order pass / derivative pass
@@ -849,11 +822,11 @@ constexpr size_t kLdsMax = 64 * 1024;
 // LDS of the post-order pass: root exchange, tip ring, (fused) stack
 size_t down_lds(int R, size_t esz, bool fused, int depth) {
   const size_t bs = 64 * (size_t)R;
-  return (size_t)R * 64 * 12 + (size_t)kTipBatch * bs * 4 +
+  return (size_t)R * 64 * 12 + (size_t)2 * kTipBatch * bs * 4 +
          (fused ? (size_t)std::max(depth, 1) * bs * (4 * esz + 4) : 0);
 }
 size_t deriv_lds(int R, size_t esz, int bs, int depth) {
-  return (size_t)2 * 6 * R * 16 * esz + (size_t)std::max(depth, 1) * bs * 4 * esz;
+  return (size_t)std::max(depth, 1) * bs * 4 * esz;
 }
 
 // largest block size <= want whose LDS fits (0 if none)
@@ -877,8 +850,8 @@ Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t for
   pl.fused = fused && down_lds(R, esz, true, depth) <= kLdsMax;
   pl.lds = down_lds(R, esz, pl.fused, depth);
   if (derivs) {
-    pl.dbs = fit_block(std::max(128, env_block("PHY_PRUNE_DERIV_BLOCK", 256)),
-                       [&](int bs) { return deriv_lds(R, esz, bs, t->up_depth); }, 128);
+    pl.dbs = fit_block(env_block("PHY_PRUNE_DERIV_BLOCK", 256),
+                       [&](int bs) { return deriv_lds(R, esz, bs, t->up_depth); });
     pl.dlds = pl.dbs ? deriv_lds(R, esz, pl.dbs, t->up_depth) : 0;
   }
   pl.pitch = ((n_pat + 255) / 256) * 256;
@@ -891,8 +864,8 @@ Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t for
   const size_t pm = (size_t)M * R * K * K * pl.esz;
   pl.off_P = o; o = align256(o + pm);
   if (derivs) {
-    pl.off_P1 = o; o = align256(o + pm);
-    pl.off_P2 = o; o = align256(o + pm);
+    pl.off_E = o; o = align256(o + (size_t)M * R * K * pl.esz);
+    pl.off_tab = o; o = align256(o + (size_t)(2 * K * K + R * K) * pl.esz);
   }
   if (!pl.fused) {
     pl.off_part = o; o = align256(o + (size_t)(N - 1) * R * K * pl.pitch * pl.esz);
@@ -916,14 +889,12 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
   const double* d_pi = cst + 2 * K * K + K + R;
   const double* d_probs = d_pi + K;
   real* P = reinterpret_cast<real*>(work + pl.off_P);
-  real* P1 = derivs ? reinterpret_cast<real*>(work + pl.off_P1) : nullptr;
-  real* P2 = derivs ? reinterpret_cast<real*>(work + pl.off_P2) : nullptr;
   if (derivs && !pl.dbs) return set_err(ctx, PHY_ERANGE, "pruning: derivative LDS stack too deep");
   fused = pl.fused;
   {
     const int64_t n = (int64_t)M * R * K * K;
     hipLaunchKernelGGL((k_pmats<real>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                       cst, K, R, blen, 0.0, 0.0, M, P, P1, P2);
+                       cst, K, R, blen, 0.0, 0.0, M, P, (real*)nullptr, (real*)nullptr);
     PHY_LAUNCHED(ctx);
   }
   real* part = fused ? nullptr : reinterpret_cast<real*>(work + pl.off_part);
@@ -942,9 +913,15 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
   PHY_LAUNCHED(ctx);
   if (derivs) {
     double* wsum = reinterpret_cast<double*>(work + pl.off_wsum);
+    real* E = reinterpret_cast<real*>(work + pl.off_E);
+    real* tab = reinterpret_cast<real*>(work + pl.off_tab);
+    const int64_t ne = std::max<int64_t>((int64_t)M * R * K, 32 + 4 * R);
+    hipLaunchKernelGGL((k_eigen_exp<real>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0,
+                       ctx->stream, cst, R, blen, M, E, tab);
+    PHY_LAUNCHED(ctx);
     hipLaunchKernelGGL((k_prune_deriv<real, R>), dim3((unsigned)pl.dblocks), dim3(pl.dbs), pl.dlds,
-                       ctx->stream, t->d_up_prog, N - 1, tips, stride, n_pat, pl.pitch, P, P1, P2,
-                       d_pi, d_probs, part, ex, weights, wsum, N);
+                       ctx->stream, t->d_up_prog, N - 1, tips, stride, n_pat, pl.pitch, tab, E,
+                       d_pi, d_probs, part, ex, weights, wsum, pl.n_dwaves, N);
     PHY_LAUNCHED(ctx);
     const int ra = t->children[2 * (t->root - N)], rb = t->children[2 * (t->root - N) + 1];
     hipLaunchKernelGGL(k_reduce_derivs, dim3((unsigned)M), dim3(256), 0, ctx->stream, wsum,
