@@ -30,6 +30,7 @@
 
 #include "common.hpp"
 #include "model_host.hpp"
+#include "stream.hpp"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -133,8 +134,10 @@ __device__ __forceinline__ bf16x8 tip_operand(uint32_t code, int q) {
 template <int WAVES, int R>
 __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
   constexpr int NT = 64 * WAVES;
-  constexpr int STAGE = 2 * R * 2 * OPB;  // uint4
-  __shared__ uint4 stage[STAGE];
+  constexpr int STAGE = 2 * R * 2 * OPB;  // uint4 per node: [c][r][blk][lane]
+  constexpr int PF = STAGE / NT;          // this thread's share
+  static_assert(STAGE % NT == 0, "stage split");
+  __shared__ uint4 stage[2][STAGE];       // double buffer: node i and node i+1
   __shared__ int s_item;
   extern __shared__ uint4 dyn[];
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, q = l >> 4, pl = l & 15;
@@ -164,17 +167,41 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
 #pragma unroll
       for (int v = 0; v < 8; ++v) acc[r][v] = 0.f;
 
-    for (int i = 0; i < N - 1; ++i) {
-      const int4 op = prog[i];
-      // stage the operands of both children: [c][r][blk][lane]
-      __syncthreads();
-      for (int e = tid; e < STAGE; e += NT) {
+    // operands of a node's two children, this thread's share, into registers
+    uint4 pf[PF];
+    auto fetch_stage = [&](const int4 o) {
+#pragma unroll
+      for (int k = 0; k < PF; ++k) {
+        const int e = tid + k * NT;
         const int c = e / (R * 2 * OPB);
         const int rem = e % (R * 2 * OPB);
-        const int child = c ? op.z : op.y;
-        stage[e] = pops[(size_t)child * R * 2 * OPB + rem];
+        const int child = c ? o.z : o.y;
+        pf[k] = pops[(size_t)child * R * 2 * OPB + rem];
       }
-      __syncthreads();
+    };
+    auto park_stage = [&](int buf) {
+#pragma unroll
+      for (int k = 0; k < PF; ++k) stage[buf][tid + k * NT] = pf[k];
+    };
+    // tip codes of the next two leaves of the stream (the fifo is padded)
+    auto tip_at = [&](int k) -> uint32_t { return tips[(size_t)fifo[k] * pitch + p]; };
+    uint32_t t0 = tip_at(0), t1 = tip_at(1);
+    auto next_tip = [&]() -> uint32_t {
+      const uint32_t v = t0;
+      t0 = t1;
+      ++fpos;
+      t1 = tip_at(fpos + 1);
+      return v;
+    };
+    fetch_stage(prog[0]);
+    park_stage(0);
+    __syncthreads();
+
+    for_each_entry_ahead(prog, N - 1, [&](int i, const int4 op, const int4 on) {
+      const int buf = i & 1;
+      const bool more = i + 1 < N - 1;
+      if (more) fetch_stage(on);  // in flight while node i computes
+      const uint4* cur = stage[buf];
       const int fl = op.w;
       if (fl & DOWN_SPILL) {
         uint4* s = stk + (size_t)sp * (R * 64 + 16);
@@ -194,9 +221,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
         bf16x8 bop[R];
         int e = 0;
         if (src == SRC_LEAF) {
-          const int leaf = fifo[fpos++];
-          const uint32_t code = tips[(size_t)leaf * pitch + p];
-          const bf16x8 tb = tip_operand(code, q);
+          const bf16x8 tb = tip_operand(next_tip(), q);
 #pragma unroll
           for (int r = 0; r < R; ++r) bop[r] = tb;
         } else if (src == SRC_ACC) {
@@ -216,8 +241,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           f32x4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
-          const uint4 a0 = stage[((c * R + r) * 2 + 0) * OPB + l];
-          const uint4 a1 = stage[((c * R + r) * 2 + 1) * OPB + l];
+          const uint4 a0 = cur[((c * R + r) * 2 + 0) * OPB + l];
+          const uint4 a1 = cur[((c * R + r) * 2 + 1) * OPB + l];
           d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8*>(&a0), bop[r], d0, 0, 0, 0);
           d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8*>(&a1), bop[r], d1, 0, 0, 0);
           if (c == 0) {
@@ -254,7 +279,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
           for (int v = 0; v < 8; ++v) acc[r][v] = ldexpf(acc[r][v], -ex);
         acc_e += ex;
       }
-    }
+      if (more) park_stage(buf ^ 1);
+      __syncthreads();
+    });
     // root: L_p = sum_r prob_r sum_i pi_i L[r][i]
     const double* pi = a.pi + (size_t)t * KAA;
     const double* pr = a.probs + (size_t)t * R;
@@ -447,6 +474,7 @@ int phy_aa_batch_create(phy_ctx* ctx, int32_t T, const int32_t* n_leaves, const 
                                topo->down_prog[k + 3]));
     fifo_off.push_back((int64_t)fifo.size());
     fifo.insert(fifo.end(), topo->down_fifo.begin(), topo->down_fifo.end());
+    fifo.insert(fifo.end(), 2, 0);  // the kernel reads two ids ahead
     phy_topo_destroy(topo);
     nl.push_back(N);
     const int64_t pp = (P + tile - 1) / tile * tile;
@@ -501,7 +529,8 @@ int phy_aa_batch_create(phy_ctx* ctx, int32_t T, const int32_t* n_leaves, const 
     delete b;
     return set_err(ctx, PHY_ERANGE, "aa_batch: too many work items");
   }
-  if (lds_bytes(waves, R, std::max(1, b->depth)) > 64 * 1024) {
+  // static operand double buffer (2 x 2R x 2 KiB) plus the dynamic stack
+  if (lds_bytes(waves, R, std::max(1, b->depth)) + (size_t)2 * 2 * R * 2 * OPB * sizeof(uint4) > 64 * 1024) {
     delete b;
     return set_err(ctx, PHY_ERANGE, "aa_batch: LDS stack too deep for this many waves per workgroup");
   }

@@ -509,25 +509,22 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
     }
   };
 
-  real Dn[2][8], En[2][8];
-  int Xn[2][2];
+  real Dc[8], Ec[8];  // children partials and eigen factors of the current entry
+  int Xc[2];
   {
     const int4 o = prog[0];
-    fetch(o.y, Dn[0], &Xn[0][0], En[0]);
-    fetch(o.z, Dn[0] + 4, &Xn[0][1], En[0] + 4);
+    fetch(o.y, Dc, &Xc[0], Ec);
+    fetch(o.z, Dc + 4, &Xc[1], Ec + 4);
   }
   real accM[4] = {(real)0, (real)0, (real)0, (real)0};
   int sp = 0;
-  for (int it = 0; it < n_ops; it += 2) {
-    static_for<0, 2>([&](auto U_) {
-      constexpr int cur = decltype(U_)::value, nxt = cur ^ 1;
-      const int i = it + cur;
-      if (i >= n_ops) return;
-      const int4 op = prog[i];
-      if (i + 1 < n_ops) {
-        const int4 on = prog[i + 1];
-        fetch(on.y, Dn[nxt], &Xn[nxt][0], En[nxt]);
-        fetch(on.z, Dn[nxt] + 4, &Xn[nxt][1], En[nxt] + 4);
+  for_each_entry_ahead(prog, n_ops, [&](int i, const int4 op, const int4 on) {
+      real Dn[8], En[8];
+      int Xn[2] = {0, 0};
+      const bool more = i + 1 < n_ops;
+      if (more) {
+        fetch(on.y, Dn, &Xn[0], En);
+        fetch(on.z, Dn + 4, &Xn[1], En + 4);
       }
       const int fl = op.w, src = fl & 3;
       const int a = op.y, b = op.z;
@@ -543,12 +540,12 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
 #pragma unroll
         for (int k = 0; k < 4; ++k) Mv[k] = stk[((size_t)sp * 4 + k) * bd + tid];
       }
-      real* Da = Dn[cur];
-      real* Db = Dn[cur] + 4;
-      const real* ea = En[cur];
-      const real* eb = En[cur] + 4;
-      align_D(Da, Xn[cur][0]);
-      align_D(Db, Xn[cur][1]);
+      real* Da = Dc;
+      real* Db = Dc + 4;
+      const real* ea = Ec;
+      const real* eb = Ec + 4;
+      align_D(Da, Xc[0]);
+      align_D(Db, Xc[1]);
       real ua[4], ub[4], wa[4], wb[4], ya[4], yb[4], Aa[4], Ab[4], ga[4], gb[4];
       matvec_u(Ui, Da, ua);
       matvec_u(Ui, Db, ub);
@@ -627,8 +624,16 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
 #pragma unroll
         for (int k = 0; k < 4; ++k) accM[k] = c[k];
       }
-    });
-  }
+      if (more) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          Dc[k] = Dn[k];
+          Ec[k] = En[k];
+        }
+        Xc[0] = Xn[0];
+        Xc[1] = Xn[1];
+      }
+  });
 }
 
 // Neumaier sum of n doubles with one block (deterministic order).

@@ -40,6 +40,40 @@ __device__ __forceinline__ void for_each_entry(const int4* __restrict__ prog, in
   }
 }
 
+// Same, passing the following entry too (equal to the last entry at the end),
+// for passes that prefetch operands one node ahead.
+template <class F>
+__device__ __forceinline__ void for_each_entry_ahead(const int4* __restrict__ prog, int n_ops, F&& f) {
+  if (n_ops <= 0) return;
+  const int lane = threadIdx.x & 63;
+  int4 cur = prog[min(lane, n_ops - 1)];
+  int4 nxt = prog[min(64 + lane, n_ops - 1)];
+  for (int base = 0; base < n_ops; base += 64) {
+    const int cnt = min(64, n_ops - base);
+    for (int j = 0; j < cnt; ++j) {
+      int4 op, on;
+      op.x = __builtin_amdgcn_readlane(cur.x, j);
+      op.y = __builtin_amdgcn_readlane(cur.y, j);
+      op.z = __builtin_amdgcn_readlane(cur.z, j);
+      op.w = __builtin_amdgcn_readlane(cur.w, j);
+      if (j + 1 < 64) {
+        on.x = __builtin_amdgcn_readlane(cur.x, j + 1);
+        on.y = __builtin_amdgcn_readlane(cur.y, j + 1);
+        on.z = __builtin_amdgcn_readlane(cur.z, j + 1);
+        on.w = __builtin_amdgcn_readlane(cur.w, j + 1);
+      } else {
+        on.x = __builtin_amdgcn_readlane(nxt.x, 0);
+        on.y = __builtin_amdgcn_readlane(nxt.y, 0);
+        on.z = __builtin_amdgcn_readlane(nxt.z, 0);
+        on.w = __builtin_amdgcn_readlane(nxt.w, 0);
+      }
+      f(base + j, op, on);
+    }
+    cur = nxt;
+    nxt = prog[min(base + 128 + lane, n_ops - 1)];
+  }
+}
+
 // Loader: uint32_t operator()(int id) const, an in-bounds load for every id
 // of the padded stream.
 template <int B, class Loader>
