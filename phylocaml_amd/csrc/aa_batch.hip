@@ -135,8 +135,7 @@ template <int WAVES, int R>
 __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
   constexpr int NT = 64 * WAVES;
   constexpr int STAGE = 2 * R * 2 * OPB;  // uint4 per node: [c][r][blk][lane]
-  constexpr int PF = STAGE / NT;          // this thread's share
-  static_assert(STAGE % NT == 0, "stage split");
+  constexpr int NBLK = 2 * R * 2;         // 1 KiB operand blocks per node
   __shared__ uint4 stage[2][STAGE];       // double buffer: node i and node i+1
   __shared__ int s_item;
   extern __shared__ uint4 dyn[];
@@ -167,21 +166,17 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
 #pragma unroll
       for (int v = 0; v < 8; ++v) acc[r][v] = 0.f;
 
-    // operands of a node's two children, this thread's share, into registers
-    uint4 pf[PF];
-    auto fetch_stage = [&](const int4 o) {
-#pragma unroll
-      for (int k = 0; k < PF; ++k) {
-        const int e = tid + k * NT;
-        const int c = e / (R * 2 * OPB);
-        const int rem = e % (R * 2 * OPB);
+    // operands of a node's two children straight into LDS (global_load_lds,
+    // 16 B per lane: one 1 KiB block per wave instruction), no registers
+    auto dma_stage = [&](const int4 o, int buf) {
+      for (int bk = wave; bk < NBLK; bk += WAVES) {
+        const int c = bk / (R * 2), rem = bk % (R * 2);
         const int child = c ? o.z : o.y;
-        pf[k] = pops[(size_t)child * R * 2 * OPB + rem];
+        const uint4* src = pops + ((size_t)child * R * 2 + rem) * OPB + l;
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                         (__attribute__((address_space(3))) void*)(&stage[buf][bk * OPB]),
+                                         16, 0, 0);
       }
-    };
-    auto park_stage = [&](int buf) {
-#pragma unroll
-      for (int k = 0; k < PF; ++k) stage[buf][tid + k * NT] = pf[k];
     };
     // tip codes of the next two leaves of the stream (the fifo is padded)
     auto tip_at = [&](int k) -> uint32_t { return tips[(size_t)fifo[k] * pitch + p]; };
@@ -193,14 +188,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
       t1 = tip_at(fpos + 1);
       return v;
     };
-    fetch_stage(prog[0]);
-    park_stage(0);
+    dma_stage(prog[0], 0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the DMA has landed
     __syncthreads();
 
     for_each_entry_ahead(prog, N - 1, [&](int i, const int4 op, const int4 on) {
       const int buf = i & 1;
       const bool more = i + 1 < N - 1;
-      if (more) fetch_stage(on);  // in flight while node i computes
+      if (more) dma_stage(on, buf ^ 1);  // in flight while node i computes
       const uint4* cur = stage[buf];
       const int fl = op.w;
       if (fl & DOWN_SPILL) {
@@ -279,7 +274,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
           for (int v = 0; v < 8; ++v) acc[r][v] = ldexpf(acc[r][v], -ex);
         acc_e += ex;
       }
-      if (more) park_stage(buf ^ 1);
+      if (more) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): node i+1 operands landed
       __syncthreads();
     });
     // root: L_p = sum_r prob_r sum_i pi_i L[r][i]
@@ -529,8 +524,12 @@ int phy_aa_batch_create(phy_ctx* ctx, int32_t T, const int32_t* n_leaves, const 
     delete b;
     return set_err(ctx, PHY_ERANGE, "aa_batch: too many work items");
   }
-  // static operand double buffer (2 x 2R x 2 KiB) plus the dynamic stack
-  if (lds_bytes(waves, R, std::max(1, b->depth)) + (size_t)2 * 2 * R * 2 * OPB * sizeof(uint4) > 64 * 1024) {
+  // static operand double buffer (2 x 2R x 2 KiB) plus the dynamic stack,
+  // within the device's LDS per workgroup
+  int lds_max = 64 * 1024, dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  if (lds_bytes(waves, R, std::max(1, b->depth)) + (size_t)2 * 2 * R * 2 * OPB * sizeof(uint4) > (size_t)lds_max) {
     delete b;
     return set_err(ctx, PHY_ERANGE, "aa_batch: LDS stack too deep for this many waves per workgroup");
   }

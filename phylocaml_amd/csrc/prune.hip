@@ -185,13 +185,61 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
 }
 
-struct TipLoad {
-  const uint64_t* tips;
-  int64_t stride, pc;
-  __device__ __forceinline__ uint32_t operator()(int id) const { return tip_code(tips, stride, id, pc); }
-};
+// Tip codes of the post-order leaf stream, 8 per packed register: the loads
+// of batch k+2 are in flight while batch k is consumed by shifts (no LDS).
+// The id stream is padded (phy_fifo_padded): up to 24 ids past the last one
+// consumed are read.
+struct TipNibbles {
+  const uint32_t* tips32;  // the nibble blocks read as uint32 (8 sites each)
+  int64_t stride32;        // uint32 per tip row
+  int64_t col;             // uint32 column of this lane's pattern
+  int shift;               // 4 * (pattern % 8)
+  const int32_t* ids;
+  uint32_t pre[8];
+  uint32_t cur, nxt;
+  int pos, next_start;
 
-constexpr int kTipBatch = 8;
+  __device__ __forceinline__ void issue(int start) {
+    const int4* iv = reinterpret_cast<const int4*>(ids + start);
+    const int4 a = iv[0], b = iv[1];
+    const int id[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) pre[j] = tips32[(int64_t)id[j] * stride32 + col];
+  }
+  __device__ __forceinline__ uint32_t pack() const {
+    uint32_t v = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v |= ((pre[j] >> shift) & 15u) << (4 * j);
+    return v;
+  }
+  __device__ __forceinline__ void init(const uint64_t* tips, int64_t stride, int64_t pc,
+                                       const int32_t* id_stream) {
+    tips32 = reinterpret_cast<const uint32_t*>(tips);
+    stride32 = 2 * stride;
+    col = pc >> 3;
+    shift = 4 * (int)(pc & 7);
+    ids = id_stream;
+    issue(0);
+    cur = pack();
+    issue(8);
+    nxt = pack();
+    issue(16);
+    next_start = 16;
+    pos = 0;
+  }
+  __device__ __forceinline__ uint32_t get() {
+    const uint32_t v = (cur >> (4 * pos)) & 15u;
+    if (++pos == 8) {
+      cur = nxt;
+      nxt = pack();
+      next_start += 8;
+      issue(next_start);
+      pos = 0;
+    }
+    return v;
+  }
+};
+static_assert(24 + 8 <= (int)kFifoSlack, "tip stream look-ahead vs id padding");
 
 // ---------------------------------------------------------------------------
 // Whole-tree post-order pass.
@@ -238,16 +286,16 @@ __device__ __forceinline__ void prune_down_body(
   const bool act = p < n_pat;
   const int64_t pc = act ? p : 0;
 
-  // LDS: root exchange [R][64] (double S, int e) | tip ring [2 kTipBatch][bd] |
-  //      (fused) stack [depth][4][bd] real + [depth][bd] int
+  // LDS: (fused) stack [depth][4][bd] real + [depth][bd] int; at the root,
+  // when the stack is empty, the same bytes hold the rate exchange
+  // [R][64] double S + [R][64] int e
+  real* stk = reinterpret_cast<real*>(lds_raw);
+  int32_t* stke = reinterpret_cast<int32_t*>(stk + (size_t)(FUSED ? stack_depth : 0) * 4 * bd);
   double* rootS = reinterpret_cast<double*>(lds_raw);
   int* rootE = reinterpret_cast<int*>(rootS + R * 64);
-  uint32_t* ring = reinterpret_cast<uint32_t*>(rootE + R * 64);
-  real* stk = reinterpret_cast<real*>(ring + (size_t)2 * kTipBatch * bd);
-  int32_t* stke = reinterpret_cast<int32_t*>(stk + (size_t)(FUSED ? stack_depth : 0) * 4 * bd);
 
-  OperandRing<kTipBatch, TipLoad> ts;
-  ts.init(ring, leaf_ids, TipLoad{tips, stride, pc}, bd, tid);
+  TipNibbles ts;
+  ts.init(tips, stride, pc, leaf_ids);
 
   real acc[4] = {(real)0, (real)0, (real)0, (real)0};
   int acc_e = 0, sp = 0;
@@ -306,6 +354,7 @@ __device__ __forceinline__ void prune_down_body(
   double S = 0.0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) S += pi[i] * (double)acc[i];
+  __syncthreads();  // every wave is done with its stack rows
   rootS[r * 64 + lane] = S;
   rootE[r * 64 + lane] = acc_e;
   __syncthreads();
@@ -824,11 +873,12 @@ struct Plan {
 
 constexpr size_t kLdsMax = 64 * 1024;
 
-// LDS of the post-order pass: root exchange, tip ring, (fused) stack
+// LDS of the post-order pass: the (fused) stack, which also holds the rate
+// exchange at the root
 size_t down_lds(int R, size_t esz, bool fused, int depth) {
   const size_t bs = 64 * (size_t)R;
-  return (size_t)R * 64 * 12 + (size_t)2 * kTipBatch * bs * 4 +
-         (fused ? (size_t)std::max(depth, 1) * bs * (4 * esz + 4) : 0);
+  const size_t root = (size_t)R * 64 * 12;
+  return std::max(root, fused ? (size_t)depth * bs * (4 * esz + 4) : 0);
 }
 size_t deriv_lds(int R, size_t esz, int bs, int depth) {
   return (size_t)std::max(depth, 1) * bs * 4 * esz;
