@@ -15,16 +15,31 @@ struct phy_ctx {
   std::string err;
 };
 
-// Operand sources of a post-order (down) program entry.
-enum : int { SRC_LEAF = 0, SRC_ACC = 1, SRC_STACK = 2 };
+// Operand sources of a program entry (SRC_EXT: pre-order entry of a part,
+// whose parent's final set is read from memory).
+enum : int { SRC_LEAF = 0, SRC_ACC = 1, SRC_STACK = 2, SRC_EXT = 3 };
 // Flag bits of a down entry (int4.w): A source in bits 0-1, B source in bits
 // 2-3, bit 4 = push the accumulator to the LDS stack before the entry.
 enum : int { DOWN_SPILL = 16 };
 // Flag bits of an up entry (int4.w): input source in bits 0-1 (SRC_LEAF means
 // "root: read P from the fifo"), bit 2 = child a is a leaf, bit 3 = child b
 // is a leaf, bit 4 = child b (not a) continues in the accumulator, bit 5 =
-// push the other internal child.
-enum : int { UP_A_LEAF = 4, UP_B_LEAF = 8, UP_CONT_B = 16, UP_PUSH = 32 };
+// push the other internal child, bits 6/7 = child a/b is a cut node (its
+// final set is computed by the program of its own part).
+enum : int { UP_A_LEAF = 4, UP_B_LEAF = 8, UP_CONT_B = 16, UP_PUSH = 32, UP_A_CUT = 64, UP_B_CUT = 128 };
+
+// Programs of a tree split at cut nodes into parts that run side by side
+// (one launch, grid.y = part) plus a top program with the cut nodes as
+// leaves.  Down: parts 0..P-1, then the top at index P.  Up: the top at
+// index 0, then the parts.  Programs and padded fifos are concatenated.
+constexpr int kMaxParts = 32;  // the default is 8 (PHY_FITCH_PARTS at topology creation)
+struct SplitProg {
+  std::vector<int32_t> prog, fifo, prog_off, n_ops, fifo_off;
+  int32_t depth = 0;
+  int4* d_prog = nullptr;
+  int32_t* d_fifo = nullptr;
+  int32_t* d_meta = nullptr;  // prog_off | n_ops | fifo_off, parts + 1 each
+};
 
 struct phy_topo {
   phy_ctx* ctx = nullptr;
@@ -46,6 +61,13 @@ struct phy_topo {
   int32_t* d_down_fifo = nullptr;
   int4* d_up_prog = nullptr;
   int32_t* d_up_fifo = nullptr;
+  // split programs (empty parts: the tree is run whole)
+  std::vector<int32_t> parts;      // cut node of each part
+  SplitProg split_down, split_up;
+  int32_t* d_part_parent = nullptr;
+  // per-part partial site costs of the split down-pass, grown on demand
+  mutable uint32_t* d_scratch = nullptr;
+  mutable size_t scratch_bytes = 0;
 };
 
 inline int set_err(phy_ctx* ctx, int code, const std::string& msg) {

@@ -16,6 +16,7 @@
 //    child first and stacking the larger; each node's preliminary set is
 //    loaded exactly once, in a fixed order.
 #include <algorithm>
+#include <cstdlib>
 #include <functional>
 
 #include "common.hpp"
@@ -103,98 +104,70 @@ int phy_copy_d2h(phy_ctx* ctx, void* h_dst, const void* d_src, size_t bytes) {
 // ---------------------------------------------------------------------------
 namespace {
 
-struct Compiler {
-  int32_t N;
-  const std::vector<int32_t>& ch;
-  const std::vector<int32_t>& size;
-  std::vector<int32_t> need;  // Sethi-Ullman register need
-  std::vector<int32_t>* prog;
-  std::vector<int32_t>* fifo;
-  bool acc_live = false;
-  int32_t depth = 0, max_depth = 0;
+using Stop = std::function<bool(int32_t)>;  // nodes compiled as leaves
 
-  Compiler(int32_t n, const std::vector<int32_t>& c, const std::vector<int32_t>& s)
-      : N(n), ch(c), size(s), need(2 * n - 1, 0) {}
-
-  void compute_need(const std::vector<int32_t>& post) {
-    for (int32_t v : post) {
-      if (v < N) { need[v] = 0; continue; }
-      const int32_t a = ch[2 * (v - N)], b = ch[2 * (v - N) + 1];
-      const int32_t na = a < N ? 0 : need[a], nb = b < N ? 0 : need[b];
-      need[v] = (na == nb) ? na + 1 : std::max(na, nb);
-    }
-  }
-  // first-evaluated child A, second B
-  void order(int32_t v, int32_t* A, int32_t* B) const {
-    int32_t a = ch[2 * (v - N)], b = ch[2 * (v - N) + 1];
-    const bool la = a < N, lb = b < N;
-    if (la && !lb) std::swap(a, b);                // internal child first
-    else if (!la && !lb && need[b] > need[a]) std::swap(a, b);  // heavier first
-    *A = a;
-    *B = b;
-  }
-};
-
-bool build_post_order(int32_t N, int32_t root, const std::vector<int32_t>& ch,
-                      std::vector<int32_t>* post) {
-  // iterative post-order
+// post-order of the subtree under `top`, stop nodes as leaves
+void post_order(int32_t N, int32_t top, const std::vector<int32_t>& ch, const Stop& stop,
+                std::vector<int32_t>* post) {
   post->clear();
-  std::vector<std::pair<int32_t, int>> st = {{root, 0}};
+  std::vector<std::pair<int32_t, int>> st = {{top, 0}};
   while (!st.empty()) {
-    auto& top = st.back();
-    const int32_t v = top.first;
-    if (v < N) { post->push_back(v); st.pop_back(); continue; }
-    if (top.second < 2) {
-      const int32_t c = ch[2 * (v - N) + top.second];
-      top.second++;
+    auto& f = st.back();
+    const int32_t v = f.first;
+    if (stop(v)) { post->push_back(v); st.pop_back(); continue; }
+    if (f.second < 2) {
+      const int32_t c = ch[2 * (v - N) + f.second];
+      f.second++;
       st.push_back({c, 0});
     } else {
       post->push_back(v);
       st.pop_back();
     }
   }
-  return (int32_t)post->size() == 2 * N - 1;
 }
 
-}  // namespace
-
-int phy_compile_topo(phy_topo* t) {
-  const int32_t N = t->n_leaves, M = 2 * N - 1;
-  const std::vector<int32_t>& ch = t->children;
+// Post-order program of the subtree under `top` with an accumulator and a
+// stack (Sethi-Ullman order); returns false on an internal inconsistency.
+bool compile_down(int32_t N, const std::vector<int32_t>& ch, int32_t top, const Stop& stop,
+                  std::vector<int32_t>* prog, std::vector<int32_t>* fifo, int32_t* depth_out) {
   std::vector<int32_t> post;
-  if (!build_post_order(N, t->root, ch, &post)) return 1;
-  t->subtree.assign(M, 0);
-  for (int32_t v : post)
-    t->subtree[v] = v < N ? 1 : t->subtree[ch[2 * (v - N)]] + t->subtree[ch[2 * (v - N) + 1]];
-
-  // ---- post-order program with accumulator + stack ----
-  Compiler c(N, ch, t->subtree);
-  c.compute_need(post);
-  t->down_prog.clear();
-  t->down_fifo.clear();
-  // recursive evaluation, made iterative with an explicit frame stack
+  post_order(N, top, ch, stop, &post);
+  std::vector<int32_t> need(2 * N - 1, 0);
+  for (int32_t v : post) {
+    if (stop(v)) continue;
+    const int32_t a = ch[2 * (v - N)], b = ch[2 * (v - N) + 1];
+    const int32_t na = stop(a) ? 0 : need[a], nb = stop(b) ? 0 : need[b];
+    need[v] = (na == nb) ? na + 1 : std::max(na, nb);
+  }
+  prog->clear();
+  fifo->clear();
+  if (stop(top)) { *depth_out = 0; return true; }
+  bool acc_live = false;
+  int32_t depth = 0, max_depth = 0, n_int = 0;
   struct Frame { int32_t v; int stage; int32_t A, B; };
-  std::vector<Frame> fs = {{t->root, 0, -1, -1}};
+  std::vector<Frame> fs = {{top, 0, -1, -1}};
   while (!fs.empty()) {
     Frame& f = fs.back();
     if (f.stage == 0) {
-      c.order(f.v, &f.A, &f.B);
+      int32_t a = ch[2 * (f.v - N)], b = ch[2 * (f.v - N) + 1];
+      const bool la = stop(a), lb = stop(b);
+      if (la && !lb) std::swap(a, b);                              // internal child first
+      else if (!la && !lb && need[b] > need[a]) std::swap(a, b);   // heavier first
+      f.A = a;
+      f.B = b;
       f.stage = 1;
-      if (f.A >= N) { fs.push_back({f.A, 0, -1, -1}); continue; }
+      if (!stop(f.A)) { fs.push_back({f.A, 0, -1, -1}); continue; }
     }
     if (f.stage == 1) {
       f.stage = 2;
-      if (f.B >= N) { fs.push_back({f.B, 0, -1, -1}); continue; }
+      if (!stop(f.B)) { fs.push_back({f.B, 0, -1, -1}); continue; }
     }
-    // emit v
-    const bool la = f.A < N, lb = f.B < N;
-    int flags = 0;
-    int srcA, srcB;
+    const bool la = stop(f.A), lb = stop(f.B);
+    int flags = 0, srcA, srcB;
     if (la && lb) {
-      if (c.acc_live) {
+      if (acc_live) {
         flags |= DOWN_SPILL;
-        c.depth++;
-        c.max_depth = std::max(c.max_depth, c.depth);
+        max_depth = std::max(max_depth, ++depth);
       }
       srcA = SRC_LEAF;
       srcB = SRC_LEAF;
@@ -204,52 +177,60 @@ int phy_compile_topo(phy_topo* t) {
     } else {  // both internal (A leaf & B internal never happens)
       srcA = SRC_STACK;
       srcB = SRC_ACC;
-      c.depth--;
+      depth--;
     }
-    if (srcA == SRC_LEAF) t->down_fifo.push_back(f.A);
-    if (srcB == SRC_LEAF) t->down_fifo.push_back(f.B);
+    if (srcA == SRC_LEAF) fifo->push_back(f.A);
+    if (srcB == SRC_LEAF) fifo->push_back(f.B);
     flags |= srcA | (srcB << 2);
-    t->down_prog.insert(t->down_prog.end(), {f.v, f.A, f.B, flags});
-    c.acc_live = true;
+    prog->insert(prog->end(), {f.v, f.A, f.B, flags});
+    acc_live = true;
+    ++n_int;
     fs.pop_back();
   }
-  t->down_depth = c.max_depth;
-  if ((int32_t)t->down_prog.size() != 4 * (N - 1) || (int32_t)t->down_fifo.size() != N || c.depth != 0)
-    return 2;
+  *depth_out = max_depth;
+  return depth == 0 && (int32_t)post.size() == 2 * n_int + 1;
+}
 
-  // ---- pre-order program ----
-  t->up_prog.clear();
-  t->up_fifo.clear();
-  t->up_fifo.push_back(t->root);
-  // pending internal nodes: (node) on the stack; the accumulator holds at most one
+// Pre-order program of the subtree under `top`.  Stop nodes are not entered:
+// a stop child below N is a leaf (its final set is written), above N a cut
+// node (left to the program of its own part).  The first entry reads the
+// subtree root's set from the fifo with source `first_src` (SRC_LEAF at the
+// tree root: F = P; SRC_EXT at a cut node: the parent's final from memory).
+bool compile_up(int32_t N, const std::vector<int32_t>& ch, const std::vector<int32_t>& size,
+                int32_t top, const Stop& stop, int first_src, std::vector<int32_t>* prog,
+                std::vector<int32_t>* fifo, int32_t* depth_out) {
+  prog->clear();
+  fifo->clear();
+  *depth_out = 0;
+  if (top < N) return true;
+  fifo->push_back(top);
   std::vector<int32_t> stack;
-  int32_t cur = t->root;
-  int src = SRC_LEAF;  // root
-  int32_t depth = 0, max_depth = 0;
+  int32_t cur = top, depth = 0, max_depth = 0;
+  int src = first_src;
   for (;;) {
     const int32_t a = ch[2 * (cur - N)], b = ch[2 * (cur - N) + 1];
-    t->up_fifo.push_back(a);
-    t->up_fifo.push_back(b);
+    fifo->push_back(a);
+    fifo->push_back(b);
+    const bool sa = stop(a), sb = stop(b);
     int flags = src;
-    if (a < N) flags |= UP_A_LEAF;
-    if (b < N) flags |= UP_B_LEAF;
+    if (sa) flags |= a < N ? UP_A_LEAF : UP_A_CUT;
+    if (sb) flags |= b < N ? UP_B_LEAF : UP_B_CUT;
     int32_t next = -1;
-    if (a >= N && b >= N) {
+    if (!sa && !sb) {
       // continue with the smaller subtree, push the larger
-      const bool contb = t->subtree[b] < t->subtree[a];
+      const bool contb = size[b] < size[a];
       if (contb) flags |= UP_CONT_B;
       flags |= UP_PUSH;
       stack.push_back(contb ? a : b);
-      depth++;
-      max_depth = std::max(max_depth, depth);
+      max_depth = std::max(max_depth, ++depth);
       next = contb ? b : a;
-    } else if (a >= N) {
+    } else if (!sa) {
       next = a;
-    } else if (b >= N) {
+    } else if (!sb) {
       flags |= UP_CONT_B;
       next = b;
     }
-    t->up_prog.insert(t->up_prog.end(), {cur, a, b, flags});
+    prog->insert(prog->end(), {cur, a, b, flags});
     if (next >= 0) {
       cur = next;
       src = SRC_ACC;
@@ -262,8 +243,103 @@ int phy_compile_topo(phy_topo* t) {
       break;
     }
   }
-  t->up_depth = max_depth;
+  *depth_out = max_depth;
+  return true;
+}
+
+// Cut nodes for side-by-side parts: split the largest internal subtree of
+// the frontier until it holds `want` internal nodes or the largest one has
+// fewer than `min_leaves` leaves.
+std::vector<int32_t> choose_cuts(int32_t N, const std::vector<int32_t>& ch,
+                                 const std::vector<int32_t>& size, int32_t root, int want,
+                                 int32_t min_leaves) {
+  std::vector<int32_t> front = {root};
+  for (;;) {
+    int best = -1, n_int = 0;
+    for (size_t k = 0; k < front.size(); ++k) {
+      if (front[k] < N) continue;
+      ++n_int;
+      if (best < 0 || size[front[k]] > size[front[best]]) best = (int)k;
+    }
+    if (best < 0 || n_int >= want || size[front[best]] < 2 * min_leaves) break;
+    const int32_t v = front[best];
+    front.erase(front.begin() + best);
+    front.push_back(ch[2 * (v - N)]);
+    front.push_back(ch[2 * (v - N) + 1]);
+  }
+  std::vector<int32_t> cuts;
+  for (int32_t v : front)
+    if (v >= N) cuts.push_back(v);
+  std::sort(cuts.begin(), cuts.end(), [&](int32_t x, int32_t y) { return size[x] > size[y]; });
+  return cuts;
+}
+
+}  // namespace
+
+int phy_compile_topo(phy_topo* t) {
+  const int32_t N = t->n_leaves, M = 2 * N - 1;
+  const std::vector<int32_t>& ch = t->children;
+  const Stop is_leaf = [N](int32_t v) { return v < N; };
+  std::vector<int32_t> post;
+  post_order(N, t->root, ch, is_leaf, &post);
+  if ((int32_t)post.size() != M) return 1;
+  t->subtree.assign(M, 0);
+  for (int32_t v : post)
+    t->subtree[v] = v < N ? 1 : t->subtree[ch[2 * (v - N)]] + t->subtree[ch[2 * (v - N) + 1]];
+  if (!compile_down(N, ch, t->root, is_leaf, &t->down_prog, &t->down_fifo, &t->down_depth) ||
+      (int32_t)t->down_prog.size() != 4 * (N - 1) || (int32_t)t->down_fifo.size() != N)
+    return 2;
+  compile_up(N, ch, t->subtree, t->root, is_leaf, SRC_LEAF, &t->up_prog, &t->up_fifo, &t->up_depth);
   if ((int32_t)t->up_prog.size() != 4 * (N - 1) || (int32_t)t->up_fifo.size() != M) return 3;
+
+  // ---- split programs: parts under cut nodes run side by side ----
+  t->parts.clear();
+  const char* env_parts = getenv("PHY_FITCH_PARTS");
+  const int max_parts = env_parts ? std::max(1, std::min(kMaxParts, atoi(env_parts))) : 8;
+  const int want = (int)std::min<int32_t>(max_parts, N / 64);
+  if (want >= 2) {
+    const std::vector<int32_t> cuts = choose_cuts(N, ch, t->subtree, t->root, want, 16);
+    if (cuts.size() >= 2) {
+      std::vector<char> is_cut(M, 0);
+      for (int32_t c : cuts) is_cut[c] = 1;
+      const Stop stop_top = [N, &is_cut](int32_t v) { return v < N || is_cut[v]; };
+      SplitProg& sd = t->split_down;
+      SplitProg& su = t->split_up;
+      sd = SplitProg();
+      su = SplitProg();
+      std::vector<int32_t> pg, ff;
+      int32_t d = 0, n_ops = 0;
+      auto add = [&](SplitProg& sp) {
+        sp.prog_off.push_back((int32_t)sp.prog.size() / 4);
+        sp.n_ops.push_back((int32_t)pg.size() / 4);
+        sp.fifo_off.push_back((int32_t)sp.fifo.size());
+        sp.prog.insert(sp.prog.end(), pg.begin(), pg.end());
+        std::vector<int32_t> pad(phy_fifo_padded(ff.size()), ff.empty() ? 0 : ff[0]);
+        std::copy(ff.begin(), ff.end(), pad.begin());
+        sp.fifo.insert(sp.fifo.end(), pad.begin(), pad.end());
+        sp.depth = std::max(sp.depth, d);
+        n_ops += (int32_t)pg.size() / 4;
+      };
+      // down: parts first, then the top (cut nodes read as leaves)
+      for (int32_t c : cuts) {
+        if (!compile_down(N, ch, c, is_leaf, &pg, &ff, &d)) return 4;
+        add(sd);
+      }
+      if (!compile_down(N, ch, t->root, stop_top, &pg, &ff, &d)) return 4;
+      add(sd);
+      if (n_ops != N - 1) return 5;
+      // up: the top first, then the parts (entered from the parent's final)
+      n_ops = 0;
+      compile_up(N, ch, t->subtree, t->root, stop_top, SRC_LEAF, &pg, &ff, &d);
+      add(su);
+      for (int32_t c : cuts) {
+        compile_up(N, ch, t->subtree, c, is_leaf, SRC_EXT, &pg, &ff, &d);
+        add(su);
+      }
+      if (n_ops != N - 1) return 6;
+      for (int32_t c : cuts) t->parts.push_back(c);
+    }
+  }
   return 0;
 }
 
@@ -328,6 +404,24 @@ int phy_topo_create(phy_ctx* ctx, int32_t N, const int32_t* children, int32_t ro
   if (e == hipSuccess) e = hipMemcpy(t->d_down_fifo, dfifo.data(), fd, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(t->d_up_prog, t->up_prog.data(), pu, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(t->d_up_fifo, ufifo.data(), fu, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !t->parts.empty()) {
+    std::vector<int32_t> pp(t->parts.size());
+    for (size_t k = 0; k < t->parts.size(); ++k) pp[k] = t->parent[t->parts[k]];
+    for (SplitProg* sp : {&t->split_down, &t->split_up}) {
+      std::vector<int32_t> meta;  // [prog_off | n_ops | fifo_off] x (parts + 1)
+      meta.insert(meta.end(), sp->prog_off.begin(), sp->prog_off.end());
+      meta.insert(meta.end(), sp->n_ops.begin(), sp->n_ops.end());
+      meta.insert(meta.end(), sp->fifo_off.begin(), sp->fifo_off.end());
+      if (e == hipSuccess) e = hipMalloc((void**)&sp->d_prog, sp->prog.size() * 4);
+      if (e == hipSuccess) e = hipMalloc((void**)&sp->d_fifo, sp->fifo.size() * 4);
+      if (e == hipSuccess) e = hipMalloc((void**)&sp->d_meta, meta.size() * 4);
+      if (e == hipSuccess) e = hipMemcpy(sp->d_prog, sp->prog.data(), sp->prog.size() * 4, hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemcpy(sp->d_fifo, sp->fifo.data(), sp->fifo.size() * 4, hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemcpy(sp->d_meta, meta.data(), meta.size() * 4, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMalloc((void**)&t->d_part_parent, pp.size() * 4);
+    if (e == hipSuccess) e = hipMemcpy(t->d_part_parent, pp.data(), pp.size() * 4, hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) {
     phy_topo_destroy(t);
     return set_err(ctx, PHY_EDEVICE, std::string("topology upload: ") + hipGetErrorString(e));
@@ -342,6 +436,13 @@ void phy_topo_destroy(phy_topo* t) {
   if (t->d_down_fifo) (void)hipFree(t->d_down_fifo);
   if (t->d_up_prog) (void)hipFree(t->d_up_prog);
   if (t->d_up_fifo) (void)hipFree(t->d_up_fifo);
+  for (SplitProg* sp : {&t->split_down, &t->split_up}) {
+    if (sp->d_prog) (void)hipFree(sp->d_prog);
+    if (sp->d_fifo) (void)hipFree(sp->d_fifo);
+    if (sp->d_meta) (void)hipFree(sp->d_meta);
+  }
+  if (t->d_part_parent) (void)hipFree(t->d_part_parent);
+  if (t->d_scratch) (void)hipFree(t->d_scratch);
   delete t;
 }
 

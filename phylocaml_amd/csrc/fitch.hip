@@ -206,14 +206,31 @@ struct SetLoad {
   __device__ __forceinline__ uint32_t operator()(int id) const { return sets[(int64_t)id * stride + wc]; }
 };
 
-template <int BATCH, bool NODE_COST>
+// Pass modes: the whole tree in one program, or a tree split at cut nodes:
+// the parts side by side (grid.y = part; per-site counts of a part go to
+// scratch[part][lane][8]) and then the top, which adds the parts' counts.
+enum : int { MODE_WHOLE = 0, MODE_PART = 1, MODE_TOP = 2 };
+
+// program k of a split program: meta = prog_off | n_ops | fifo_off, each
+// n_parts + 1 long
+#define SPLIT_SELECT(k)                                 \
+  do {                                                  \
+    prog += meta[(k)];                                  \
+    n_ops = meta[(n_parts + 1) + (k)];                  \
+    fifo_ids += meta[2 * (n_parts + 1) + (k)];          \
+  } while (0)
+
+template <int BATCH, bool NODE_COST, int MODE>
 __global__ __launch_bounds__(256) void k_fitch_down(
     const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ fifo_ids,
+    const int32_t* __restrict__ meta, int n_parts, uint4* __restrict__ scratch,
     uint32_t* __restrict__ sets, int64_t stride, int64_t W, int64_t n_sites, int N,
     const uint32_t* __restrict__ wt, uint32_t* __restrict__ site_cost,
     unsigned long long* __restrict__ node_cost, unsigned long long* __restrict__ total) {
   extern __shared__ uint32_t lds32[];
   const int tid = threadIdx.x, bd = blockDim.x;
+  const int part = MODE == MODE_PART ? (int)blockIdx.y : n_parts;
+  if (MODE != MODE_WHOLE) SPLIT_SELECT(part);
   const int64_t w = (int64_t)blockIdx.x * bd + tid;  // 32-bit lane index
   const bool act = w < W;                              // W = 2 * words
   const int64_t wc = act ? w : 0;
@@ -263,6 +280,22 @@ __global__ __launch_bounds__(256) void k_fitch_down(
   });
 #pragma unroll
   for (int k = 0; k < 8; ++k) site[k] += (cnt >> (4 * k)) & 15u;
+  if (MODE == MODE_PART) {
+    if (act) {
+      uint4* dst = scratch + ((size_t)part * W + w) * 2;
+      dst[0] = make_uint4(site[0], site[1], site[2], site[3]);
+      dst[1] = make_uint4(site[4], site[5], site[6], site[7]);
+    }
+    return;
+  }
+  if (MODE == MODE_TOP && act) {
+    for (int q = 0; q < n_parts; ++q) {
+      const uint4* src = scratch + ((size_t)q * W + w) * 2;
+      const uint4 x = src[0], y = src[1];
+      site[0] += x.x; site[1] += x.y; site[2] += x.z; site[3] += x.w;
+      site[4] += y.x; site[5] += y.y; site[6] += y.z; site[7] += y.w;
+    }
+  }
   uint64_t tot = 0;
   if (act) {
     const int64_t s0 = w * 8;
@@ -297,14 +330,19 @@ __global__ __launch_bounds__(256) void k_fitch_down(
 // ---------------------------------------------------------------------------
 // Whole-tree up-pass (32-bit lanes)
 // ---------------------------------------------------------------------------
-template <int BATCH>
+template <int BATCH, int MODE>
 __global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog, int n_ops,
                                                   const int32_t* __restrict__ fifo_ids,
+                                                  const int32_t* __restrict__ meta, int n_parts,
+                                                  const int32_t* __restrict__ part_parent,
                                                   const uint32_t* __restrict__ sets,
                                                   uint32_t* __restrict__ fin, int64_t stride,
                                                   int64_t W, int N) {
   extern __shared__ uint32_t lds32[];
   const int tid = threadIdx.x, bd = blockDim.x;
+  // up: the top is program 0, part k is program 1 + k
+  const int part = MODE == MODE_PART ? (int)blockIdx.y : -1;
+  if (MODE != MODE_WHOLE) SPLIT_SELECT(part + 1);
   const int64_t w = (int64_t)blockIdx.x * bd + tid;
   const bool act = w < W;
   const int64_t wc = act ? w : 0;
@@ -320,6 +358,10 @@ __global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog,
     uint32_t pv, fp = 0;
     if (src == SRC_LEAF) pv = nodes.get();
     else if (src == SRC_ACC) { pv = accP; fp = accF; }
+    else if (MODE == MODE_PART && src == SRC_EXT) {  // a part's root: parent final from the top pass
+      pv = nodes.get();
+      fp = fin[(int64_t)part_parent[part] * stride + wc];
+    }
     else {
       --sp;
       pv = stk[(2 * sp) * bd + tid];
@@ -383,7 +425,6 @@ int block_size_env(const char* name, int dflt) {
   return (v == 64 || v == 128 || v == 256) ? v : dflt;
 }
 
-constexpr int kBatch = 32;
 
 }  // namespace
 
@@ -443,6 +484,113 @@ int phy_fitch_adjust3(phy_ctx* ctx, const uint64_t* pv, const uint64_t* pa, cons
   return run_elem<OP_ADJUST3>(ctx, pv, pa, pb, fp, out, n_sites, nullptr, changed);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Split passes (parts side by side, then the top) unless disabled; the
+// whole-tree program otherwise.
+bool use_split(const phy_topo* t) {
+  const char* e = getenv("PHY_FITCH_SPLIT");
+  return t->parts.size() >= 2 && t->split_down.d_prog && !(e && e[0] == '0');
+}
+
+template <int BATCH>
+int downpass_launch(phy_ctx* ctx, const phy_topo* topo, uint32_t* s32, int64_t stride32,
+                    int64_t W32, int64_t n_sites, const uint32_t* wt, uint32_t* site_cost,
+                    unsigned long long* node_cost, unsigned long long* total, int bs) {
+  const int N = topo->n_leaves;
+  const int64_t g = (W32 + bs - 1) / bs;
+  if (!use_split(topo)) {
+    const size_t lds = (size_t)(2 * BATCH + topo->down_depth) * bs * sizeof(uint32_t);
+    if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "downpass: LDS stack too deep for this tree");
+    if (node_cost)
+      hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_WHOLE>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
+                         topo->d_down_prog, N - 1, topo->d_down_fifo, nullptr, 0, nullptr, s32, stride32, W32,
+                         n_sites, N, wt, site_cost, node_cost, total);
+    else
+      hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_WHOLE>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
+                         topo->d_down_prog, N - 1, topo->d_down_fifo, nullptr, 0, nullptr, s32, stride32, W32,
+                         n_sites, N, wt, site_cost, node_cost, total);
+    PHY_LAUNCHED(ctx);
+    return PHY_OK;
+  }
+  const SplitProg& sp = topo->split_down;
+  const int P = (int)topo->parts.size();
+  const size_t need = (size_t)P * W32 * 8 * sizeof(uint32_t);
+  if (topo->scratch_bytes < need) {
+    if (topo->d_scratch) PHY_HIP(ctx, hipFree(topo->d_scratch));
+    topo->d_scratch = nullptr;
+    topo->scratch_bytes = 0;
+    if (hipMalloc((void**)&topo->d_scratch, need) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_err(ctx, PHY_ENOMEM, "downpass: scratch allocation failed");
+    }
+    topo->scratch_bytes = need;
+  }
+  uint4* scr = reinterpret_cast<uint4*>(topo->d_scratch);
+  const size_t lds = (size_t)(2 * BATCH + sp.depth) * bs * sizeof(uint32_t);
+  if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "downpass: LDS stack too deep for this tree");
+  if (node_cost) {
+    hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_PART>), dim3((unsigned)g, (unsigned)P), dim3(bs), lds,
+                       ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites,
+                       N, wt, site_cost, node_cost, total);
+    PHY_LAUNCHED(ctx);
+    hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_TOP>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
+                       sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites, N, wt,
+                       site_cost, node_cost, total);
+  } else {
+    hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_PART>), dim3((unsigned)g, (unsigned)P), dim3(bs), lds,
+                       ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites,
+                       N, wt, site_cost, node_cost, total);
+    PHY_LAUNCHED(ctx);
+    hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_TOP>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
+                       sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites, N, wt,
+                       site_cost, node_cost, total);
+  }
+  PHY_LAUNCHED(ctx);
+  return PHY_OK;
+}
+
+template <int BATCH>
+int uppass_launch(phy_ctx* ctx, const phy_topo* topo, const uint32_t* s32, uint32_t* f32,
+                  int64_t stride32, int64_t W32, int bs) {
+  const int N = topo->n_leaves;
+  const int64_t g = (W32 + bs - 1) / bs;
+  if (!use_split(topo)) {
+    const size_t lds = (size_t)(2 * BATCH + 2 * topo->up_depth) * bs * sizeof(uint32_t);
+    if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "uppass: LDS stack too deep for this tree");
+    hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_WHOLE>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
+                       topo->d_up_prog, N - 1, topo->d_up_fifo, nullptr, 0, nullptr, s32, f32, stride32,
+                       W32, N);
+    PHY_LAUNCHED(ctx);
+    return PHY_OK;
+  }
+  const SplitProg& sp = topo->split_up;
+  const int P = (int)topo->parts.size();
+  const size_t lds = (size_t)(2 * BATCH + 2 * sp.depth) * bs * sizeof(uint32_t);
+  if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "uppass: LDS stack too deep for this tree");
+  hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_TOP>), dim3((unsigned)g), dim3(bs), lds, ctx->stream, sp.d_prog,
+                     0, sp.d_fifo, sp.d_meta, P, topo->d_part_parent, s32, f32, stride32, W32, N);
+  PHY_LAUNCHED(ctx);
+  hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_PART>), dim3((unsigned)g, (unsigned)P), dim3(bs), lds,
+                     ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, topo->d_part_parent, s32, f32,
+                     stride32, W32, N);
+  PHY_LAUNCHED(ctx);
+  return PHY_OK;
+}
+
+// operand prefetch depth (leaves / node sets per batch), default 8
+int batch_env() {
+  const char* s = getenv("PHY_FITCH_BATCH");
+  const int v = s ? atoi(s) : 8;
+  return (v == 8 || v == 16 || v == 32) ? v : 8;
+}
+
+}  // namespace
+
+extern "C" {
+
 int phy_fitch_downpass(phy_ctx* ctx, const phy_topo* topo, uint64_t* sets, int64_t stride,
                        int64_t n_sites, const uint32_t* wt, uint32_t* site_cost,
                        uint64_t* node_cost, uint64_t* total) {
@@ -456,22 +604,16 @@ int phy_fitch_downpass(phy_ctx* ctx, const phy_topo* topo, uint64_t* sets, int64
   if ((rc = phy_zero_u64(ctx, total, 1))) return rc;
   if (node_cost && (rc = phy_zero_u64(ctx, node_cost, N - 1))) return rc;
   if (W == 0) return PHY_OK;
-  const int bs = block_size_env("PHY_FITCH_BLOCK", 64);
+  const int bs = block_size_env("PHY_FITCH_BLOCK", 256);
   const int64_t W32 = 2 * W;  // 32-bit lanes
-  const int64_t g = (W32 + bs - 1) / bs;
-  const size_t lds = (size_t)(2 * kBatch + topo->down_depth) * bs * sizeof(uint32_t);
-  if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "downpass: LDS stack too deep for this tree");
   uint32_t* s32 = reinterpret_cast<uint32_t*>(sets);
-  if (node_cost)
-    hipLaunchKernelGGL((k_fitch_down<kBatch, true>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
-                       topo->d_down_prog, N - 1, topo->d_down_fifo, s32, 2 * stride, W32, n_sites, N,
-                       wt, site_cost, (unsigned long long*)node_cost, (unsigned long long*)total);
-  else
-    hipLaunchKernelGGL((k_fitch_down<kBatch, false>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
-                       topo->d_down_prog, N - 1, topo->d_down_fifo, s32, 2 * stride, W32, n_sites, N,
-                       wt, site_cost, (unsigned long long*)nullptr, (unsigned long long*)total);
-  PHY_LAUNCHED(ctx);
-  return PHY_OK;
+  auto* nc = reinterpret_cast<unsigned long long*>(node_cost);
+  auto* tt = reinterpret_cast<unsigned long long*>(total);
+  switch (batch_env()) {
+    case 8: return downpass_launch<8>(ctx, topo, s32, 2 * stride, W32, n_sites, wt, site_cost, nc, tt, bs);
+    case 32: return downpass_launch<32>(ctx, topo, s32, 2 * stride, W32, n_sites, wt, site_cost, nc, tt, bs);
+    default: return downpass_launch<16>(ctx, topo, s32, 2 * stride, W32, n_sites, wt, site_cost, nc, tt, bs);
+  }
 }
 
 int phy_fitch_uppass(phy_ctx* ctx, const phy_topo* topo, const uint64_t* sets, uint64_t* fin,
@@ -482,18 +624,15 @@ int phy_fitch_uppass(phy_ctx* ctx, const phy_topo* topo, const uint64_t* sets, u
   if (n_sites < 0 || stride < W) return set_err(ctx, PHY_EINVAL, "uppass: stride < words");
   if (!sets || !fin) return set_err(ctx, PHY_EINVAL, "uppass: null buffer");
   if (W == 0) return PHY_OK;
-  const int N = topo->n_leaves;
-  const int bs = block_size_env("PHY_FITCH_BLOCK", 64);
+  const int bs = block_size_env("PHY_FITCH_BLOCK", 256);
   const int64_t W32 = 2 * W;
-  const int64_t g = (W32 + bs - 1) / bs;
-  const size_t lds = (size_t)(2 * kBatch + 2 * topo->up_depth) * bs * sizeof(uint32_t);
-  if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "uppass: LDS stack too deep for this tree");
-  hipLaunchKernelGGL((k_fitch_up<kBatch>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
-                     topo->d_up_prog, N - 1, topo->d_up_fifo,
-                     reinterpret_cast<const uint32_t*>(sets), reinterpret_cast<uint32_t*>(fin),
-                     2 * stride, W32, N);
-  PHY_LAUNCHED(ctx);
-  return PHY_OK;
+  const uint32_t* s32 = reinterpret_cast<const uint32_t*>(sets);
+  uint32_t* f32 = reinterpret_cast<uint32_t*>(fin);
+  switch (batch_env()) {
+    case 8: return uppass_launch<8>(ctx, topo, s32, f32, 2 * stride, W32, bs);
+    case 32: return uppass_launch<32>(ctx, topo, s32, f32, 2 * stride, W32, bs);
+    default: return uppass_launch<16>(ctx, topo, s32, f32, 2 * stride, W32, bs);
+  }
 }
 
 int phy_fitch_insertion_costs(phy_ctx* ctx, const uint64_t* pre, const uint64_t* fin,

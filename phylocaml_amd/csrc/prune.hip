@@ -272,91 +272,90 @@ __device__ __forceinline__ void rescale4(real* v, int* e) {
   }
 }
 
+// State of one wave of the post-order pass: accumulator, stack, tip stream.
 template <typename real, int R, bool FUSED>
-__device__ __forceinline__ void prune_down_body(
-    const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids, int n_leaf_ids,
-    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
-    const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
-    real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
-    double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth) {
-  extern __shared__ unsigned char lds_raw[];
-  const int tid = threadIdx.x, bd = blockDim.x, lane = tid & 63;
-  const int r = __builtin_amdgcn_readfirstlane(tid >> 6);  // rate class of this wave
-  const int64_t p = (int64_t)blockIdx.x * 64 + lane;
-  const bool act = p < n_pat;
-  const int64_t pc = act ? p : 0;
-
-  // LDS: (fused) stack [depth][4][bd] real + [depth][bd] int; at the root,
-  // when the stack is empty, the same bytes hold the rate exchange
-  // [R][64] double S + [R][64] int e
-  real* stk = reinterpret_cast<real*>(lds_raw);
-  int32_t* stke = reinterpret_cast<int32_t*>(stk + (size_t)(FUSED ? stack_depth : 0) * 4 * bd);
-  double* rootS = reinterpret_cast<double*>(lds_raw);
-  int* rootE = reinterpret_cast<int*>(rootS + R * 64);
-
-  TipNibbles ts;
-  ts.init(tips, stride, pc, leaf_ids);
-
+struct DownState {
   real acc[4] = {(real)0, (real)0, (real)0, (real)0};
   int acc_e = 0, sp = 0;
-  for_each_entry(prog, n_ops, [&](int, const int4 op) {
-    real PA[16], PB[16];
-    load_mat<real, R>(Pm + ((size_t)op.y * R + r) * 16, PA);
-    load_mat<real, R>(Pm + ((size_t)op.z * R + r) * 16, PB);
-    const int fl = op.w;
+  real* stk;       // [depth][4][bd]
+  int32_t* stke;   // [depth][bd]
+  int bd, tid, r, N;
+  int64_t pitch, pc, p;
+  bool act;
+  real* part;
+  int32_t* ex;
+  TipNibbles ts;
+
+  __device__ __forceinline__ void spill_if(int fl) {
     if (FUSED && (fl & DOWN_SPILL)) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) stk[((size_t)sp * 4 + k) * bd + tid] = acc[k];
       stke[(size_t)sp * bd + tid] = acc_e;
       ++sp;
     }
-    const int sa = fl & 3, sb = (fl >> 2) & 3;
-    real xa[4], xb[4], src[4];
-    int ea = 0, eb = 0;
+  }
+  // operand A of an entry (leaf, accumulator, stack or stored partial of child a)
+  __device__ __forceinline__ void operand_a(int sa, int a, real* src, int* e) {
+    *e = 0;
     if (sa == SRC_LEAF) {
       tipvec(ts.get(), src);
     } else if (sa == SRC_ACC) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) src[k] = acc[k];
-      ea = acc_e;
+      *e = acc_e;
     } else if (FUSED) {
       --sp;
 #pragma unroll
       for (int k = 0; k < 4; ++k) src[k] = stk[((size_t)sp * 4 + k) * bd + tid];
-      ea = stke[(size_t)sp * bd + tid];
+      *e = stke[(size_t)sp * bd + tid];
     } else {
-      const real* sv = part + (((size_t)(op.y - N) * R + r) * 4) * pitch + pc;
+      const real* sv = part + (((size_t)(a - N) * R + r) * 4) * pitch + pc;
 #pragma unroll
       for (int k = 0; k < 4; ++k) src[k] = sv[(size_t)k * pitch];
-      ea = ex[((size_t)(op.y - N) * R + r) * pitch + pc];
+      *e = ex[((size_t)(a - N) * R + r) * pitch + pc];
     }
-    matvec4(PA, src, xa);
+  }
+  // operand B (leaf or accumulator)
+  __device__ __forceinline__ void operand_b(int sb, real* src, int* e) {
+    *e = 0;
     if (sb == SRC_LEAF) {
       tipvec(ts.get(), src);
     } else {
 #pragma unroll
       for (int k = 0; k < 4; ++k) src[k] = acc[k];
-      eb = acc_e;
+      *e = acc_e;
     }
-    matvec4(PB, src, xb);
+  }
+  __device__ __forceinline__ void combine(int v, const real* xa, const real* xb, int e) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) acc[k] = xa[k] * xb[k];
-    acc_e = ea + eb;
+    acc_e = e;
     rescale4(acc, &acc_e);
     if (!FUSED && act) {
-      real* d = part + (((size_t)(op.x - N) * R + r) * 4) * pitch + p;
+      real* d = part + (((size_t)(v - N) * R + r) * 4) * pitch + p;
 #pragma unroll
       for (int k = 0; k < 4; ++k) d[(size_t)k * pitch] = acc[k];
-      ex[((size_t)(op.x - N) * R + r) * pitch + p] = acc_e;
+      ex[((size_t)(v - N) * R + r) * pitch + p] = acc_e;
     }
-  });
-  // root: S_r = sum_i pi_i L[r][i]; combine the R classes in wave 0
+  }
+};
+
+// root: S_r = sum_i pi_i L[r][i]; combine the R classes in wave 0
+template <typename real, int R, bool FUSED>
+__device__ __forceinline__ void down_root(const DownState<real, R, FUSED>& st, unsigned char* lds_raw,
+                                          const double* __restrict__ pi,
+                                          const double* __restrict__ probs,
+                                          const uint32_t* __restrict__ weights,
+                                          double* __restrict__ site_lnl, double* __restrict__ wave_sums) {
+  const int lane = st.tid & 63, r = st.r;
+  double* rootS = reinterpret_cast<double*>(lds_raw);
+  int* rootE = reinterpret_cast<int*>(rootS + R * 64);
   double S = 0.0;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) S += pi[i] * (double)acc[i];
+  for (int i = 0; i < 4; ++i) S += pi[i] * (double)st.acc[i];
   __syncthreads();  // every wave is done with its stack rows
   rootS[r * 64 + lane] = S;
-  rootE[r * 64 + lane] = acc_e;
+  rootE[r * 64 + lane] = st.acc_e;
   __syncthreads();
   if (r == 0) {
     int m = rootE[lane];
@@ -367,9 +366,9 @@ __device__ __forceinline__ void prune_down_body(
     for (int q = 0; q < R; ++q) L += probs[q] * ldexp(rootS[q * 64 + lane], rootE[q * 64 + lane] - m);
     const double slnl = log(L) + (double)m * 0.6931471805599453;
     double contrib = 0.0;
-    if (act) {
-      if (site_lnl) site_lnl[p] = slnl;
-      contrib = (double)weights[p] * slnl;
+    if (st.act) {
+      if (site_lnl) site_lnl[st.p] = slnl;
+      contrib = (double)weights[st.p] * slnl;
     }
     // one partial sum per 64 patterns, independent of the launch shape
     const double ws = wave_sum_d(contrib);
@@ -378,15 +377,66 @@ __device__ __forceinline__ void prune_down_body(
 }
 
 template <typename real, int R, bool FUSED>
-__global__ __launch_bounds__(512) void k_prune_down(
-    const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids, int n_leaf_ids,
+__device__ __forceinline__ void down_init(DownState<real, R, FUSED>& st, unsigned char* lds_raw,
+                                          const int32_t* leaf_ids, const uint64_t* tips,
+                                          int64_t stride, int64_t n_pat, int64_t pitch, real* part,
+                                          int32_t* ex, int N, int stack_depth) {
+  st.tid = threadIdx.x;
+  st.bd = blockDim.x;
+  st.r = __builtin_amdgcn_readfirstlane(st.tid >> 6);  // rate class of this wave
+  st.p = (int64_t)blockIdx.x * 64 + (st.tid & 63);
+  st.act = st.p < n_pat;
+  st.pc = st.act ? st.p : 0;
+  st.pitch = pitch;
+  st.N = N;
+  st.part = part;
+  st.ex = ex;
+  // LDS: (fused) stack [depth][4][bd] real + [depth][bd] int; at the root,
+  // when the stack is empty, the same bytes hold the rate exchange
+  // [R][64] double S + [R][64] int e
+  st.stk = reinterpret_cast<real*>(lds_raw);
+  st.stke = reinterpret_cast<int32_t*>(st.stk + (size_t)(FUSED ? stack_depth : 0) * 4 * st.bd);
+  st.ts.init(tips, stride, st.pc, leaf_ids);
+}
+
+// VALU form: the wave's 4x4 P of both children are wave-uniform (SGPRs)
+template <typename real, int R, bool FUSED>
+__device__ __forceinline__ void prune_down_body(
+    const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids,
     const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
     const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
     real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
     double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth) {
-  prune_down_body<real, R, FUSED>(prog, n_ops, leaf_ids, n_leaf_ids, tips, stride, n_pat, pitch, Pm,
-                                  pi, probs, part, ex, weights, site_lnl, wave_sums, N,
-                                  stack_depth);
+  extern __shared__ unsigned char lds_raw[];
+  DownState<real, R, FUSED> st;
+  down_init(st, lds_raw, leaf_ids, tips, stride, n_pat, pitch, part, ex, N, stack_depth);
+  const int r = st.r;
+  for_each_entry(prog, n_ops, [&](int, const int4 op) {
+    real PA[16], PB[16];
+    load_mat<real, R>(Pm + ((size_t)op.y * R + r) * 16, PA);
+    load_mat<real, R>(Pm + ((size_t)op.z * R + r) * 16, PB);
+    const int fl = op.w;
+    st.spill_if(fl);
+    real src[4], xa[4], xb[4];
+    int ea, eb;
+    st.operand_a(fl & 3, op.y, src, &ea);
+    matvec4(PA, src, xa);
+    st.operand_b((fl >> 2) & 3, src, &eb);
+    matvec4(PB, src, xb);
+    st.combine(op.x, xa, xb, ea + eb);
+  });
+  down_root(st, lds_raw, pi, probs, weights, site_lnl, wave_sums);
+}
+
+template <typename real, int R, bool FUSED>
+__global__ __launch_bounds__(512) void k_prune_down(
+    const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids,
+    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
+    const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
+    real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
+    double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth) {
+  prune_down_body<real, R, FUSED>(prog, n_ops, leaf_ids, tips, stride, n_pat, pitch, Pm, pi, probs,
+                                  part, ex, weights, site_lnl, wave_sums, N, stack_depth);
 }
 
 // Many candidate topologies over the same tips and weights (fused form):
@@ -401,10 +451,11 @@ __global__ __launch_bounds__(512) void k_prune_down_batch(
     int stack_depth) {
   const int64_t c = blockIdx.y;
   const int M = 2 * N - 1;
-  prune_down_body<real, R, true>(prog + c * (N - 1), N - 1, leaf_ids + c * (int64_t)phy_fifo_padded(N), N, tips, stride,
-                                 n_pat, pitch, Pm + c * (int64_t)M * R * 16, pi, probs, nullptr,
-                                 nullptr, weights, nullptr, wave_sums + c * n_waves, N,
-                                 stack_depth);
+  const int4* pg = prog + c * (N - 1);
+  const int32_t* li = leaf_ids + c * (int64_t)phy_fifo_padded(N);
+  const real* P = Pm + c * (int64_t)M * R * 16;
+  prune_down_body<real, R, true>(pg, N - 1, li, tips, stride, n_pat, pitch, P, pi, probs, nullptr,
+                                 nullptr, weights, nullptr, wave_sums + c * n_waves, N, stack_depth);
 }
 
 // Helpers of the derivative pass, whose threads are (pattern, rate) quads.
@@ -957,11 +1008,11 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
   double* bsum = reinterpret_cast<double*>(work + pl.off_bsum);
   if (fused)
     hipLaunchKernelGGL((k_prune_down<real, R, true>), dim3((unsigned)pl.n_blocks), dim3(pl.bs), pl.lds,
-                       ctx->stream, t->d_down_prog, N - 1, t->d_down_fifo, N, tips, stride, n_pat,
+                       ctx->stream, t->d_down_prog, N - 1, t->d_down_fifo, tips, stride, n_pat,
                        pl.pitch, P, d_pi, d_probs, part, ex, weights, site_lnl, bsum, N, t->down_depth);
   else
     hipLaunchKernelGGL((k_prune_down<real, R, false>), dim3((unsigned)pl.n_blocks), dim3(pl.bs), pl.lds,
-                       ctx->stream, t->d_down_prog, N - 1, t->d_down_fifo, N, tips, stride, n_pat,
+                       ctx->stream, t->d_down_prog, N - 1, t->d_down_fifo, tips, stride, n_pat,
                        pl.pitch, P, d_pi, d_probs, part, ex, weights, site_lnl, bsum, N, t->down_depth);
   PHY_LAUNCHED(ctx);
   hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, ctx->stream, bsum, pl.n_waves, (int64_t)1, lnl);
@@ -1172,8 +1223,8 @@ int spr_launch(phy_ctx* ctx, const phy_model* m, int N, int64_t n_pat, const uin
   const size_t lds = down_lds(R, sizeof(real), true, depth);
   if (lds > kLdsMax) return set_err(ctx, PHY_ERANGE, "spr scores: LDS stack too deep");
   double* ws = reinterpret_cast<double*>(work + pl.off_ws);
-  hipLaunchKernelGGL((k_prune_down_batch<real, R>), dim3((unsigned)(pl.pitch / 64), (unsigned)C), dim3(64 * R),
-                     lds, ctx->stream, reinterpret_cast<const int4*>(work + pl.off_prog),
+  hipLaunchKernelGGL((k_prune_down_batch<real, R>), dim3((unsigned)(pl.pitch / 64), (unsigned)C),
+                     dim3(64 * R), lds, ctx->stream, reinterpret_cast<const int4*>(work + pl.off_prog),
                      reinterpret_cast<const int32_t*>(work + pl.off_fifo), N, tips, stride, n_pat,
                      pl.pitch, P, d_pi, d_pi + K, w, ws, pl.n_waves, depth);
   PHY_LAUNCHED(ctx);

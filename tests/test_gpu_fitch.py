@@ -97,6 +97,21 @@ def test_tree_passes_match_oracle(ctx, rng, N, S):
         _tree_case(ctx, N, ch, root, leaves, w)
 
 
+@pytest.mark.parametrize("batch", ["8", "16", "32"])
+@pytest.mark.parametrize("split", ["0", "1"])
+def test_prefetch_depth_and_split_forms(ctx, rng, monkeypatch, batch, split):
+    # the whole-tree and the split (parts side by side, then the top) forms,
+    # at every operand prefetch depth, give the oracle's bits
+    monkeypatch.setenv("PHY_FITCH_BATCH", batch)
+    monkeypatch.setenv("PHY_FITCH_SPLIT", split)
+    N, S = 300, 3001
+    _say(f"fitch forms batch={batch} split={split}")
+    ch, root = random_tree(rng, N)
+    leaves = rng.integers(1, 16, (N, S)).astype(np.uint8)
+    w = rng.integers(1, 5, S).astype(np.uint32)
+    _tree_case(ctx, N, ch, root, leaves, w)
+
+
 def test_config1_full_size_bit_exact(ctx):
     """1,024-taxon Yule tree, 10^6 IUPAC sites (the parsimony benchmark
     workload): every preliminary and final set, every site cost and the total
