@@ -32,7 +32,7 @@ enum : int { UP_A_LEAF = 4, UP_B_LEAF = 8, UP_CONT_B = 16, UP_PUSH = 32, UP_A_CU
 // (one launch, grid.y = part) plus a top program with the cut nodes as
 // leaves.  Down: parts 0..P-1, then the top at index P.  Up: the top at
 // index 0, then the parts.  Programs and padded fifos are concatenated.
-constexpr int kMaxParts = 32;  // the default is 8 (PHY_FITCH_PARTS at topology creation)
+constexpr int kMaxParts = 32;  // the default is 16 (PHY_FITCH_PARTS at topology creation)
 struct SplitProg {
   std::vector<int32_t> prog, fifo, prog_off, n_ops, fifo_off;
   int32_t depth = 0;

@@ -295,7 +295,7 @@ int phy_compile_topo(phy_topo* t) {
   // ---- split programs: parts under cut nodes run side by side ----
   t->parts.clear();
   const char* env_parts = getenv("PHY_FITCH_PARTS");
-  const int max_parts = env_parts ? std::max(1, std::min(kMaxParts, atoi(env_parts))) : 8;
+  const int max_parts = env_parts ? std::max(1, std::min(kMaxParts, atoi(env_parts))) : 16;
   const int want = (int)std::min<int32_t>(max_parts, N / 64);
   if (want >= 2) {
     const std::vector<int32_t> cuts = choose_cuts(N, ch, t->subtree, t->root, want, 16);

@@ -500,16 +500,18 @@ int downpass_launch(phy_ctx* ctx, const phy_topo* topo, uint32_t* s32, int64_t s
                     int64_t W32, int64_t n_sites, const uint32_t* wt, uint32_t* site_cost,
                     unsigned long long* node_cost, unsigned long long* total, int bs) {
   const int N = topo->n_leaves;
-  const int64_t g = (W32 + bs - 1) / bs;
+  // the block shrinks (down to one wave) until the ring and stack fit in LDS
+  auto grid = [&]() { return (unsigned)((W32 + bs - 1) / bs); };
   if (!use_split(topo)) {
+    while (bs > 64 && (size_t)(2 * BATCH + topo->down_depth) * bs * sizeof(uint32_t) > 64 * 1024) bs /= 2;
     const size_t lds = (size_t)(2 * BATCH + topo->down_depth) * bs * sizeof(uint32_t);
     if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "downpass: LDS stack too deep for this tree");
     if (node_cost)
-      hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_WHOLE>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
+      hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_WHOLE>), dim3(grid()), dim3(bs), lds, ctx->stream,
                          topo->d_down_prog, N - 1, topo->d_down_fifo, nullptr, 0, nullptr, s32, stride32, W32,
                          n_sites, N, wt, site_cost, node_cost, total);
     else
-      hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_WHOLE>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
+      hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_WHOLE>), dim3(grid()), dim3(bs), lds, ctx->stream,
                          topo->d_down_prog, N - 1, topo->d_down_fifo, nullptr, 0, nullptr, s32, stride32, W32,
                          n_sites, N, wt, site_cost, node_cost, total);
     PHY_LAUNCHED(ctx);
@@ -529,22 +531,23 @@ int downpass_launch(phy_ctx* ctx, const phy_topo* topo, uint32_t* s32, int64_t s
     topo->scratch_bytes = need;
   }
   uint4* scr = reinterpret_cast<uint4*>(topo->d_scratch);
+  while (bs > 64 && (size_t)(2 * BATCH + sp.depth) * bs * sizeof(uint32_t) > 64 * 1024) bs /= 2;
   const size_t lds = (size_t)(2 * BATCH + sp.depth) * bs * sizeof(uint32_t);
   if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "downpass: LDS stack too deep for this tree");
   if (node_cost) {
-    hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_PART>), dim3((unsigned)g, (unsigned)P), dim3(bs), lds,
+    hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_PART>), dim3(grid(), (unsigned)P), dim3(bs), lds,
                        ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites,
                        N, wt, site_cost, node_cost, total);
     PHY_LAUNCHED(ctx);
-    hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_TOP>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
+    hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_TOP>), dim3(grid()), dim3(bs), lds, ctx->stream,
                        sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites, N, wt,
                        site_cost, node_cost, total);
   } else {
-    hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_PART>), dim3((unsigned)g, (unsigned)P), dim3(bs), lds,
+    hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_PART>), dim3(grid(), (unsigned)P), dim3(bs), lds,
                        ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites,
                        N, wt, site_cost, node_cost, total);
     PHY_LAUNCHED(ctx);
-    hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_TOP>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
+    hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_TOP>), dim3(grid()), dim3(bs), lds, ctx->stream,
                        sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites, N, wt,
                        site_cost, node_cost, total);
   }
@@ -556,11 +559,12 @@ template <int BATCH>
 int uppass_launch(phy_ctx* ctx, const phy_topo* topo, const uint32_t* s32, uint32_t* f32,
                   int64_t stride32, int64_t W32, int bs) {
   const int N = topo->n_leaves;
-  const int64_t g = (W32 + bs - 1) / bs;
+  auto grid = [&]() { return (unsigned)((W32 + bs - 1) / bs); };
   if (!use_split(topo)) {
+    while (bs > 64 && (size_t)(2 * BATCH + 2 * topo->up_depth) * bs * sizeof(uint32_t) > 64 * 1024) bs /= 2;
     const size_t lds = (size_t)(2 * BATCH + 2 * topo->up_depth) * bs * sizeof(uint32_t);
     if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "uppass: LDS stack too deep for this tree");
-    hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_WHOLE>), dim3((unsigned)g), dim3(bs), lds, ctx->stream,
+    hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_WHOLE>), dim3(grid()), dim3(bs), lds, ctx->stream,
                        topo->d_up_prog, N - 1, topo->d_up_fifo, nullptr, 0, nullptr, s32, f32, stride32,
                        W32, N);
     PHY_LAUNCHED(ctx);
@@ -568,12 +572,13 @@ int uppass_launch(phy_ctx* ctx, const phy_topo* topo, const uint32_t* s32, uint3
   }
   const SplitProg& sp = topo->split_up;
   const int P = (int)topo->parts.size();
+  while (bs > 64 && (size_t)(2 * BATCH + 2 * sp.depth) * bs * sizeof(uint32_t) > 64 * 1024) bs /= 2;
   const size_t lds = (size_t)(2 * BATCH + 2 * sp.depth) * bs * sizeof(uint32_t);
   if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "uppass: LDS stack too deep for this tree");
-  hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_TOP>), dim3((unsigned)g), dim3(bs), lds, ctx->stream, sp.d_prog,
+  hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_TOP>), dim3(grid()), dim3(bs), lds, ctx->stream, sp.d_prog,
                      0, sp.d_fifo, sp.d_meta, P, topo->d_part_parent, s32, f32, stride32, W32, N);
   PHY_LAUNCHED(ctx);
-  hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_PART>), dim3((unsigned)g, (unsigned)P), dim3(bs), lds,
+  hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_PART>), dim3(grid(), (unsigned)P), dim3(bs), lds,
                      ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, topo->d_part_parent, s32, f32,
                      stride32, W32, N);
   PHY_LAUNCHED(ctx);
