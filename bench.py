@@ -43,6 +43,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(prefix):
+    """HBM bytes per step of the kernels whose name starts with `prefix`, from
+    the rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in
+    profiles/hbm_traffic.json (already scaled by the copy calibration); None
+    when that kernel was not profiled."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    hits = [v for name, v in k.items() if name.startswith(prefix)]
+    if not hits:
+        return None
+    return sum(v["read_bytes"] + v["write_bytes"] for v in hits)
+
+
 def cpu_info():
     model = platform.processor() or ""
     try:
@@ -102,7 +119,7 @@ def bench_aa(torch, ctx, n_trees, steps):
            "autotune": tune,
            "rel_err_vs_fp64_oracle": {"max": float(max(errs)), "mean": float(np.mean(errs)),
                                       "trees": [int(i) for i in order]},
-           "roofline": {"bound": "mfma", "achieved": useful_tf, "peak": 2500.0,
+           "roofline": {"bound": "mfma", "traffic": None, "achieved": useful_tf, "peak": 2500.0,
                         "unit": "TFLOP/s", "frac": useful_tf / 2500.0,
                         "executed_tflops": executed_tf,
                         "note": "6,400 useful BF16 MFMA flop per internal pattern.node "
@@ -211,9 +228,10 @@ def main():
             "ms_downpass": ms_down, "ms_uppass": ms_up, "total_cost": total,
             "roofline_downpass": {"bound": "hbm", "achieved": gbs_down, "peak": HBM_PEAK_GBS,
                                   "unit": "GB/s", "frac": gbs_down / HBM_PEAK_GBS,
-                                  "traffic": None},
+                                  "traffic": measured_traffic("k_fitch_down<")},
             "roofline_uppass": {"bound": "hbm", "achieved": gbs_up, "peak": HBM_PEAK_GBS,
-                                "unit": "GB/s", "frac": gbs_up / HBM_PEAK_GBS, "traffic": None},
+                                "unit": "GB/s", "frac": gbs_up / HBM_PEAK_GBS,
+                                "traffic": measured_traffic("k_fitch_up<")},
         }
         log(f"  fitch down {ms_down:.3f} ms ({gbs_down:.0f} GB/s alg), "
             f"up {ms_up:.3f} ms ({gbs_up:.0f} GB/s alg), total {total}")
@@ -275,15 +293,19 @@ def main():
         "config": {"workload": "DNA pruning, 512-taxon Yule tree (bl~Exp(0.05)), 200,000 "
                                "unique GTR+G4 patterns (alpha 0.5, Yang mean-rate "
                                "categories), weights Poisson(3)+1; FP32 partials with "
-                               "per-pattern int32 exponents, fused post-order pass",
+                               "int32 exponents per (pattern, rate), fused post-order pass",
                    "n_taxa": N, "n_patterns": P, "n_rates": 4, "form": "f32_fused"},
         "roofline": {"bound": "hbm", "achieved": alg_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": alg_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": alg_gbs / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("k_prune_down<float, 4, true>"),
                      "compute_tflops": tflops, "compute_peak_tflops": FP32_PEAK_TF,
                      "note": "achieved = 136 B per internal pattern.node (the unfused "
-                             "algorithmic traffic) / step time; the fused kernel keeps "
-                             "interior partials on chip, so frac > 1 is the fusion gain; "
-                             "traffic = measured HBM bytes per launch (rocprofv3)"},
+                             "algorithmic traffic) x 511 nodes x 200,000 patterns / step "
+                             "time; the fused kernel keeps interior partials on chip, so "
+                             "frac > 1 is the fusion gain and the kernel is bound by FP32 "
+                             "VALU issue (compute_tflops); traffic = HBM bytes per launch "
+                             "measured by rocprofv3 FETCH_SIZE + WRITE_SIZE (copy-"
+                             "calibrated, profiles/hbm_traffic.json)"},
         "cpu_baseline": cpu,
         "prune": {
             "ms": res, "ms_derivs_f64": ms_d64, "ms_derivs_f32": ms_d32,
