@@ -181,7 +181,11 @@ int phy_fitch_insertion_costs(phy_ctx* ctx, const uint64_t* d_pre, const uint64_
  *   d_site_cost n_sites uint32: number of union events at each site.
  *   d_node_cost (nullable) N-1 uint64: weighted local cost of each internal
  *               node N+k (NonAdditive_c.t.cost of that node).
- *   d_total     1 uint64: sum_s w_s site_cost_s. */
+ *   d_total     1 uint64: sum_s w_s site_cost_s.
+ * A tree of >= 128 leaves runs split (parts side by side, then the top);
+ * the per-part site counts live in a device buffer owned by the topology,
+ * allocated on the first call and grown with n_sites, so two down-passes of
+ * one topology must not run at the same time on different streams. */
 int phy_fitch_downpass(phy_ctx* ctx, const phy_topo* topo, uint64_t* d_sets,
                        int64_t stride, int64_t n_sites, const uint32_t* d_weights,
                        uint32_t* d_site_cost, uint64_t* d_node_cost,
