@@ -121,6 +121,19 @@ int phy_topo_info(const phy_topo* topo, int32_t* down_stack_depth,
  * argument may be NULL. */
 int phy_topo_programs(const phy_topo* topo, int32_t* down_prog, int32_t* down_fifo,
                       int32_t* up_prog, int32_t* up_fifo);
+/* The split programs of the Fitch passes (for host-side checking).  A tree of
+ * >= 128 leaves is cut under n_parts cut nodes (returned in cut_nodes, at
+ * most 32; n_parts = 0 means the tree runs whole).  Program k of pass `up`
+ * (0 = post-order: parts 0..n_parts-1, then the top at n_parts; 1 =
+ * pre-order: the top at 0, then part k at k + 1) has *n_ops entries (4 ints
+ * each) and *n_fifo operand ids; prog and fifo may be NULL to query the
+ * sizes.  A pre-order part starts with source SRC_EXT = 3 (its parent's final
+ * set is read from memory); the top marks cut children with flag bits 64 (a)
+ * and 128 (b).  No counterpart in the reference: its traversal is
+ * sequential OCaml (lib/tree.ml:123-187). */
+int phy_topo_split(const phy_topo* topo, int32_t* n_parts, int32_t* cut_nodes);
+int phy_topo_split_program(const phy_topo* topo, int32_t up, int32_t k, int32_t* prog,
+                           int32_t* n_ops, int32_t* fifo, int32_t* n_fifo);
 
 /* ------------------------------------------------------------------------- */
 /* Fitch parsimony: NodeData operations of NonAdditive_c on nibble blocks.   */

@@ -49,6 +49,8 @@ SIGNATURES = {
     "phy_topo_destroy": (None, [vp]),
     "phy_topo_info": (C.c_int, [vp, P_i32, P_i32, P_i32]),
     "phy_topo_programs": (C.c_int, [vp, P_i32, P_i32, P_i32, P_i32]),
+    "phy_topo_split": (C.c_int, [vp, P_i32, P_i32]),
+    "phy_topo_split_program": (C.c_int, [vp, C.c_int32, C.c_int32, P_i32, P_i32, P_i32, P_i32]),
     "phy_fitch_median2": (C.c_int, [vp, vp, vp, vp, i64, vp, vp]),
     "phy_fitch_median3": (C.c_int, [vp, vp, vp, vp, vp, i64, vp, vp]),
     "phy_fitch_distance1": (C.c_int, [vp, vp, vp, i64, vp, vp]),

@@ -456,6 +456,34 @@ int phy_topo_programs(const phy_topo* t, int32_t* down_prog, int32_t* down_fifo,
   return PHY_OK;
 }
 
+int phy_topo_split(const phy_topo* t, int32_t* n_parts, int32_t* cut_nodes) {
+  if (!t) return PHY_EINVAL;
+  if (n_parts) *n_parts = (int32_t)t->parts.size();
+  if (cut_nodes) std::copy(t->parts.begin(), t->parts.end(), cut_nodes);
+  return PHY_OK;
+}
+
+int phy_topo_split_program(const phy_topo* t, int32_t up, int32_t k, int32_t* prog, int32_t* n_ops,
+                           int32_t* fifo, int32_t* n_fifo) {
+  if (!t || t->parts.empty()) return PHY_EINVAL;
+  const SplitProg& sp = up ? t->split_up : t->split_down;
+  if (k < 0 || k > (int32_t)t->parts.size()) return PHY_EINVAL;
+  const int32_t n = sp.n_ops[k];
+  // the fifo of program k ends where the next one's padded fifo begins; its
+  // unpadded length is the number of operands the program reads
+  int32_t nf = 0;
+  for (int32_t e = 0; e < n; ++e) {
+    const int fl = sp.prog[4 * (sp.prog_off[k] + e) + 3];
+    if (up) nf += 2 + (((fl & 3) == SRC_LEAF || (fl & 3) == SRC_EXT) ? 1 : 0);
+    else nf += (((fl & 3) == SRC_LEAF) ? 1 : 0) + ((((fl >> 2) & 3) == SRC_LEAF) ? 1 : 0);
+  }
+  if (n_ops) *n_ops = n;
+  if (n_fifo) *n_fifo = nf;
+  if (prog) std::copy(sp.prog.begin() + 4 * (size_t)sp.prog_off[k], sp.prog.begin() + 4 * ((size_t)sp.prog_off[k] + n), prog);
+  if (fifo) std::copy(sp.fifo.begin() + sp.fifo_off[k], sp.fifo.begin() + sp.fifo_off[k] + nf, fifo);
+  return PHY_OK;
+}
+
 int phy_topo_info(const phy_topo* t, int32_t* dd, int32_t* ud, int32_t* parent_out) {
   if (!t) return PHY_EINVAL;
   if (dd) *dd = t->down_depth;

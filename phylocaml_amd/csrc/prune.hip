@@ -499,6 +499,17 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
+// Sum over the lanes of a 16-lane row that are congruent mod S (S = 1, 2, 4,
+// 8): each such lane receives the sum, via DPP row rotations.
+template <int S>
+__device__ __forceinline__ double row_sum_stride(double v) {
+  if (S <= 8) v += dpp<0x128>(v);  // row_ror:8
+  if (S <= 4) v += dpp<0x124>(v);  // row_ror:4
+  if (S <= 2) v += dpp<0x122>(v);  // row_ror:2
+  if (S <= 1) v += dpp<0x121>(v);  // row_ror:1
+  return v;
+}
+
 // ---------------------------------------------------------------------------
 // Pre-order derivative pass: for every branch c (parent p, sibling s)
 //   A_c = M_p (*) (P_s D_s),  M_root = pi,  M_c = A_c^T P_c
@@ -555,7 +566,8 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
     int64_t n_pat, int64_t pitch, const real* __restrict__ tab, const real* __restrict__ E,
     const double* __restrict__ pi, const double* __restrict__ probs, const real* __restrict__ part,
     const int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
-    double* __restrict__ wsum, int64_t n_waves, int N) {
+    double* __restrict__ wsum, int64_t n_slots, int N) {
+  constexpr int QPL = R >= 4 ? 1 : 4 / R;  // derivative quantities per lane
   extern __shared__ unsigned char lds_raw[];
   real* stk = reinterpret_cast<real*>(lds_raw);  // [depth][4][bd]
   const int tid = threadIdx.x, bd = blockDim.x;
@@ -677,24 +689,36 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
       const double L0 = rate_sum<R>(pr * (double)s0);
       const double L1a = rate_sum<R>(pr * (double)s1a), L2a = rate_sum<R>(pr * (double)s2a);
       const double L1b = rate_sum<R>(pr * (double)s1b), L2b = rate_sum<R>(pr * (double)s2b);
-      double dga = 0, dha = 0, dgb = 0, dhb = 0;
-      if (act && r == 0) {
+      // The four per-pattern quantities q = (g_a, h_a, g_b, h_b) are spread
+      // over the R lanes of the pattern (lane r holds q = r * QPL + j), summed
+      // over the lanes of each 16-lane row that hold the same q (DPP row
+      // rotations), and each row's sums are written to their own slot.
+      {
         const double inv = 1.0 / L0;
-        const double g1 = L1a * inv, g2 = L1b * inv;
-        dga = wp * g1;
-        dha = wp * (L2a * inv - g1 * g1);
-        dgb = wp * g2;
-        dhb = wp * (L2b * inv - g2 * g2);
-      }
-      dga = wave_sum_d(dga);
-      dha = wave_sum_d(dha);
-      dgb = wave_sum_d(dgb);
-      dhb = wave_sum_d(dhb);
-      if (lane == 0) {
-        wsum[((size_t)a * 2) * n_waves + wave] = dga;
-        wsum[((size_t)a * 2 + 1) * n_waves + wave] = dha;
-        wsum[((size_t)b * 2) * n_waves + wave] = dgb;
-        wsum[((size_t)b * 2 + 1) * n_waves + wave] = dhb;
+        const double ga1 = L1a * inv, gb1 = L1b * inv;
+        double qv[QPL];
+#pragma unroll
+        for (int j = 0; j < QPL; ++j) {
+          const int q = r * QPL + j;
+          double v = 0.0;
+          if (act && q < 4) {
+            const double g = (q < 2) ? ga1 : gb1;
+            const double l2 = (q < 2) ? L2a : L2b;
+            v = wp * ((q & 1) ? (l2 * inv - g * g) : g);
+          }
+          qv[j] = row_sum_stride<R>(v);
+        }
+        if ((lane & 15) < R) {
+          const int64_t slot = wave * 4 + (lane >> 4);
+#pragma unroll
+          for (int j = 0; j < QPL; ++j) {
+            const int q = r * QPL + j;
+            if (q < 4) {
+              const int edge = (q < 2) ? a : b;
+              wsum[((size_t)edge * 2 + (q & 1)) * n_slots + slot] = qv[j];
+            }
+          }
+        }
       }
       // messages to internal children: M_c = Ui^T (e_c (*) g_c)
       real Ma[4], Mb[4];
@@ -965,7 +989,7 @@ Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t for
   pl.n_blocks = pl.pitch / 64;
   pl.dblocks = pl.dbs ? pl.pitch * R / pl.dbs : 0;
   pl.n_waves = pl.pitch / 64;          // partial lnL sums: one per 64 patterns
-  pl.n_dwaves = pl.pitch * R / 64;     // derivative sums: one per wave
+  pl.n_dwaves = pl.pitch * R / 64 * 4;  // derivative sums: one per 16-lane row
   size_t o = 0;
   const size_t pm = (size_t)M * R * K * K * pl.esz;
   pl.off_P = o; o = align256(o + pm);

@@ -126,6 +126,26 @@ class Topology:
                                     ptr(up, C.c_int32), ptr(uf, C.c_int32)))
         return dp.reshape(-1, 4), df, up.reshape(-1, 4), uf
 
+    def split(self):
+        """(cut nodes, {("down"|"up", k): (entries [n, 4], operand ids)}) of the
+        split Fitch programs; no cut nodes when the tree runs whole."""
+        n = C.c_int32()
+        cuts = np.zeros(32, np.int32)
+        check(lib.phy_topo_split(self.handle, C.byref(n), ptr(cuts, C.c_int32)))
+        cuts = cuts[: n.value]
+        progs = {}
+        for up, name in ((0, "down"), (1, "up")):
+            for k in range(len(cuts) + 1 if len(cuts) else 0):
+                no, nf = C.c_int32(), C.c_int32()
+                check(lib.phy_topo_split_program(self.handle, up, k, None, C.byref(no), None,
+                                                 C.byref(nf)))
+                pg = np.zeros(4 * no.value, np.int32)
+                ff = np.zeros(max(nf.value, 1), np.int32)
+                check(lib.phy_topo_split_program(self.handle, up, k, ptr(pg, C.c_int32),
+                                                 C.byref(no), ptr(ff, C.c_int32), C.byref(nf)))
+                progs[(name, k)] = (pg.reshape(-1, 4), ff[: nf.value])
+        return cuts, progs
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             lib.phy_topo_destroy(self.handle)

@@ -92,8 +92,8 @@ def _interp_down(N, dprog, dfifo, leaves):
     return sets
 
 
-def _interp_up(N, uprog, ufifo, sets):
-    fin = {}
+def _interp_up(N, uprog, ufifo, sets, fin=None, parent_of_root=None, expect=None):
+    fin = {} if fin is None else fin
     accP = accF = None
     stack, f = [], 0
 
@@ -114,6 +114,8 @@ def _interp_up(N, uprog, ufifo, sets):
         if src == 0:
             pv = nxt()
             fp = None
+        elif src == 3:  # a part's root: the parent's final from memory
+            pv, fp = nxt(), fin[parent_of_root]
         elif src == 1:
             pv, fp = accP, accF
         else:
@@ -128,7 +130,7 @@ def _interp_up(N, uprog, ufifo, sets):
         if fl & 32:
             stack.append(((pa if fl & 16 else pb), fv))
         accP, accF = (pb if fl & 16 else pa), fv
-    assert not stack and f == 2 * N - 1
+    assert not stack and f == (2 * N - 1 if expect is None else expect)
     return fin
 
 
@@ -159,6 +161,72 @@ def test_programs_reproduce_oracle(rng, N):
         ofin = O.fitch_uppass(N, ch, root, osets)
         allsets = {v: osets[v] for v in range(2 * N - 1)}
         fin = _interp_up(N, uprog, ufifo, allsets)
+        for v in range(2 * N - 1):
+            assert np.array_equal(fin[v], ofin[v]), (kind, v)
+
+
+@pytest.mark.parametrize("N", [64, 127, 128, 300, 1024])
+def test_split_programs_reproduce_oracle(rng, N):
+    # parts side by side, then the top (down); the top, then the parts (up):
+    # every internal node once, and the oracle's sets
+    for kind in ("random", "yule", "caterpillar"):
+        if kind == "random":
+            ch, root = random_tree(rng, N)
+        elif kind == "caterpillar":
+            ch, root = caterpillar(N)
+        else:
+            t = synth.yule(N, int(rng.integers(1 << 30)), 0.1)
+            ch, root = t.children, t.root
+        topo = D.Topology(N, ch, root)
+        cuts, progs = topo.split()
+        if N < 128 or kind == "caterpillar":
+            assert len(cuts) == 0  # too small / no balanced cut: the tree runs whole
+            continue
+        assert 2 <= len(cuts) <= 16
+        P = len(cuts)
+        dd, ud, parent = topo.info()
+        leaves = rng.integers(1, 16, (N, 23)).astype(np.uint8)
+        osets, *_ = O.fitch_downpass(N, ch, root, leaves)
+        ofin = O.fitch_uppass(N, ch, root, osets)
+        # down: parts, then the top reading the cut nodes' rows
+        seen = []
+        sets = {v: osets[v] for v in range(N)}
+        for k in range(P + 1):
+            prog, fifo = progs[("down", k)]
+            seen += list(prog[:, 0])
+            rows = np.array([sets[v] for v in range(N)] + [np.zeros(23, np.uint8)] * (N - 1))
+            for v, s_ in sets.items():
+                rows[v] = s_
+            acc, stack, f = None, [], 0
+            for v, A, B, fl in prog:
+                if fl & 16:
+                    stack.append(acc)
+                sa, sb = fl & 3, (fl >> 2) & 3
+                x = rows[fifo[f]] if sa == 0 else (acc if sa == 1 else stack.pop())
+                f += sa == 0
+                y = rows[fifo[f]] if sb == 0 else acc
+                f += sb == 0
+                tt = x & y
+                acc = np.where(tt != 0, tt, x | y).astype(np.uint8)
+                sets[v] = acc
+            assert not stack and f == len(fifo)
+            if k < P:
+                assert prog[-1, 0] == cuts[k]
+        assert sorted(seen) == list(range(N, 2 * N - 1))
+        for v in range(N, 2 * N - 1):
+            assert np.array_equal(sets[v], osets[v]), v
+        # up: the top, then every part from its parent's final
+        allsets = {v: osets[v] for v in range(2 * N - 1)}
+        prog, fifo = progs[("up", 0)]
+        fin = _interp_up(N, prog, fifo, allsets, expect=len(fifo))
+        seen = list(prog[:, 0])
+        for k in range(P):
+            prog, fifo = progs[("up", k + 1)]
+            assert prog[0, 0] == cuts[k] and (prog[0, 3] & 3) == 3
+            fin = _interp_up(N, prog, fifo, allsets, fin=fin, parent_of_root=parent[cuts[k]],
+                             expect=len(fifo))
+            seen += list(prog[:, 0])
+        assert sorted(seen) == list(range(N, 2 * N - 1))
         for v in range(2 * N - 1):
             assert np.array_equal(fin[v], ofin[v]), (kind, v)
 
