@@ -19,6 +19,14 @@ if not os.path.exists(LIB_PATH):
         "(python -m phylocaml_amd.build)"
     )
 
+# torch brings its own HIP runtime; load it first so that the library binds to
+# the same runtime instance as the tensors and streams it is handed (loaded
+# the other way round, the library's HIP calls fail with no device).
+try:
+    import torch as _torch  # noqa: F401
+except ImportError:  # host-only use (programs, synthetic data, oracle checks)
+    _torch = None
+
 lib = C.CDLL(LIB_PATH)
 
 i32, i64, u64, f64, f32 = C.c_int32, C.c_int64, C.c_uint64, C.c_double, C.c_float
