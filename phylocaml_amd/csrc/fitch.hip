@@ -200,15 +200,23 @@ __device__ __forceinline__ uint64_t weigh32(uint32_t e, const uint32_t* __restri
   return c;
 }
 
+// V 32-bit words (8 sites each) per thread, loaded and stored as one
+// 4V-byte access
+template <int V>
+struct alignas(4 * V) Words {
+  uint32_t w[V];
+};
+
+template <int V>
 struct SetLoad {
-  const uint32_t* sets;
-  int64_t stride, wc;
-  __device__ __forceinline__ uint32_t operator()(int id) const { return sets[(int64_t)id * stride + wc]; }
+  const Words<V>* sets;
+  int64_t stride, wc;  // in units of V words
+  __device__ __forceinline__ Words<V> operator()(int id) const { return sets[(int64_t)id * stride + wc]; }
 };
 
 // Pass modes: the whole tree in one program, or a tree split at cut nodes:
 // the parts side by side (grid.y = part; per-site counts of a part go to
-// scratch[part][lane][8]) and then the top, which adds the parts' counts.
+// scratch[part][lane][8V]) and then the top, which adds the parts' counts.
 enum : int { MODE_WHOLE = 0, MODE_PART = 1, MODE_TOP = 2 };
 
 // program k of a split program: meta = prog_off | n_ops | fifo_off, each
@@ -220,32 +228,43 @@ enum : int { MODE_WHOLE = 0, MODE_PART = 1, MODE_TOP = 2 };
     fifo_ids += meta[2 * (n_parts + 1) + (k)];          \
   } while (0)
 
-template <int BATCH, bool NODE_COST, int MODE>
+// W = number of 32-bit words per node row; lane w owns words V*w .. V*w+V-1
+template <int BATCH, bool NODE_COST, int MODE, int V>
 __global__ __launch_bounds__(256) void k_fitch_down(
     const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ fifo_ids,
     const int32_t* __restrict__ meta, int n_parts, uint4* __restrict__ scratch,
-    uint32_t* __restrict__ sets, int64_t stride, int64_t W, int64_t n_sites, int N,
+    uint32_t* __restrict__ sets32, int64_t stride, int64_t W, int64_t n_sites, int N,
     const uint32_t* __restrict__ wt, uint32_t* __restrict__ site_cost,
     unsigned long long* __restrict__ node_cost, unsigned long long* __restrict__ total) {
-  extern __shared__ uint32_t lds32[];
+  using T = Words<V>;
+  extern __shared__ uint4 lds_dyn[];
   const int tid = threadIdx.x, bd = blockDim.x;
   const int part = MODE == MODE_PART ? (int)blockIdx.y : n_parts;
   if (MODE != MODE_WHOLE) SPLIT_SELECT(part);
-  const int64_t w = (int64_t)blockIdx.x * bd + tid;  // 32-bit lane index
-  const bool act = w < W;                              // W = 2 * words
+  const int64_t L = W / V;                           // lanes (W is a multiple of V)
+  const int64_t w = (int64_t)blockIdx.x * bd + tid;  // lane index
+  const bool act = w < L;
   const int64_t wc = act ? w : 0;
-  uint32_t* stk = lds32 + (size_t)2 * BATCH * bd;
-  OperandRing<BATCH, SetLoad> leaves;
-  leaves.init(lds32, fifo_ids, SetLoad{sets, stride, wc}, bd, tid);
+  T* sets = reinterpret_cast<T*>(sets32);
+  const int64_t strideT = stride / V;
+  T* ring = reinterpret_cast<T*>(lds_dyn);
+  T* stk = ring + (size_t)2 * BATCH * bd;
+  OperandRing<BATCH, SetLoad<V>, T> leaves;
+  leaves.init(ring, fifo_ids, SetLoad<V>{sets, strideT, wc}, bd, tid);
 
-  uint32_t cnt = 0;  // nibble counters, flushed every 15 nodes
-  uint32_t site[8];
+  uint32_t cnt[V];  // nibble counters, flushed every 15 nodes
+  uint32_t site[8 * V];
+  uint32_t vm[V];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) site[k] = 0;
+  for (int i = 0; i < V; ++i) {
+    cnt[i] = 0;
+    vm[i] = valid_nibbles32(wc * V + i, n_sites);
+  }
+#pragma unroll
+  for (int k = 0; k < 8 * V; ++k) site[k] = 0;
   int since = 0;
-  uint32_t acc = 0;
+  T acc = {};
   int sp = 0;
-  const uint32_t vm = valid_nibbles32(wc, n_sites);
   for_each_entry(prog, n_ops, [&](int, const int4 op) {
     const int fl = op.w;
     if (fl & DOWN_SPILL) {
@@ -253,71 +272,92 @@ __global__ __launch_bounds__(256) void k_fitch_down(
       ++sp;
     }
     const int sa = fl & 3, sb = (fl >> 2) & 3;
-    uint32_t x, y;
+    T x, y;
     if (sa == SRC_LEAF) x = leaves.get();
     else if (sa == SRC_ACC) x = acc;
     else { --sp; x = stk[sp * bd + tid]; }
     if (sb == SRC_LEAF) y = leaves.get();
     else y = acc;  // SRC_ACC (B is never taken from the stack)
-    uint32_t r;
-    const uint32_t e = fitch32(x, y, &r);
-    acc = r;
-    if (act) sets[(int64_t)op.x * stride + w] = acc;
-    cnt += e;
+    uint32_t e[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      uint32_t r;
+      e[i] = fitch32(x.w[i], y.w[i], &r);
+      acc.w[i] = r;
+      cnt[i] += e[i];
+    }
+    if (act) sets[(int64_t)op.x * strideT + w] = acc;
     if (NODE_COST) {
-      const uint32_t m = e & vm;
       uint64_t c = 0;
-      if (act) c = wt ? weigh32(m, wt, w) : (uint64_t)__popc(m);
+      if (act) {
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+          const uint32_t m = e[i] & vm[i];
+          c += wt ? weigh32(m, wt, w * V + i) : (uint64_t)__popc(m);
+        }
+      }
       c = wave_sum(c);
       if ((tid & 63) == 0 && c) atomicAdd(&node_cost[op.x - N], (unsigned long long)c);
     }
     if (++since == 15) {
 #pragma unroll
-      for (int k = 0; k < 8; ++k) site[k] += (cnt >> (4 * k)) & 15u;
-      cnt = 0;
+      for (int i = 0; i < V; ++i) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) site[8 * i + k] += (cnt[i] >> (4 * k)) & 15u;
+        cnt[i] = 0;
+      }
       since = 0;
     }
   });
 #pragma unroll
-  for (int k = 0; k < 8; ++k) site[k] += (cnt >> (4 * k)) & 15u;
+  for (int i = 0; i < V; ++i)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) site[8 * i + k] += (cnt[i] >> (4 * k)) & 15u;
   if (MODE == MODE_PART) {
     if (act) {
-      uint4* dst = scratch + ((size_t)part * W + w) * 2;
-      dst[0] = make_uint4(site[0], site[1], site[2], site[3]);
-      dst[1] = make_uint4(site[4], site[5], site[6], site[7]);
+      uint4* dst = scratch + ((size_t)part * L + w) * 2 * V;
+#pragma unroll
+      for (int q = 0; q < 2 * V; ++q)
+        dst[q] = make_uint4(site[4 * q], site[4 * q + 1], site[4 * q + 2], site[4 * q + 3]);
     }
     return;
   }
   if (MODE == MODE_TOP && act) {
-    for (int q = 0; q < n_parts; ++q) {
-      const uint4* src = scratch + ((size_t)q * W + w) * 2;
-      const uint4 x = src[0], y = src[1];
-      site[0] += x.x; site[1] += x.y; site[2] += x.z; site[3] += x.w;
-      site[4] += y.x; site[5] += y.y; site[6] += y.z; site[7] += y.w;
+    for (int pq = 0; pq < n_parts; ++pq) {
+      const uint4* src = scratch + ((size_t)pq * L + w) * 2 * V;
+#pragma unroll
+      for (int q = 0; q < 2 * V; ++q) {
+        const uint4 x = src[q];
+        site[4 * q] += x.x;
+        site[4 * q + 1] += x.y;
+        site[4 * q + 2] += x.z;
+        site[4 * q + 3] += x.w;
+      }
     }
   }
   uint64_t tot = 0;
   if (act) {
-    const int64_t s0 = w * 8;
-    if (s0 + 8 <= n_sites) {
+    const int64_t s0 = w * 8 * V;
+    if (s0 + 8 * V <= n_sites) {
       uint4* dst = reinterpret_cast<uint4*>(site_cost + s0);
-      dst[0] = make_uint4(site[0], site[1], site[2], site[3]);
-      dst[1] = make_uint4(site[4], site[5], site[6], site[7]);
+#pragma unroll
+      for (int q = 0; q < 2 * V; ++q)
+        dst[q] = make_uint4(site[4 * q], site[4 * q + 1], site[4 * q + 2], site[4 * q + 3]);
       if (wt) {
         const uint4* ws = reinterpret_cast<const uint4*>(wt + s0);
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < 2 * V; ++q) {
           const uint4 xw = ws[q];
           tot += (uint64_t)site[4 * q] * xw.x + (uint64_t)site[4 * q + 1] * xw.y +
                  (uint64_t)site[4 * q + 2] * xw.z + (uint64_t)site[4 * q + 3] * xw.w;
         }
       } else {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) tot += site[k];
+        for (int k = 0; k < 8 * V; ++k) tot += site[k];
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
+      for (int k = 0; k < 8 * V; ++k)
         if (s0 + k < n_sites) {
           site_cost[s0 + k] = site[k];
           tot += (uint64_t)site[k] * (wt ? wt[s0 + k] : 1u);
@@ -328,54 +368,65 @@ __global__ __launch_bounds__(256) void k_fitch_down(
 }
 
 // ---------------------------------------------------------------------------
-// Whole-tree up-pass (32-bit lanes)
+// Whole-tree up-pass
 // ---------------------------------------------------------------------------
-template <int BATCH, int MODE>
+template <int BATCH, int MODE, int V>
 __global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog, int n_ops,
                                                   const int32_t* __restrict__ fifo_ids,
                                                   const int32_t* __restrict__ meta, int n_parts,
                                                   const int32_t* __restrict__ part_parent,
-                                                  const uint32_t* __restrict__ sets,
-                                                  uint32_t* __restrict__ fin, int64_t stride,
+                                                  const uint32_t* __restrict__ sets32,
+                                                  uint32_t* __restrict__ fin32, int64_t stride,
                                                   int64_t W, int N) {
-  extern __shared__ uint32_t lds32[];
+  using T = Words<V>;
+  extern __shared__ uint4 lds_dyn[];
   const int tid = threadIdx.x, bd = blockDim.x;
   // up: the top is program 0, part k is program 1 + k
   const int part = MODE == MODE_PART ? (int)blockIdx.y : -1;
   if (MODE != MODE_WHOLE) SPLIT_SELECT(part + 1);
+  const int64_t L = W / V;
   const int64_t w = (int64_t)blockIdx.x * bd + tid;
-  const bool act = w < W;
+  const bool act = w < L;
   const int64_t wc = act ? w : 0;
-  uint32_t* stk = lds32 + (size_t)2 * BATCH * bd;  // pairs (P, F)
-  OperandRing<BATCH, SetLoad> nodes;
-  nodes.init(lds32, fifo_ids, SetLoad{sets, stride, wc}, bd, tid);
+  const T* sets = reinterpret_cast<const T*>(sets32);
+  T* fin = reinterpret_cast<T*>(fin32);
+  const int64_t strideT = stride / V;
+  T* ring = reinterpret_cast<T*>(lds_dyn);
+  T* stk = ring + (size_t)2 * BATCH * bd;  // pairs (P, F)
+  OperandRing<BATCH, SetLoad<V>, T> nodes;
+  nodes.init(ring, fifo_ids, SetLoad<V>{sets, strideT, wc}, bd, tid);
 
-  uint32_t accP = 0, accF = 0;
+  T accP = {}, accF = {};
   int sp = 0;
   for_each_entry(prog, n_ops, [&](int, const int4 op) {
     const int fl = op.w;
     const int src = fl & 3;
-    uint32_t pv, fp = 0;
+    T pv, fp = {};
     if (src == SRC_LEAF) pv = nodes.get();
     else if (src == SRC_ACC) { pv = accP; fp = accF; }
     else if (MODE == MODE_PART && src == SRC_EXT) {  // a part's root: parent final from the top pass
       pv = nodes.get();
-      fp = fin[(int64_t)part_parent[part] * stride + wc];
+      fp = fin[(int64_t)part_parent[part] * strideT + wc];
     }
     else {
       --sp;
       pv = stk[(2 * sp) * bd + tid];
       fp = stk[(2 * sp + 1) * bd + tid];
     }
-    const uint32_t pa = nodes.get();
-    const uint32_t pb = nodes.get();
-    const uint32_t fv = (src == SRC_LEAF) ? pv : final32(pv, pa, pb, fp);
-    if (act) fin[(int64_t)op.x * stride + w] = fv;
-    if (fl & UP_A_LEAF) { if (act) fin[(int64_t)op.y * stride + w] = leaf32(fv, pa); }
-    if (fl & UP_B_LEAF) { if (act) fin[(int64_t)op.z * stride + w] = leaf32(fv, pb); }
+    const T pa = nodes.get();
+    const T pb = nodes.get();
+    T fv, la, lb;
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      fv.w[i] = (src == SRC_LEAF) ? pv.w[i] : final32(pv.w[i], pa.w[i], pb.w[i], fp.w[i]);
+      la.w[i] = leaf32(fv.w[i], pa.w[i]);
+      lb.w[i] = leaf32(fv.w[i], pb.w[i]);
+    }
+    if (act) fin[(int64_t)op.x * strideT + w] = fv;
+    if (fl & UP_A_LEAF) { if (act) fin[(int64_t)op.y * strideT + w] = la; }
+    if (fl & UP_B_LEAF) { if (act) fin[(int64_t)op.z * strideT + w] = lb; }
     if (fl & UP_PUSH) {
-      const uint32_t other = (fl & UP_CONT_B) ? pa : pb;
-      stk[(2 * sp) * bd + tid] = other;
+      stk[(2 * sp) * bd + tid] = (fl & UP_CONT_B) ? pa : pb;
       stk[(2 * sp + 1) * bd + tid] = fv;
       ++sp;
     }
@@ -495,23 +546,27 @@ bool use_split(const phy_topo* t) {
   return t->parts.size() >= 2 && t->split_down.d_prog && !(e && e[0] == '0');
 }
 
-template <int BATCH>
+template <int BATCH, int V>
 int downpass_launch(phy_ctx* ctx, const phy_topo* topo, uint32_t* s32, int64_t stride32,
                     int64_t W32, int64_t n_sites, const uint32_t* wt, uint32_t* site_cost,
                     unsigned long long* node_cost, unsigned long long* total, int bs) {
   const int N = topo->n_leaves;
-  // the block shrinks (down to one wave) until the ring and stack fit in LDS
-  auto grid = [&]() { return (unsigned)((W32 + bs - 1) / bs); };
-  if (!use_split(topo)) {
-    while (bs > 64 && (size_t)(2 * BATCH + topo->down_depth) * bs * sizeof(uint32_t) > 64 * 1024) bs /= 2;
-    const size_t lds = (size_t)(2 * BATCH + topo->down_depth) * bs * sizeof(uint32_t);
-    if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "downpass: LDS stack too deep for this tree");
+  const bool split = use_split(topo);
+  const int depth = split ? topo->split_down.depth : topo->down_depth;
+  // LDS per thread: the operand ring (2 BATCH entries) and the stack, V words
+  // each; the block shrinks (down to one wave) until they fit
+  auto lds_of = [&](int b) { return (size_t)(2 * BATCH + depth) * b * 4 * V; };
+  while (bs > 64 && lds_of(bs) > 64 * 1024) bs /= 2;
+  const size_t lds = lds_of(bs);
+  if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "downpass: LDS stack too deep for this tree");
+  const unsigned grid = (unsigned)((W32 / V + bs - 1) / bs);
+  if (!split) {
     if (node_cost)
-      hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_WHOLE>), dim3(grid()), dim3(bs), lds, ctx->stream,
+      hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_WHOLE, V>), dim3(grid), dim3(bs), lds, ctx->stream,
                          topo->d_down_prog, N - 1, topo->d_down_fifo, nullptr, 0, nullptr, s32, stride32, W32,
                          n_sites, N, wt, site_cost, node_cost, total);
     else
-      hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_WHOLE>), dim3(grid()), dim3(bs), lds, ctx->stream,
+      hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_WHOLE, V>), dim3(grid), dim3(bs), lds, ctx->stream,
                          topo->d_down_prog, N - 1, topo->d_down_fifo, nullptr, 0, nullptr, s32, stride32, W32,
                          n_sites, N, wt, site_cost, node_cost, total);
     PHY_LAUNCHED(ctx);
@@ -519,7 +574,7 @@ int downpass_launch(phy_ctx* ctx, const phy_topo* topo, uint32_t* s32, int64_t s
   }
   const SplitProg& sp = topo->split_down;
   const int P = (int)topo->parts.size();
-  const size_t need = (size_t)P * W32 * 8 * sizeof(uint32_t);
+  const size_t need = (size_t)P * W32 * 8 * sizeof(uint32_t);  // 8 counts per 32-bit word
   if (topo->scratch_bytes < need) {
     if (topo->d_scratch) PHY_HIP(ctx, hipFree(topo->d_scratch));
     topo->d_scratch = nullptr;
@@ -531,23 +586,20 @@ int downpass_launch(phy_ctx* ctx, const phy_topo* topo, uint32_t* s32, int64_t s
     topo->scratch_bytes = need;
   }
   uint4* scr = reinterpret_cast<uint4*>(topo->d_scratch);
-  while (bs > 64 && (size_t)(2 * BATCH + sp.depth) * bs * sizeof(uint32_t) > 64 * 1024) bs /= 2;
-  const size_t lds = (size_t)(2 * BATCH + sp.depth) * bs * sizeof(uint32_t);
-  if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "downpass: LDS stack too deep for this tree");
   if (node_cost) {
-    hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_PART>), dim3(grid(), (unsigned)P), dim3(bs), lds,
+    hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_PART, V>), dim3(grid, (unsigned)P), dim3(bs), lds,
                        ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites,
                        N, wt, site_cost, node_cost, total);
     PHY_LAUNCHED(ctx);
-    hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_TOP>), dim3(grid()), dim3(bs), lds, ctx->stream,
+    hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_TOP, V>), dim3(grid), dim3(bs), lds, ctx->stream,
                        sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites, N, wt,
                        site_cost, node_cost, total);
   } else {
-    hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_PART>), dim3(grid(), (unsigned)P), dim3(bs), lds,
+    hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_PART, V>), dim3(grid, (unsigned)P), dim3(bs), lds,
                        ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites,
                        N, wt, site_cost, node_cost, total);
     PHY_LAUNCHED(ctx);
-    hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_TOP>), dim3(grid()), dim3(bs), lds, ctx->stream,
+    hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_TOP, V>), dim3(grid), dim3(bs), lds, ctx->stream,
                        sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites, N, wt,
                        site_cost, node_cost, total);
   }
@@ -555,16 +607,19 @@ int downpass_launch(phy_ctx* ctx, const phy_topo* topo, uint32_t* s32, int64_t s
   return PHY_OK;
 }
 
-template <int BATCH>
+template <int BATCH, int V>
 int uppass_launch(phy_ctx* ctx, const phy_topo* topo, const uint32_t* s32, uint32_t* f32,
                   int64_t stride32, int64_t W32, int bs) {
   const int N = topo->n_leaves;
-  auto grid = [&]() { return (unsigned)((W32 + bs - 1) / bs); };
-  if (!use_split(topo)) {
-    while (bs > 64 && (size_t)(2 * BATCH + 2 * topo->up_depth) * bs * sizeof(uint32_t) > 64 * 1024) bs /= 2;
-    const size_t lds = (size_t)(2 * BATCH + 2 * topo->up_depth) * bs * sizeof(uint32_t);
-    if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "uppass: LDS stack too deep for this tree");
-    hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_WHOLE>), dim3(grid()), dim3(bs), lds, ctx->stream,
+  const bool split = use_split(topo);
+  const int depth = split ? topo->split_up.depth : topo->up_depth;
+  auto lds_of = [&](int b) { return (size_t)(2 * BATCH + 2 * depth) * b * 4 * V; };
+  while (bs > 64 && lds_of(bs) > 64 * 1024) bs /= 2;
+  const size_t lds = lds_of(bs);
+  if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "uppass: LDS stack too deep for this tree");
+  const unsigned grid = (unsigned)((W32 / V + bs - 1) / bs);
+  if (!split) {
+    hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_WHOLE, V>), dim3(grid), dim3(bs), lds, ctx->stream,
                        topo->d_up_prog, N - 1, topo->d_up_fifo, nullptr, 0, nullptr, s32, f32, stride32,
                        W32, N);
     PHY_LAUNCHED(ctx);
@@ -572,17 +627,20 @@ int uppass_launch(phy_ctx* ctx, const phy_topo* topo, const uint32_t* s32, uint3
   }
   const SplitProg& sp = topo->split_up;
   const int P = (int)topo->parts.size();
-  while (bs > 64 && (size_t)(2 * BATCH + 2 * sp.depth) * bs * sizeof(uint32_t) > 64 * 1024) bs /= 2;
-  const size_t lds = (size_t)(2 * BATCH + 2 * sp.depth) * bs * sizeof(uint32_t);
-  if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "uppass: LDS stack too deep for this tree");
-  hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_TOP>), dim3(grid()), dim3(bs), lds, ctx->stream, sp.d_prog,
+  hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_TOP, V>), dim3(grid), dim3(bs), lds, ctx->stream, sp.d_prog,
                      0, sp.d_fifo, sp.d_meta, P, topo->d_part_parent, s32, f32, stride32, W32, N);
   PHY_LAUNCHED(ctx);
-  hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_PART>), dim3(grid(), (unsigned)P), dim3(bs), lds,
+  hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_PART, V>), dim3(grid, (unsigned)P), dim3(bs), lds,
                      ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, topo->d_part_parent, s32, f32,
                      stride32, W32, N);
   PHY_LAUNCHED(ctx);
   return PHY_OK;
+}
+
+// 32-bit words per thread: 1 or 2 (PHY_FITCH_WORDS, default 1)
+int words_env() {
+  const char* s = getenv("PHY_FITCH_WORDS");
+  return (s && atoi(s) == 2) ? 2 : 1;
 }
 
 // operand prefetch depth (leaves / node sets per batch), default 8
@@ -614,11 +672,14 @@ int phy_fitch_downpass(phy_ctx* ctx, const phy_topo* topo, uint64_t* sets, int64
   uint32_t* s32 = reinterpret_cast<uint32_t*>(sets);
   auto* nc = reinterpret_cast<unsigned long long*>(node_cost);
   auto* tt = reinterpret_cast<unsigned long long*>(total);
+  const int v = words_env();
+#define PHY_DOWN(B, VV) downpass_launch<B, VV>(ctx, topo, s32, 2 * stride, W32, n_sites, wt, site_cost, nc, tt, bs)
   switch (batch_env()) {
-    case 8: return downpass_launch<8>(ctx, topo, s32, 2 * stride, W32, n_sites, wt, site_cost, nc, tt, bs);
-    case 32: return downpass_launch<32>(ctx, topo, s32, 2 * stride, W32, n_sites, wt, site_cost, nc, tt, bs);
-    default: return downpass_launch<16>(ctx, topo, s32, 2 * stride, W32, n_sites, wt, site_cost, nc, tt, bs);
+    case 8: return v == 2 ? PHY_DOWN(8, 2) : PHY_DOWN(8, 1);
+    case 32: return v == 2 ? PHY_DOWN(32, 2) : PHY_DOWN(32, 1);
+    default: return v == 2 ? PHY_DOWN(16, 2) : PHY_DOWN(16, 1);
   }
+#undef PHY_DOWN
 }
 
 int phy_fitch_uppass(phy_ctx* ctx, const phy_topo* topo, const uint64_t* sets, uint64_t* fin,
@@ -633,11 +694,14 @@ int phy_fitch_uppass(phy_ctx* ctx, const phy_topo* topo, const uint64_t* sets, u
   const int64_t W32 = 2 * W;
   const uint32_t* s32 = reinterpret_cast<const uint32_t*>(sets);
   uint32_t* f32 = reinterpret_cast<uint32_t*>(fin);
+  const int v = words_env();
+#define PHY_UP(B, VV) uppass_launch<B, VV>(ctx, topo, s32, f32, 2 * stride, W32, bs)
   switch (batch_env()) {
-    case 8: return uppass_launch<8>(ctx, topo, s32, f32, 2 * stride, W32, bs);
-    case 32: return uppass_launch<32>(ctx, topo, s32, f32, 2 * stride, W32, bs);
-    default: return uppass_launch<16>(ctx, topo, s32, f32, 2 * stride, W32, bs);
+    case 8: return v == 2 ? PHY_UP(8, 2) : PHY_UP(8, 1);
+    case 32: return v == 2 ? PHY_UP(32, 2) : PHY_UP(32, 1);
+    default: return v == 2 ? PHY_UP(16, 2) : PHY_UP(16, 1);
   }
+#undef PHY_UP
 }
 
 int phy_fitch_insertion_costs(phy_ctx* ctx, const uint64_t* pre, const uint64_t* fin,

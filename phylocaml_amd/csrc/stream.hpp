@@ -74,18 +74,18 @@ __device__ __forceinline__ void for_each_entry_ahead(const int4* __restrict__ pr
   }
 }
 
-// Loader: uint32_t operator()(int id) const, an in-bounds load for every id
-// of the padded stream.
-template <int B, class Loader>
+// Loader: T operator()(int id) const, an in-bounds load for every id of the
+// padded stream.
+template <int B, class Loader, class T = uint32_t>
 struct OperandRing {
   static_assert(B % 4 == 0 && 3 * B <= (int)kFifoSlack, "ring batch vs id padding");
-  uint32_t pre[B];
-  uint32_t* ring;  // [2B][bd]
+  T pre[B];
+  T* ring;  // [2B][bd]
   const int32_t* ids;
   Loader load;
   int bd, tid;
   int c, filled;
-  uint32_t la0, la1;
+  T la0, la1;
 
   __device__ __forceinline__ void issue(int start) {
     const int4* iv = reinterpret_cast<const int4*>(ids + start);
@@ -103,9 +103,9 @@ struct OperandRing {
 #pragma unroll
     for (int j = 0; j < B; ++j) ring[(h * B + j) * bd + tid] = pre[j];
   }
-  __device__ __forceinline__ uint32_t slot(int k) const { return ring[(k % (2 * B)) * bd + tid]; }
+  __device__ __forceinline__ T slot(int k) const { return ring[(k % (2 * B)) * bd + tid]; }
 
-  __device__ __forceinline__ void init(uint32_t* lds, const int32_t* id_stream, Loader ld,
+  __device__ __forceinline__ void init(T* lds, const int32_t* id_stream, Loader ld,
                                        int block_dim, int thread) {
     ring = lds;
     ids = id_stream;
@@ -122,8 +122,8 @@ struct OperandRing {
     la0 = slot(0);
     la1 = slot(1);
   }
-  __device__ __forceinline__ uint32_t get() {
-    const uint32_t v = la0;
+  __device__ __forceinline__ T get() {
+    const T v = la0;
     la0 = la1;
     ++c;
     if (c == filled - B) {  // the older half is consumed: refill it

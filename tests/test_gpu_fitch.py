@@ -97,15 +97,17 @@ def test_tree_passes_match_oracle(ctx, rng, N, S):
         _tree_case(ctx, N, ch, root, leaves, w)
 
 
+@pytest.mark.parametrize("words", ["1", "2"])
 @pytest.mark.parametrize("batch", ["8", "16", "32"])
 @pytest.mark.parametrize("split", ["0", "1"])
-def test_prefetch_depth_and_split_forms(ctx, rng, monkeypatch, batch, split):
+def test_prefetch_depth_and_split_forms(ctx, rng, monkeypatch, batch, split, words):
     # the whole-tree and the split (parts side by side, then the top) forms,
-    # at every operand prefetch depth, give the oracle's bits
+    # at every operand prefetch depth and lane width, give the oracle's bits
     monkeypatch.setenv("PHY_FITCH_BATCH", batch)
     monkeypatch.setenv("PHY_FITCH_SPLIT", split)
+    monkeypatch.setenv("PHY_FITCH_WORDS", words)
     N, S = 300, 3001
-    _say(f"fitch forms batch={batch} split={split}")
+    _say(f"fitch forms batch={batch} split={split} words={words}")
     ch, root = random_tree(rng, N)
     leaves = rng.integers(1, 16, (N, S)).astype(np.uint8)
     w = rng.integers(1, 5, S).astype(np.uint32)
