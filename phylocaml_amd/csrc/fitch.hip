@@ -8,15 +8,16 @@
 //   (the masked add cannot carry across nibbles), a full-nibble mask from it
 //   by  m8 | (m8 - (m8 >> 3)),  and the median by one bit-field insert.
 //
-// Whole-tree passes: one thread owns one word (16 sites) of every node and
-// walks the uniform program compiled by phy_topo_create.  Intermediate sets
-// live in an accumulator register and a per-thread LDS stack, so the
+// Whole-tree passes: one thread owns V 32-bit words (8V sites) of every node
+// and walks the uniform program compiled by phy_topo_create.  Intermediate
+// sets live in an accumulator register and a per-thread LDS stack, so the
 // algorithmic HBM traffic is the minimum: leaves read once and every
 // preliminary set written once (down), every set read once and every final
 // set written once (up).  The operand stream (leaves for the down-pass, node
-// sets for the up-pass) is consumed in a fixed order and is prefetched BATCH
-// (32) loads ahead through registers into an LDS ring, which keeps enough bytes in
-// flight to cover HBM latency with one wave per SIMD.
+// sets for the up-pass) is consumed in a fixed order and is prefetched a
+// batch ahead through registers into an LDS ring.  Trees of >= 128 leaves
+// run split: parts under cut nodes side by side (grid.y = part), plus a top
+// program, which multiplies the waves in flight by the number of parts.
 #include "common.hpp"
 #include "stream.hpp"
 

@@ -631,132 +631,132 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
   real accM[4] = {(real)0, (real)0, (real)0, (real)0};
   int sp = 0;
   for_each_entry_ahead(prog, n_ops, [&](int i, const int4 op, const int4 on) {
-      real Dn[8], En[8];
-      int Xn[2] = {0, 0};
-      const bool more = i + 1 < n_ops;
-      if (more) {
-        fetch(on.y, Dn, &Xn[0], En);
-        fetch(on.z, Dn + 4, &Xn[1], En + 4);
+    real Dn[8], En[8];
+    int Xn[2] = {0, 0};
+    const bool more = i + 1 < n_ops;
+    if (more) {
+      fetch(on.y, Dn, &Xn[0], En);
+      fetch(on.z, Dn + 4, &Xn[1], En + 4);
+    }
+    const int fl = op.w, src = fl & 3;
+    const int a = op.y, b = op.z;
+    real Mv[4];
+    if (src == SRC_LEAF) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) Mv[k] = (real)pi[k];
+    } else if (src == SRC_ACC) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) Mv[k] = accM[k];
+    } else {
+      --sp;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) Mv[k] = stk[((size_t)sp * 4 + k) * bd + tid];
+    }
+    real* Da = Dc;
+    real* Db = Dc + 4;
+    const real* ea = Ec;
+    const real* eb = Ec + 4;
+    align_D(Da, Xc[0]);
+    align_D(Db, Xc[1]);
+    real ua[4], ub[4], wa[4], wb[4], ya[4], yb[4], Aa[4], Ab[4], ga[4], gb[4];
+    matvec_u(Ui, Da, ua);
+    matvec_u(Ui, Db, ub);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      wa[k] = ea[k] * ua[k];
+      wb[k] = eb[k] * ub[k];
+    }
+    matvec_u(U, wa, ya);
+    matvec_u(U, wb, yb);
+    real s0 = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      Aa[k] = Mv[k] * yb[k];
+      Ab[k] = Mv[k] * ya[k];
+      s0 = fma(Aa[k], ya[k], s0);
+    }
+    vecmat_u(U, Aa, ga);
+    vecmat_u(U, Ab, gb);
+    real s1a = 0, s2a = 0, s1b = 0, s2b = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const real qa = ga[k] * wa[k], qb = gb[k] * wb[k];
+      s1a = fma(qa, l1[k], s1a);
+      s2a = fma(qa, l2[k], s2a);
+      s1b = fma(qb, l1[k], s1b);
+      s2b = fma(qb, l2[k], s2b);
+    }
+    const double L0 = rate_sum<R>(pr * (double)s0);
+    const double L1a = rate_sum<R>(pr * (double)s1a), L2a = rate_sum<R>(pr * (double)s2a);
+    const double L1b = rate_sum<R>(pr * (double)s1b), L2b = rate_sum<R>(pr * (double)s2b);
+    // The four per-pattern quantities q = (g_a, h_a, g_b, h_b) are spread
+    // over the R lanes of the pattern (lane r holds q = r * QPL + j), summed
+    // over the lanes of each 16-lane row that hold the same q (DPP row
+    // rotations), and each row's sums are written to their own slot.
+    {
+      const double inv = 1.0 / L0;
+      const double ga1 = L1a * inv, gb1 = L1b * inv;
+      double qv[QPL];
+#pragma unroll
+      for (int j = 0; j < QPL; ++j) {
+        const int q = r * QPL + j;
+        double v = 0.0;
+        if (act && q < 4) {
+          const double g = (q < 2) ? ga1 : gb1;
+          const double l2 = (q < 2) ? L2a : L2b;
+          v = wp * ((q & 1) ? (l2 * inv - g * g) : g);
+        }
+        qv[j] = row_sum_stride<R>(v);
       }
-      const int fl = op.w, src = fl & 3;
-      const int a = op.y, b = op.z;
-      real Mv[4];
-      if (src == SRC_LEAF) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) Mv[k] = (real)pi[k];
-      } else if (src == SRC_ACC) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) Mv[k] = accM[k];
-      } else {
-        --sp;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) Mv[k] = stk[((size_t)sp * 4 + k) * bd + tid];
-      }
-      real* Da = Dc;
-      real* Db = Dc + 4;
-      const real* ea = Ec;
-      const real* eb = Ec + 4;
-      align_D(Da, Xc[0]);
-      align_D(Db, Xc[1]);
-      real ua[4], ub[4], wa[4], wb[4], ya[4], yb[4], Aa[4], Ab[4], ga[4], gb[4];
-      matvec_u(Ui, Da, ua);
-      matvec_u(Ui, Db, ub);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        wa[k] = ea[k] * ua[k];
-        wb[k] = eb[k] * ub[k];
-      }
-      matvec_u(U, wa, ya);
-      matvec_u(U, wb, yb);
-      real s0 = 0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        Aa[k] = Mv[k] * yb[k];
-        Ab[k] = Mv[k] * ya[k];
-        s0 = fma(Aa[k], ya[k], s0);
-      }
-      vecmat_u(U, Aa, ga);
-      vecmat_u(U, Ab, gb);
-      real s1a = 0, s2a = 0, s1b = 0, s2b = 0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const real qa = ga[k] * wa[k], qb = gb[k] * wb[k];
-        s1a = fma(qa, l1[k], s1a);
-        s2a = fma(qa, l2[k], s2a);
-        s1b = fma(qb, l1[k], s1b);
-        s2b = fma(qb, l2[k], s2b);
-      }
-      const double L0 = rate_sum<R>(pr * (double)s0);
-      const double L1a = rate_sum<R>(pr * (double)s1a), L2a = rate_sum<R>(pr * (double)s2a);
-      const double L1b = rate_sum<R>(pr * (double)s1b), L2b = rate_sum<R>(pr * (double)s2b);
-      // The four per-pattern quantities q = (g_a, h_a, g_b, h_b) are spread
-      // over the R lanes of the pattern (lane r holds q = r * QPL + j), summed
-      // over the lanes of each 16-lane row that hold the same q (DPP row
-      // rotations), and each row's sums are written to their own slot.
-      {
-        const double inv = 1.0 / L0;
-        const double ga1 = L1a * inv, gb1 = L1b * inv;
-        double qv[QPL];
+      if ((lane & 15) < R) {
+        const int64_t slot = wave * 4 + (lane >> 4);
 #pragma unroll
         for (int j = 0; j < QPL; ++j) {
           const int q = r * QPL + j;
-          double v = 0.0;
-          if (act && q < 4) {
-            const double g = (q < 2) ? ga1 : gb1;
-            const double l2 = (q < 2) ? L2a : L2b;
-            v = wp * ((q & 1) ? (l2 * inv - g * g) : g);
-          }
-          qv[j] = row_sum_stride<R>(v);
-        }
-        if ((lane & 15) < R) {
-          const int64_t slot = wave * 4 + (lane >> 4);
-#pragma unroll
-          for (int j = 0; j < QPL; ++j) {
-            const int q = r * QPL + j;
-            if (q < 4) {
-              const int edge = (q < 2) ? a : b;
-              wsum[((size_t)edge * 2 + (q & 1)) * n_slots + slot] = qv[j];
-            }
+          if (q < 4) {
+            const int edge = (q < 2) ? a : b;
+            wsum[((size_t)edge * 2 + (q & 1)) * n_slots + slot] = qv[j];
           }
         }
       }
-      // messages to internal children: M_c = Ui^T (e_c (*) g_c)
-      real Ma[4], Mb[4];
-      int dummy = 0;
-      if (!(fl & UP_A_LEAF)) {
-        real t[4];
+    }
+    // messages to internal children: M_c = Ui^T (e_c (*) g_c)
+    real Ma[4], Mb[4];
+    int dummy = 0;
+    if (!(fl & UP_A_LEAF)) {
+      real t[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) t[k] = ea[k] * ga[k];
-        vecmat_u(Ui, t, Ma);
-        rescale_pattern<real, R>(Ma, &dummy);
-      }
-      if (!(fl & UP_B_LEAF)) {
-        real t[4];
+      for (int k = 0; k < 4; ++k) t[k] = ea[k] * ga[k];
+      vecmat_u(Ui, t, Ma);
+      rescale_pattern<real, R>(Ma, &dummy);
+    }
+    if (!(fl & UP_B_LEAF)) {
+      real t[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) t[k] = eb[k] * gb[k];
-        vecmat_u(Ui, t, Mb);
-        rescale_pattern<real, R>(Mb, &dummy);
-      }
-      if (fl & UP_PUSH) {
-        const real* o = (fl & UP_CONT_B) ? Ma : Mb;
+      for (int k = 0; k < 4; ++k) t[k] = eb[k] * gb[k];
+      vecmat_u(Ui, t, Mb);
+      rescale_pattern<real, R>(Mb, &dummy);
+    }
+    if (fl & UP_PUSH) {
+      const real* o = (fl & UP_CONT_B) ? Ma : Mb;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) stk[((size_t)sp * 4 + k) * bd + tid] = o[k];
-        ++sp;
-      }
-      if (!(fl & UP_A_LEAF) || !(fl & UP_B_LEAF)) {
-        const real* c = (fl & UP_CONT_B) ? Mb : Ma;
+      for (int k = 0; k < 4; ++k) stk[((size_t)sp * 4 + k) * bd + tid] = o[k];
+      ++sp;
+    }
+    if (!(fl & UP_A_LEAF) || !(fl & UP_B_LEAF)) {
+      const real* c = (fl & UP_CONT_B) ? Mb : Ma;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) accM[k] = c[k];
-      }
-      if (more) {
+      for (int k = 0; k < 4; ++k) accM[k] = c[k];
+    }
+    if (more) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          Dc[k] = Dn[k];
-          Ec[k] = En[k];
-        }
-        Xc[0] = Xn[0];
-        Xc[1] = Xn[1];
+      for (int k = 0; k < 8; ++k) {
+        Dc[k] = Dn[k];
+        Ec[k] = En[k];
       }
+      Xc[0] = Xn[0];
+      Xc[1] = Xn[1];
+    }
   });
 }
 
