@@ -275,11 +275,12 @@ __device__ __forceinline__ void rescale4(real* v, int* e) {
 // State of one wave of the post-order pass: accumulator, stack, tip stream.
 template <typename real, int R, bool FUSED>
 struct DownState {
+  static constexpr int BD = 64 * R;  // block: one wave per rate class
   real acc[4] = {(real)0, (real)0, (real)0, (real)0};
   int acc_e = 0, sp = 0;
-  real* stk;       // [depth][4][bd]
-  int32_t* stke;   // [depth][bd]
-  int bd, tid, r, N;
+  real* stk;       // [depth][4][BD]
+  int32_t* stke;   // [depth][BD]
+  int tid, r, N;
   int64_t pitch, pc, p;
   bool act;
   real* part;
@@ -289,8 +290,8 @@ struct DownState {
   __device__ __forceinline__ void spill_if(int fl) {
     if (FUSED && (fl & DOWN_SPILL)) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) stk[((size_t)sp * 4 + k) * bd + tid] = acc[k];
-      stke[(size_t)sp * bd + tid] = acc_e;
+      for (int k = 0; k < 4; ++k) stk[(sp * 4 + k) * BD + tid] = acc[k];
+      stke[sp * BD + tid] = acc_e;
       ++sp;
     }
   }
@@ -306,8 +307,8 @@ struct DownState {
     } else if (FUSED) {
       --sp;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) src[k] = stk[((size_t)sp * 4 + k) * bd + tid];
-      *e = stke[(size_t)sp * bd + tid];
+      for (int k = 0; k < 4; ++k) src[k] = stk[(sp * 4 + k) * BD + tid];
+      *e = stke[sp * BD + tid];
     } else {
       const real* sv = part + (((size_t)(a - N) * R + r) * 4) * pitch + pc;
 #pragma unroll
@@ -382,7 +383,6 @@ __device__ __forceinline__ void down_init(DownState<real, R, FUSED>& st, unsigne
                                           int64_t stride, int64_t n_pat, int64_t pitch, real* part,
                                           int32_t* ex, int N, int stack_depth) {
   st.tid = threadIdx.x;
-  st.bd = blockDim.x;
   st.r = __builtin_amdgcn_readfirstlane(st.tid >> 6);  // rate class of this wave
   st.p = (int64_t)blockIdx.x * 64 + (st.tid & 63);
   st.act = st.p < n_pat;
@@ -395,7 +395,7 @@ __device__ __forceinline__ void down_init(DownState<real, R, FUSED>& st, unsigne
   // when the stack is empty, the same bytes hold the rate exchange
   // [R][64] double S + [R][64] int e
   st.stk = reinterpret_cast<real*>(lds_raw);
-  st.stke = reinterpret_cast<int32_t*>(st.stk + (size_t)(FUSED ? stack_depth : 0) * 4 * st.bd);
+  st.stke = reinterpret_cast<int32_t*>(st.stk + (size_t)(FUSED ? stack_depth : 0) * 4 * st.BD);
   st.ts.init(tips, stride, st.pc, leaf_ids);
 }
 
@@ -415,14 +415,24 @@ __device__ __forceinline__ void prune_down_body(
     real PA[16], PB[16];
     load_mat<real, R>(Pm + ((size_t)op.y * R + r) * 16, PA);
     load_mat<real, R>(Pm + ((size_t)op.z * R + r) * 16, PB);
-    const int fl = op.w;
+    const int fl = op.w, sa = fl & 3, sb = (fl >> 2) & 3;
     st.spill_if(fl);
     real src[4], xa[4], xb[4];
     int ea, eb;
-    st.operand_a(fl & 3, op.y, src, &ea);
-    matvec4(PA, src, xa);
-    st.operand_b((fl >> 2) & 3, src, &eb);
-    matvec4(PB, src, xb);
+    if (sa == SRC_ACC) {  // the accumulator is read in place
+      matvec4(PA, st.acc, xa);
+      ea = st.acc_e;
+    } else {
+      st.operand_a(sa, op.y, src, &ea);
+      matvec4(PA, src, xa);
+    }
+    if (sb == SRC_ACC) {
+      matvec4(PB, st.acc, xb);
+      eb = st.acc_e;
+    } else {
+      st.operand_b(sb, src, &eb);
+      matvec4(PB, src, xb);
+    }
     st.combine(op.x, xa, xb, ea + eb);
   });
   down_root(st, lds_raw, pi, probs, weights, site_lnl, wave_sums);

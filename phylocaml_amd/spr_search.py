@@ -168,6 +168,8 @@ def search(c: Context, net, norm, guided: bool, k: int = 64, eps: float = 1e-6,
     while rounds < max_rounds:
         mv = spr.moves(inst.N, ch, r)
         rounds += 1
+        log(f"  {'guided' if guided else 'exhaustive'} round {rounds}: lnL {cur:.3f}, "
+            f"{c.n_exact} exact evaluations, {time.time() - t0:.1f}s")
         if guided:
             X = norm(c.features(ch, r, bl, mv))
             pred = net.forward(D.to_device(X)).cpu().numpy()
