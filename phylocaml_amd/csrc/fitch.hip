@@ -638,10 +638,12 @@ int uppass_launch(phy_ctx* ctx, const phy_topo* topo, const uint32_t* s32, uint3
   return PHY_OK;
 }
 
-// 32-bit words per thread: 1 or 2 (PHY_FITCH_WORDS, default 1)
+// 32-bit words per thread: 1, 2 or 4 (PHY_FITCH_WORDS, default 2: 8-byte
+// accesses)
 int words_env() {
   const char* s = getenv("PHY_FITCH_WORDS");
-  return (s && atoi(s) == 2) ? 2 : 1;
+  const int v = s ? atoi(s) : 2;
+  return (v == 1 || v == 4) ? v : 2;
 }
 
 // operand prefetch depth (leaves / node sets per batch), default 8
@@ -649,6 +651,14 @@ int batch_env() {
   const char* s = getenv("PHY_FITCH_BATCH");
   const int v = s ? atoi(s) : 8;
   return (v == 8 || v == 16 || v == 32) ? v : 8;
+}
+
+// the widest lane (<= want words) whose accesses stay aligned and whose
+// lanes tile the rows exactly
+int lane_words(int want, const void* base, int64_t stride32, int64_t W32) {
+  for (int v = want; v > 1; v /= 2)
+    if (stride32 % v == 0 && W32 % v == 0 && reinterpret_cast<uintptr_t>(base) % (4 * v) == 0) return v;
+  return 1;
 }
 
 }  // namespace
@@ -673,12 +683,12 @@ int phy_fitch_downpass(phy_ctx* ctx, const phy_topo* topo, uint64_t* sets, int64
   uint32_t* s32 = reinterpret_cast<uint32_t*>(sets);
   auto* nc = reinterpret_cast<unsigned long long*>(node_cost);
   auto* tt = reinterpret_cast<unsigned long long*>(total);
-  const int v = words_env();
+  const int v = lane_words(words_env(), s32, 2 * stride, W32);
 #define PHY_DOWN(B, VV) downpass_launch<B, VV>(ctx, topo, s32, 2 * stride, W32, n_sites, wt, site_cost, nc, tt, bs)
   switch (batch_env()) {
-    case 8: return v == 2 ? PHY_DOWN(8, 2) : PHY_DOWN(8, 1);
-    case 32: return v == 2 ? PHY_DOWN(32, 2) : PHY_DOWN(32, 1);
-    default: return v == 2 ? PHY_DOWN(16, 2) : PHY_DOWN(16, 1);
+    case 8: return v == 4 ? PHY_DOWN(8, 4) : v == 2 ? PHY_DOWN(8, 2) : PHY_DOWN(8, 1);
+    case 32: return v == 4 ? PHY_DOWN(32, 4) : v == 2 ? PHY_DOWN(32, 2) : PHY_DOWN(32, 1);
+    default: return v == 4 ? PHY_DOWN(16, 4) : v == 2 ? PHY_DOWN(16, 2) : PHY_DOWN(16, 1);
   }
 #undef PHY_DOWN
 }
@@ -695,12 +705,13 @@ int phy_fitch_uppass(phy_ctx* ctx, const phy_topo* topo, const uint64_t* sets, u
   const int64_t W32 = 2 * W;
   const uint32_t* s32 = reinterpret_cast<const uint32_t*>(sets);
   uint32_t* f32 = reinterpret_cast<uint32_t*>(fin);
-  const int v = words_env();
+  const int v = std::min(lane_words(words_env(), s32, 2 * stride, W32),
+                         lane_words(words_env(), f32, 2 * stride, W32));
 #define PHY_UP(B, VV) uppass_launch<B, VV>(ctx, topo, s32, f32, 2 * stride, W32, bs)
   switch (batch_env()) {
-    case 8: return v == 2 ? PHY_UP(8, 2) : PHY_UP(8, 1);
-    case 32: return v == 2 ? PHY_UP(32, 2) : PHY_UP(32, 1);
-    default: return v == 2 ? PHY_UP(16, 2) : PHY_UP(16, 1);
+    case 8: return v == 4 ? PHY_UP(8, 4) : v == 2 ? PHY_UP(8, 2) : PHY_UP(8, 1);
+    case 32: return v == 4 ? PHY_UP(32, 4) : v == 2 ? PHY_UP(32, 2) : PHY_UP(32, 1);
+    default: return v == 4 ? PHY_UP(16, 4) : v == 2 ? PHY_UP(16, 2) : PHY_UP(16, 1);
   }
 #undef PHY_UP
 }
