@@ -208,11 +208,38 @@ struct alignas(4 * V) Words {
   uint32_t w[V];
 };
 
-template <int V>
+// Row accesses, optionally non-temporal (streamed past the caches: every
+// row is read once and written once per pass)
+template <int V> struct RawVec;
+template <> struct RawVec<1> { typedef uint32_t T; };
+template <> struct RawVec<2> { typedef uint32_t T __attribute__((ext_vector_type(2))); };
+template <> struct RawVec<4> { typedef uint32_t T __attribute__((ext_vector_type(4))); };
+
+template <int V, bool NT>
+__device__ __forceinline__ Words<V> ld_row(const Words<V>* p) {
+  if constexpr (NT) {
+    const typename RawVec<V>::T r = __builtin_nontemporal_load(reinterpret_cast<const typename RawVec<V>::T*>(p));
+    return __builtin_bit_cast(Words<V>, r);
+  } else {
+    return *p;
+  }
+}
+template <int V, bool NT>
+__device__ __forceinline__ void st_row(Words<V>* p, const Words<V>& v) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(__builtin_bit_cast(typename RawVec<V>::T, v),
+                                reinterpret_cast<typename RawVec<V>::T*>(p));
+  else
+    *p = v;
+}
+
+template <int V, bool NT>
 struct SetLoad {
   const Words<V>* sets;
   int64_t stride, wc;  // in units of V words
-  __device__ __forceinline__ Words<V> operator()(int id) const { return sets[(int64_t)id * stride + wc]; }
+  __device__ __forceinline__ Words<V> operator()(int id) const {
+    return ld_row<V, NT>(sets + (int64_t)id * stride + wc);
+  }
 };
 
 // Pass modes: the whole tree in one program, or a tree split at cut nodes:
@@ -230,7 +257,7 @@ enum : int { MODE_WHOLE = 0, MODE_PART = 1, MODE_TOP = 2 };
   } while (0)
 
 // W = number of 32-bit words per node row; lane w owns words V*w .. V*w+V-1
-template <int BATCH, bool NODE_COST, int MODE, int V>
+template <int BATCH, bool NODE_COST, int MODE, int V, bool NT>
 __global__ __launch_bounds__(256) void k_fitch_down(
     const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ fifo_ids,
     const int32_t* __restrict__ meta, int n_parts, uint4* __restrict__ scratch,
@@ -250,8 +277,8 @@ __global__ __launch_bounds__(256) void k_fitch_down(
   const int64_t strideT = stride / V;
   T* ring = reinterpret_cast<T*>(lds_dyn);
   T* stk = ring + (size_t)2 * BATCH * bd;
-  OperandRing<BATCH, SetLoad<V>, T> leaves;
-  leaves.init(ring, fifo_ids, SetLoad<V>{sets, strideT, wc}, bd, tid);
+  OperandRing<BATCH, SetLoad<V, NT>, T> leaves;
+  leaves.init(ring, fifo_ids, SetLoad<V, NT>{sets, strideT, wc}, bd, tid);
 
   uint32_t cnt[V];  // nibble counters, flushed every 15 nodes
   uint32_t site[8 * V];
@@ -287,7 +314,7 @@ __global__ __launch_bounds__(256) void k_fitch_down(
       acc.w[i] = r;
       cnt[i] += e[i];
     }
-    if (act) sets[(int64_t)op.x * strideT + w] = acc;
+    if (act) st_row<V, NT>(sets + (int64_t)op.x * strideT + w, acc);
     if (NODE_COST) {
       uint64_t c = 0;
       if (act) {
@@ -371,7 +398,7 @@ __global__ __launch_bounds__(256) void k_fitch_down(
 // ---------------------------------------------------------------------------
 // Whole-tree up-pass
 // ---------------------------------------------------------------------------
-template <int BATCH, int MODE, int V>
+template <int BATCH, int MODE, int V, bool NT>
 __global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog, int n_ops,
                                                   const int32_t* __restrict__ fifo_ids,
                                                   const int32_t* __restrict__ meta, int n_parts,
@@ -394,8 +421,8 @@ __global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog,
   const int64_t strideT = stride / V;
   T* ring = reinterpret_cast<T*>(lds_dyn);
   T* stk = ring + (size_t)2 * BATCH * bd;  // pairs (P, F)
-  OperandRing<BATCH, SetLoad<V>, T> nodes;
-  nodes.init(ring, fifo_ids, SetLoad<V>{sets, strideT, wc}, bd, tid);
+  OperandRing<BATCH, SetLoad<V, NT>, T> nodes;
+  nodes.init(ring, fifo_ids, SetLoad<V, NT>{sets, strideT, wc}, bd, tid);
 
   T accP = {}, accF = {};
   int sp = 0;
@@ -423,9 +450,9 @@ __global__ __launch_bounds__(256) void k_fitch_up(const int4* __restrict__ prog,
       la.w[i] = leaf32(fv.w[i], pa.w[i]);
       lb.w[i] = leaf32(fv.w[i], pb.w[i]);
     }
-    if (act) fin[(int64_t)op.x * strideT + w] = fv;
-    if (fl & UP_A_LEAF) { if (act) fin[(int64_t)op.y * strideT + w] = la; }
-    if (fl & UP_B_LEAF) { if (act) fin[(int64_t)op.z * strideT + w] = lb; }
+    if (act) st_row<V, NT>(fin + (int64_t)op.x * strideT + w, fv);
+    if (fl & UP_A_LEAF) { if (act) st_row<V, NT>(fin + (int64_t)op.y * strideT + w, la); }
+    if (fl & UP_B_LEAF) { if (act) st_row<V, NT>(fin + (int64_t)op.z * strideT + w, lb); }
     if (fl & UP_PUSH) {
       stk[(2 * sp) * bd + tid] = (fl & UP_CONT_B) ? pa : pb;
       stk[(2 * sp + 1) * bd + tid] = fv;
@@ -547,7 +574,7 @@ bool use_split(const phy_topo* t) {
   return t->parts.size() >= 2 && t->split_down.d_prog && !(e && e[0] == '0');
 }
 
-template <int BATCH, int V>
+template <int BATCH, int V, bool NT>
 int downpass_launch(phy_ctx* ctx, const phy_topo* topo, uint32_t* s32, int64_t stride32,
                     int64_t W32, int64_t n_sites, const uint32_t* wt, uint32_t* site_cost,
                     unsigned long long* node_cost, unsigned long long* total, int bs) {
@@ -563,11 +590,11 @@ int downpass_launch(phy_ctx* ctx, const phy_topo* topo, uint32_t* s32, int64_t s
   const unsigned grid = (unsigned)((W32 / V + bs - 1) / bs);
   if (!split) {
     if (node_cost)
-      hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_WHOLE, V>), dim3(grid), dim3(bs), lds, ctx->stream,
+      hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_WHOLE, V, NT>), dim3(grid), dim3(bs), lds, ctx->stream,
                          topo->d_down_prog, N - 1, topo->d_down_fifo, nullptr, 0, nullptr, s32, stride32, W32,
                          n_sites, N, wt, site_cost, node_cost, total);
     else
-      hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_WHOLE, V>), dim3(grid), dim3(bs), lds, ctx->stream,
+      hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_WHOLE, V, NT>), dim3(grid), dim3(bs), lds, ctx->stream,
                          topo->d_down_prog, N - 1, topo->d_down_fifo, nullptr, 0, nullptr, s32, stride32, W32,
                          n_sites, N, wt, site_cost, node_cost, total);
     PHY_LAUNCHED(ctx);
@@ -588,19 +615,19 @@ int downpass_launch(phy_ctx* ctx, const phy_topo* topo, uint32_t* s32, int64_t s
   }
   uint4* scr = reinterpret_cast<uint4*>(topo->d_scratch);
   if (node_cost) {
-    hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_PART, V>), dim3(grid, (unsigned)P), dim3(bs), lds,
+    hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_PART, V, NT>), dim3(grid, (unsigned)P), dim3(bs), lds,
                        ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites,
                        N, wt, site_cost, node_cost, total);
     PHY_LAUNCHED(ctx);
-    hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_TOP, V>), dim3(grid), dim3(bs), lds, ctx->stream,
+    hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_TOP, V, NT>), dim3(grid), dim3(bs), lds, ctx->stream,
                        sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites, N, wt,
                        site_cost, node_cost, total);
   } else {
-    hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_PART, V>), dim3(grid, (unsigned)P), dim3(bs), lds,
+    hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_PART, V, NT>), dim3(grid, (unsigned)P), dim3(bs), lds,
                        ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites,
                        N, wt, site_cost, node_cost, total);
     PHY_LAUNCHED(ctx);
-    hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_TOP, V>), dim3(grid), dim3(bs), lds, ctx->stream,
+    hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_TOP, V, NT>), dim3(grid), dim3(bs), lds, ctx->stream,
                        sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites, N, wt,
                        site_cost, node_cost, total);
   }
@@ -608,7 +635,7 @@ int downpass_launch(phy_ctx* ctx, const phy_topo* topo, uint32_t* s32, int64_t s
   return PHY_OK;
 }
 
-template <int BATCH, int V>
+template <int BATCH, int V, bool NT>
 int uppass_launch(phy_ctx* ctx, const phy_topo* topo, const uint32_t* s32, uint32_t* f32,
                   int64_t stride32, int64_t W32, int bs) {
   const int N = topo->n_leaves;
@@ -620,7 +647,7 @@ int uppass_launch(phy_ctx* ctx, const phy_topo* topo, const uint32_t* s32, uint3
   if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "uppass: LDS stack too deep for this tree");
   const unsigned grid = (unsigned)((W32 / V + bs - 1) / bs);
   if (!split) {
-    hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_WHOLE, V>), dim3(grid), dim3(bs), lds, ctx->stream,
+    hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_WHOLE, V, NT>), dim3(grid), dim3(bs), lds, ctx->stream,
                        topo->d_up_prog, N - 1, topo->d_up_fifo, nullptr, 0, nullptr, s32, f32, stride32,
                        W32, N);
     PHY_LAUNCHED(ctx);
@@ -628,10 +655,10 @@ int uppass_launch(phy_ctx* ctx, const phy_topo* topo, const uint32_t* s32, uint3
   }
   const SplitProg& sp = topo->split_up;
   const int P = (int)topo->parts.size();
-  hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_TOP, V>), dim3(grid), dim3(bs), lds, ctx->stream, sp.d_prog,
+  hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_TOP, V, NT>), dim3(grid), dim3(bs), lds, ctx->stream, sp.d_prog,
                      0, sp.d_fifo, sp.d_meta, P, topo->d_part_parent, s32, f32, stride32, W32, N);
   PHY_LAUNCHED(ctx);
-  hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_PART, V>), dim3(grid, (unsigned)P), dim3(bs), lds,
+  hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_PART, V, NT>), dim3(grid, (unsigned)P), dim3(bs), lds,
                      ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, topo->d_part_parent, s32, f32,
                      stride32, W32, N);
   PHY_LAUNCHED(ctx);
@@ -651,6 +678,12 @@ int batch_env() {
   const char* s = getenv("PHY_FITCH_BATCH");
   const int v = s ? atoi(s) : 8;
   return (v == 8 || v == 16 || v == 32) ? v : 8;
+}
+
+// non-temporal row loads and stores (PHY_FITCH_NT=1)
+bool nt_env() {
+  const char* s = getenv("PHY_FITCH_NT");
+  return s && s[0] == '1';
 }
 
 // the widest lane (<= want words) whose accesses stay aligned and whose
@@ -684,7 +717,8 @@ int phy_fitch_downpass(phy_ctx* ctx, const phy_topo* topo, uint64_t* sets, int64
   auto* nc = reinterpret_cast<unsigned long long*>(node_cost);
   auto* tt = reinterpret_cast<unsigned long long*>(total);
   const int v = lane_words(words_env(), s32, 2 * stride, W32);
-#define PHY_DOWN(B, VV) downpass_launch<B, VV>(ctx, topo, s32, 2 * stride, W32, n_sites, wt, site_cost, nc, tt, bs)
+#define PHY_DOWN(B, VV) (nt_env() ? downpass_launch<B, VV, true>(ctx, topo, s32, 2 * stride, W32, n_sites, wt, site_cost, nc, tt, bs) \
+                              : downpass_launch<B, VV, false>(ctx, topo, s32, 2 * stride, W32, n_sites, wt, site_cost, nc, tt, bs))
   switch (batch_env()) {
     case 8: return v == 4 ? PHY_DOWN(8, 4) : v == 2 ? PHY_DOWN(8, 2) : PHY_DOWN(8, 1);
     case 32: return v == 4 ? PHY_DOWN(32, 4) : v == 2 ? PHY_DOWN(32, 2) : PHY_DOWN(32, 1);
@@ -707,7 +741,8 @@ int phy_fitch_uppass(phy_ctx* ctx, const phy_topo* topo, const uint64_t* sets, u
   uint32_t* f32 = reinterpret_cast<uint32_t*>(fin);
   const int v = std::min(lane_words(words_env(), s32, 2 * stride, W32),
                          lane_words(words_env(), f32, 2 * stride, W32));
-#define PHY_UP(B, VV) uppass_launch<B, VV>(ctx, topo, s32, f32, 2 * stride, W32, bs)
+#define PHY_UP(B, VV) (nt_env() ? uppass_launch<B, VV, true>(ctx, topo, s32, f32, 2 * stride, W32, bs) \
+                            : uppass_launch<B, VV, false>(ctx, topo, s32, f32, 2 * stride, W32, bs))
   switch (batch_env()) {
     case 8: return v == 4 ? PHY_UP(8, 4) : v == 2 ? PHY_UP(8, 2) : PHY_UP(8, 1);
     case 32: return v == 4 ? PHY_UP(32, 4) : v == 2 ? PHY_UP(32, 2) : PHY_UP(32, 1);
