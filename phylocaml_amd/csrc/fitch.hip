@@ -351,6 +351,7 @@ __global__ __launch_bounds__(256) void k_fitch_down(
     return;
   }
   if (MODE == MODE_TOP && act) {
+#pragma unroll 4
     for (int pq = 0; pq < n_parts; ++pq) {
       const uint4* src = scratch + ((size_t)pq * L + w) * 2 * V;
 #pragma unroll
