@@ -770,202 +770,6 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
   });
 }
 
-// The same pass with one lane per pattern: the R rate classes of a pattern
-// live in one lane, so the rate sums, the alignment of the stored partials'
-// per-rate exponents and the joint rescaling of the messages are register
-// work, and the per-branch sums reduce over the wave once per node for 64
-// patterns.  Stack entries hold R x 4 values per lane.
-template <typename real, int R>
-__global__ __launch_bounds__(256) void k_prune_deriv_lane(
-    const int4* __restrict__ prog, int n_ops, const uint64_t* __restrict__ tips, int64_t stride,
-    int64_t n_pat, int64_t pitch, const real* __restrict__ tab, const real* __restrict__ E,
-    const double* __restrict__ pi, const double* __restrict__ probs, const real* __restrict__ part,
-    const int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
-    double* __restrict__ wsum, int64_t n_slots, int N) {
-  extern __shared__ unsigned char lds_raw[];
-  real* stk = reinterpret_cast<real*>(lds_raw);  // [depth][R * 4][bd]
-  const int tid = threadIdx.x, bd = blockDim.x, lane = tid & 63;
-  const int64_t p = (int64_t)blockIdx.x * bd + tid;
-  const bool act = p < n_pat;
-  const int64_t pc = act ? p : 0;
-  const double wp = act ? (double)weights[p] : 0.0;
-  const int64_t wave = p >> 6;
-
-  real U[16], Ui[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    U[k] = tab[k];
-    Ui[k] = tab[16 + k];
-  }
-  // children operands for every rate class, brought to one exponent
-  auto load_child = [&](int c, real (*D)[4]) {
-    if (c < N) {
-      real x[4];
-      tipvec(tip_code(tips, stride, c, pc), x);
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) D[r][k] = x[k];
-    } else {
-      int e[R], m = -0x7fffffff;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const real* sv = part + (((size_t)(c - N) * R + r) * 4) * pitch + pc;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) D[r][k] = sv[(size_t)k * pitch];
-        e[r] = ex[((size_t)(c - N) * R + r) * pitch + pc];
-        m = max(m, e[r]);
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        if (e[r] != m) {
-#pragma unroll
-          for (int k = 0; k < 4; ++k) D[r][k] = ldexp(D[r][k], e[r] - m);
-        }
-    }
-  };
-  // joint power-of-two rescale of a message (its scale cancels in L'/L)
-  auto rescale_msg = [&](real (*Mm)[4]) {
-    real mx = (real)0;
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-#pragma unroll
-      for (int k = 0; k < 4; ++k) mx = fmax(mx, Mm[r][k]);
-    if (mx < Thr<real>::lo && mx > (real)0) {
-      int ee;
-      (void)frexp(mx, &ee);
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) Mm[r][k] = ldexp(Mm[r][k], -ee);
-    }
-  };
-
-  real accM[R][4];
-#pragma unroll
-  for (int r = 0; r < R; ++r)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) accM[r][k] = (real)0;
-  int sp = 0;
-  for_each_entry(prog, n_ops, [&](int, const int4 op) {
-    const int fl = op.w, src = fl & 3;
-    const int a = op.y, b = op.z;
-    real Mv[R][4];
-    if (src == SRC_LEAF) {
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) Mv[r][k] = (real)pi[k];
-    } else if (src == SRC_ACC) {
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) Mv[r][k] = accM[r][k];
-    } else {
-      --sp;
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) Mv[r][k] = stk[((size_t)sp * R * 4 + r * 4 + k) * bd + tid];
-    }
-    real Da[R][4], Db[R][4];
-    load_child(a, Da);
-    load_child(b, Db);
-    double L0 = 0, L1a = 0, L2a = 0, L1b = 0, L2b = 0;
-    real Ma[R][4], Mb[R][4];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const real* ea = E + ((size_t)a * R + r) * 4;
-      const real* eb = E + ((size_t)b * R + r) * 4;
-      const real* lr = tab + 32 + 4 * r;
-      real ua[4], ub[4], wa[4], wb[4], ya[4], yb[4], Aa[4], Ab[4], ga[4], gb[4];
-      matvec_u(Ui, Da[r], ua);
-      matvec_u(Ui, Db[r], ub);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        wa[k] = ea[k] * ua[k];
-        wb[k] = eb[k] * ub[k];
-      }
-      matvec_u(U, wa, ya);
-      matvec_u(U, wb, yb);
-      real s0 = 0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        Aa[k] = Mv[r][k] * yb[k];
-        Ab[k] = Mv[r][k] * ya[k];
-        s0 = fma(Aa[k], ya[k], s0);
-      }
-      vecmat_u(U, Aa, ga);
-      vecmat_u(U, Ab, gb);
-      real s1a = 0, s2a = 0, s1b = 0, s2b = 0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const real l1 = lr[k], l2 = l1 * l1;
-        const real qa = ga[k] * wa[k], qb = gb[k] * wb[k];
-        s1a = fma(qa, l1, s1a);
-        s2a = fma(qa, l2, s2a);
-        s1b = fma(qb, l1, s1b);
-        s2b = fma(qb, l2, s2b);
-      }
-      const double pr = probs[r];
-      L0 += pr * (double)s0;
-      L1a += pr * (double)s1a;
-      L2a += pr * (double)s2a;
-      L1b += pr * (double)s1b;
-      L2b += pr * (double)s2b;
-      // messages to internal children: M_c = Ui^T (e_c (*) g_c)
-      if (!(fl & UP_A_LEAF)) {
-        real t[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) t[k] = ea[k] * ga[k];
-        vecmat_u(Ui, t, Ma[r]);
-      }
-      if (!(fl & UP_B_LEAF)) {
-        real t[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) t[k] = eb[k] * gb[k];
-        vecmat_u(Ui, t, Mb[r]);
-      }
-    }
-    double dga = 0, dha = 0, dgb = 0, dhb = 0;
-    if (act) {
-      const double inv = 1.0 / L0;
-      const double g1 = L1a * inv, g2 = L1b * inv;
-      dga = wp * g1;
-      dha = wp * (L2a * inv - g1 * g1);
-      dgb = wp * g2;
-      dhb = wp * (L2b * inv - g2 * g2);
-    }
-    dga = wave_sum_d(dga);
-    dha = wave_sum_d(dha);
-    dgb = wave_sum_d(dgb);
-    dhb = wave_sum_d(dhb);
-    if (lane == 0) {
-      wsum[((size_t)a * 2) * n_slots + wave] = dga;
-      wsum[((size_t)a * 2 + 1) * n_slots + wave] = dha;
-      wsum[((size_t)b * 2) * n_slots + wave] = dgb;
-      wsum[((size_t)b * 2 + 1) * n_slots + wave] = dhb;
-    }
-    if (!(fl & UP_A_LEAF)) rescale_msg(Ma);
-    if (!(fl & UP_B_LEAF)) rescale_msg(Mb);
-    if (fl & UP_PUSH) {
-      auto& o = (fl & UP_CONT_B) ? Ma : Mb;
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) stk[((size_t)sp * R * 4 + r * 4 + k) * bd + tid] = o[r][k];
-      ++sp;
-    }
-    if (!(fl & UP_A_LEAF) || !(fl & UP_B_LEAF)) {
-      auto& c = (fl & UP_CONT_B) ? Mb : Ma;
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) accM[r][k] = c[r][k];
-    }
-  });
-}
-
 // Neumaier sum of n doubles with one block (deterministic order).
 __global__ void k_reduce_sum(const double* __restrict__ x, int64_t n, int64_t stride_x,
                              double* __restrict__ out) {
@@ -1150,7 +954,6 @@ struct Plan {
   int bs = 0, dbs = 0;  // block size of the post-order pass / derivative pass
   size_t esz = 8, lds = 0, dlds = 0;
   bool fused = false;
-  bool lane_form = false;  // derivative pass: one lane per pattern
 };
 
 constexpr size_t kLdsMax = 64 * 1024;
@@ -1162,15 +965,8 @@ size_t down_lds(int R, size_t esz, bool fused, int depth) {
   const size_t root = (size_t)R * 64 * 12;
   return std::max(root, fused ? (size_t)depth * bs * (4 * esz + 4) : 0);
 }
-size_t deriv_lds(int R, size_t esz, int bs, int depth, bool lane_form) {
-  return (size_t)std::max(depth, 1) * bs * 4 * esz * (lane_form ? R : 1);
-}
-
-// derivative pass form: one lane per pattern (PHY_PRUNE_DERIV_FORM=lane) or
-// one lane per (pattern, rate class) (quad, the default)
-bool deriv_lane_form() {
-  const char* e = getenv("PHY_PRUNE_DERIV_FORM");
-  return e && e[0] == 'l';
+size_t deriv_lds(int R, size_t esz, int bs, int depth) {
+  return (size_t)std::max(depth, 1) * bs * 4 * esz;
 }
 
 // largest block size <= want whose LDS fits (0 if none)
@@ -1194,18 +990,16 @@ Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t for
   pl.fused = fused && down_lds(R, esz, true, depth) <= kLdsMax;
   pl.lds = down_lds(R, esz, pl.fused, depth);
   if (derivs) {
-    pl.lane_form = deriv_lane_form();
     pl.dbs = fit_block(env_block("PHY_PRUNE_DERIV_BLOCK", 256),
-                       [&](int bs) { return deriv_lds(R, esz, bs, t->up_depth, pl.lane_form); });
-    pl.dlds = pl.dbs ? deriv_lds(R, esz, pl.dbs, t->up_depth, pl.lane_form) : 0;
+                       [&](int bs) { return deriv_lds(R, esz, bs, t->up_depth); });
+    pl.dlds = pl.dbs ? deriv_lds(R, esz, pl.dbs, t->up_depth) : 0;
   }
   pl.pitch = ((n_pat + 255) / 256) * 256;
   if (pl.pitch == 0) pl.pitch = 256;
   pl.n_blocks = pl.pitch / 64;
-  pl.dblocks = pl.dbs ? pl.pitch * (pl.lane_form ? 1 : R) / pl.dbs : 0;
+  pl.dblocks = pl.dbs ? pl.pitch * R / pl.dbs : 0;
   pl.n_waves = pl.pitch / 64;          // partial lnL sums: one per 64 patterns
-  // derivative sums: one per 16-lane row (quad form) or per wave (lane form)
-  pl.n_dwaves = pl.lane_form ? pl.pitch / 64 : pl.pitch * R / 64 * 4;
+  pl.n_dwaves = pl.pitch * R / 64 * 4;  // derivative sums: one per 16-lane row
   size_t o = 0;
   const size_t pm = (size_t)M * R * K * K * pl.esz;
   pl.off_P = o; o = align256(o + pm);
@@ -1265,14 +1059,9 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
     hipLaunchKernelGGL((k_eigen_exp<real>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0,
                        ctx->stream, cst, R, blen, M, E, tab);
     PHY_LAUNCHED(ctx);
-    if (pl.lane_form)
-      hipLaunchKernelGGL((k_prune_deriv_lane<real, R>), dim3((unsigned)pl.dblocks), dim3(pl.dbs), pl.dlds,
-                         ctx->stream, t->d_up_prog, N - 1, tips, stride, n_pat, pl.pitch, tab, E,
-                         d_pi, d_probs, part, ex, weights, wsum, pl.n_dwaves, N);
-    else
-      hipLaunchKernelGGL((k_prune_deriv<real, R>), dim3((unsigned)pl.dblocks), dim3(pl.dbs), pl.dlds,
-                         ctx->stream, t->d_up_prog, N - 1, tips, stride, n_pat, pl.pitch, tab, E,
-                         d_pi, d_probs, part, ex, weights, wsum, pl.n_dwaves, N);
+    hipLaunchKernelGGL((k_prune_deriv<real, R>), dim3((unsigned)pl.dblocks), dim3(pl.dbs), pl.dlds,
+                       ctx->stream, t->d_up_prog, N - 1, tips, stride, n_pat, pl.pitch, tab, E,
+                       d_pi, d_probs, part, ex, weights, wsum, pl.n_dwaves, N);
     PHY_LAUNCHED(ctx);
     const int ra = t->children[2 * (t->root - N)], rb = t->children[2 * (t->root - N) + 1];
     hipLaunchKernelGGL(k_reduce_derivs, dim3((unsigned)M), dim3(256), 0, ctx->stream, wsum,

@@ -111,32 +111,6 @@ def test_lnl_and_derivatives_match_oracle(ctx, rng, N, R):
         assert fu64["lnl"] == g["lnl"]
 
 
-@pytest.mark.parametrize("R", [1, 4, 8])
-def test_lane_form_derivatives_match_oracle(ctx, rng, monkeypatch, R):
-    # the derivative pass with one lane per pattern (all rate classes in a
-    # lane) gives the same derivatives as the (pattern, rate) form
-    monkeypatch.setenv("PHY_PRUNE_DERIV_FORM", "lane")
-    for N in (5, 64, 300):
-        _say(f"lane-form derivatives N={N} R={R}")
-        ch, root = random_tree(rng, N)
-        P = int(rng.integers(100, 900))
-        bl = rng.exponential(0.1, 2 * N - 1) + 1e-3
-        bl[root] = 0.0
-        model = _model(rng, R)
-        tips = _tips(rng, N, P)
-        w = rng.integers(1, 6, P).astype(np.uint32)
-        ov, osl, od1, od2 = _oracle(N, ch, root, bl, model, tips, w, True)
-        scale = _deriv_scale(N, ch, root, bl, model, tips, w)
-        mask = np.arange(2 * N - 1) != root
-        g = _run(ctx, N, ch, root, bl, model, tips, w, F64, True)
-        assert abs(g["lnl"] - ov) <= 1e-10 * abs(ov)
-        assert np.all(np.abs(g["d1"] - od1)[mask] <= 1e-8 * (np.abs(od1[mask]) + scale))
-        assert np.all(np.abs(g["d2"] - od2)[mask] <= 1e-8 * (np.abs(od2[mask]) + scale))
-        g32 = _run(ctx, N, ch, root, bl, model, tips, w, F32, True)
-        assert np.all(np.abs(g32["d1"] - od1)[mask] <= 2e-3 * (np.abs(od1[mask]) + scale))
-        assert np.all(np.abs(g32["d2"] - od2)[mask] <= 2e-3 * (np.abs(od2[mask]) + scale))
-
-
 def test_stored_partials_match_oracle_pieces(ctx, rng):
     # the partial of a cherry is (P_a t_a) * (P_b t_b): check one node directly
     from phylocaml_amd import device as D
