@@ -147,10 +147,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
     __syncthreads();
     if (tid == 0) s_item = (int)atomicAdd(a.counter, 1u);
     __syncthreads();
-    const int item = s_item;
+    // uniform in fact; readfirstlane tells the compiler, so everything
+    // derived from the item (tree offsets, fifo ids, operand addresses) is
+    // scalar
+    const int item = __builtin_amdgcn_readfirstlane(s_item);
     if (item >= a.n_items) break;
     const int2 it = a.items[item];
-    const int t = it.x, tile = it.y;
+    const int t = __builtin_amdgcn_readfirstlane(it.x), tile = __builtin_amdgcn_readfirstlane(it.y);
     const int64_t pitch = a.pitch[t];
     const int64_t p = (int64_t)tile * (16 * WAVES) + wave * 16 + pl;  // < pitch by construction
     const int N = a.n_leaves[t];
@@ -178,8 +181,22 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
                                          16, 0, 0);
       }
     };
-    // tip codes of the next two leaves of the stream (the fifo is padded)
-    auto tip_at = [&](int k) -> uint32_t { return tips[(size_t)fifo[k] * pitch + p]; };
+    // leaf ids of the stream, 64 to a chunk in VGPRs (lane l holds id base + l)
+    // and read with v_readlane; ids are asked for in increasing order
+    const int nf = N + 2;  // the fifo is padded with two ids
+    int idb = 0;
+    int id_cur = fifo[min(l, nf - 1)], id_nxt = fifo[min(64 + l, nf - 1)];
+    auto fifo_id = [&](int k) -> int {
+      if (k >= idb + 64) {
+        id_cur = id_nxt;
+        idb += 64;
+        id_nxt = fifo[min(idb + 64 + l, nf - 1)];
+      }
+      const int j = k - idb;
+      return j < 64 ? __builtin_amdgcn_readlane(id_cur, j) : __builtin_amdgcn_readlane(id_nxt, j - 64);
+    };
+    // tip codes of the next two leaves of the stream
+    auto tip_at = [&](int k) -> uint32_t { return tips[(size_t)fifo_id(k) * pitch + p]; };
     uint32_t t0 = tip_at(0), t1 = tip_at(1);
     auto next_tip = [&]() -> uint32_t {
       const uint32_t v = t0;
