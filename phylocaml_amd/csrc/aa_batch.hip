@@ -181,22 +181,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
                                          16, 0, 0);
       }
     };
-    // leaf ids of the stream, 64 to a chunk in VGPRs (lane l holds id base + l)
-    // and read with v_readlane; ids are asked for in increasing order
-    const int nf = N + 2;  // the fifo is padded with two ids
-    int idb = 0;
-    int id_cur = fifo[min(l, nf - 1)], id_nxt = fifo[min(64 + l, nf - 1)];
-    auto fifo_id = [&](int k) -> int {
-      if (k >= idb + 64) {
-        id_cur = id_nxt;
-        idb += 64;
-        id_nxt = fifo[min(idb + 64 + l, nf - 1)];
-      }
-      const int j = k - idb;
-      return j < 64 ? __builtin_amdgcn_readlane(id_cur, j) : __builtin_amdgcn_readlane(id_nxt, j - 64);
-    };
-    // tip codes of the next two leaves of the stream
-    auto tip_at = [&](int k) -> uint32_t { return tips[(size_t)fifo_id(k) * pitch + p]; };
+    // tip codes of the next two leaves of the stream (the fifo is padded)
+    auto tip_at = [&](int k) -> uint32_t { return tips[(size_t)fifo[k] * pitch + p]; };
     uint32_t t0 = tip_at(0), t1 = tip_at(1);
     auto next_tip = [&]() -> uint32_t {
       const uint32_t v = t0;
