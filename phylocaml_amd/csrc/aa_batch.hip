@@ -137,11 +137,18 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
   constexpr int STAGE = 2 * R * 2 * OPB;  // uint4 per node: [c][r][blk][lane]
   constexpr int NBLK = 2 * R * 2;         // 1 KiB operand blocks per node
   __shared__ uint4 stage[2][STAGE];       // double buffer: node i and node i+1
+  __shared__ uint4 tipop[(KAA + 1) * 4];  // B operand of a tip: [code 0..20][quarter]
   __shared__ int s_item;
   extern __shared__ uint4 dyn[];
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, q = l >> 4, pl = l & 15;
   // per-wave stack: depth x (R operands of 64 lanes) + exponents
   uint4* stk = dyn + (size_t)wave * a.depth * (R * 64 + 16);
+  // the 21 tip operands (a state 0..19, or missing = all 20 states) for each
+  // quarter of the lanes, built once; a tip child is then one LDS read
+  for (int e = tid; e < (KAA + 1) * 4; e += NT) {
+    const bf16x8 b = tip_operand((uint32_t)(e >> 2), e & 3);
+    tipop[e] = *reinterpret_cast<const uint4*>(&b);
+  }
 
   for (;;) {
     __syncthreads();
@@ -219,7 +226,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
         bf16x8 bop[R];
         int e = 0;
         if (src == SRC_LEAF) {
-          const bf16x8 tb = tip_operand(next_tip(), q);
+          const uint4 tu = tipop[min(next_tip(), (uint32_t)KAA) * 4 + q];
+          const bf16x8 tb = *reinterpret_cast<const bf16x8*>(&tu);
 #pragma unroll
           for (int r = 0; r < R; ++r) bop[r] = tb;
         } else if (src == SRC_ACC) {
