@@ -274,8 +274,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
       for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int v = 0; v < 8; ++v) mx = fmaxf(mx, acc[r][v]);
-      mx = fmaxf(mx, __shfl_xor(mx, 16, 64));
-      mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+      // with lanes l ^ 16 and l ^ 32 (v_permlane16/32_swap: no LDS round trip)
+      {
+        const unsigned u = __float_as_uint(mx);
+        const auto s16 = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+        mx = fmaxf(__uint_as_float(s16[0]), __uint_as_float(s16[1]));
+        const unsigned u2 = __float_as_uint(mx);
+        const auto s32 = __builtin_amdgcn_permlane32_swap(u2, u2, false, false);
+        mx = fmaxf(__uint_as_float(s32[0]), __uint_as_float(s32[1]));
+      }
       if (mx < 0x1p-24f && mx > 0.f) {
         int ex;
         (void)frexpf(mx, &ex);
