@@ -272,15 +272,22 @@ __device__ __forceinline__ void rescale4(real* v, int* e) {
   }
 }
 
-// State of one wave of the post-order pass: accumulator, stack, tip stream.
-template <typename real, int R, bool FUSED>
+// State of one wave of the post-order pass: accumulators of its G rate
+// classes, stack, tip stream.  A block is R / G waves over the same 64
+// patterns; wave w evaluates rate classes w*G .. w*G+G-1, each with its own
+// per-(pattern, rate) exponent, so the classes never interact before the
+// root.  With G > 1 a leaf's 0/1 expansion, the program bookkeeping and the
+// stack addressing are shared by G classes.
+template <typename real, int R, int G, bool FUSED>
 struct DownState {
-  static constexpr int BD = 64 * R;  // block: one wave per rate class
-  real acc[4] = {(real)0, (real)0, (real)0, (real)0};
-  int acc_e = 0, sp = 0;
-  real* stk;       // [depth][4][BD]
-  int32_t* stke;   // [depth][BD]
-  int tid, r, N;
+  static_assert(R % G == 0, "rate classes per wave must divide R");
+  static constexpr int BD = 64 * (R / G);
+  real acc[G][4];
+  int acc_e[G];
+  int sp = 0;
+  real* stk;       // [depth][G * 4][BD]
+  int32_t* stke;   // [depth][G][BD]
+  int tid, r0, N;
   int64_t pitch, pc, p;
   bool act;
   real* part;
@@ -290,75 +297,113 @@ struct DownState {
   __device__ __forceinline__ void spill_if(int fl) {
     if (FUSED && (fl & DOWN_SPILL)) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) stk[(sp * 4 + k) * BD + tid] = acc[k];
-      stke[sp * BD + tid] = acc_e;
+      for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) stk[(sp * G * 4 + g * 4 + k) * BD + tid] = acc[g][k];
+        stke[(sp * G + g) * BD + tid] = acc_e[g];
+      }
       ++sp;
     }
   }
-  // operand A of an entry (leaf, accumulator, stack or stored partial of child a)
-  __device__ __forceinline__ void operand_a(int sa, int a, real* src, int* e) {
-    *e = 0;
+  // y[g] = P[g] x[g] for operand A (leaf, accumulator, stack or stored child a)
+  __device__ __forceinline__ void product_a(int sa, int a, const real (*P)[16], real (*y)[4], int* e) {
     if (sa == SRC_LEAF) {
-      tipvec(ts.get(), src);
+      real x[4];
+      tipvec(ts.get(), x);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        matvec4(P[g], x, y[g]);
+        e[g] = 0;
+      }
     } else if (sa == SRC_ACC) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) src[k] = acc[k];
-      *e = acc_e;
+      for (int g = 0; g < G; ++g) {
+        matvec4(P[g], acc[g], y[g]);
+        e[g] = acc_e[g];
+      }
     } else if (FUSED) {
       --sp;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) src[k] = stk[(sp * 4 + k) * BD + tid];
-      *e = stke[sp * BD + tid];
-    } else {
-      const real* sv = part + (((size_t)(a - N) * R + r) * 4) * pitch + pc;
+      for (int g = 0; g < G; ++g) {
+        real x[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) src[k] = sv[(size_t)k * pitch];
-      *e = ex[((size_t)(a - N) * R + r) * pitch + pc];
+        for (int k = 0; k < 4; ++k) x[k] = stk[(sp * G * 4 + g * 4 + k) * BD + tid];
+        e[g] = stke[(sp * G + g) * BD + tid];
+        matvec4(P[g], x, y[g]);
+      }
+    } else {
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const real* sv = part + (((size_t)(a - N) * R + r0 + g) * 4) * pitch + pc;
+        real x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = sv[(size_t)k * pitch];
+        e[g] = ex[((size_t)(a - N) * R + r0 + g) * pitch + pc];
+        matvec4(P[g], x, y[g]);
+      }
     }
   }
   // operand B (leaf or accumulator)
-  __device__ __forceinline__ void operand_b(int sb, real* src, int* e) {
-    *e = 0;
+  __device__ __forceinline__ void product_b(int sb, const real (*P)[16], real (*y)[4], int* e) {
     if (sb == SRC_LEAF) {
-      tipvec(ts.get(), src);
+      real x[4];
+      tipvec(ts.get(), x);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        matvec4(P[g], x, y[g]);
+        e[g] = 0;
+      }
     } else {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) src[k] = acc[k];
-      *e = acc_e;
+      for (int g = 0; g < G; ++g) {
+        matvec4(P[g], acc[g], y[g]);
+        e[g] = acc_e[g];
+      }
     }
   }
-  __device__ __forceinline__ void combine(int v, const real* xa, const real* xb, int e) {
+  __device__ __forceinline__ void combine(int v, const real (*xa)[4], const real (*xb)[4],
+                                          const int* ea, const int* eb) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) acc[k] = xa[k] * xb[k];
-    acc_e = e;
-    rescale4(acc, &acc_e);
-    if (!FUSED && act) {
-      real* d = part + (((size_t)(v - N) * R + r) * 4) * pitch + p;
+    for (int g = 0; g < G; ++g) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) d[(size_t)k * pitch] = acc[k];
-      ex[((size_t)(v - N) * R + r) * pitch + p] = acc_e;
+      for (int k = 0; k < 4; ++k) acc[g][k] = xa[g][k] * xb[g][k];
+      acc_e[g] = ea[g] + eb[g];
+      rescale4(acc[g], &acc_e[g]);
+      if (!FUSED && act) {
+        real* d = part + (((size_t)(v - N) * R + r0 + g) * 4) * pitch + p;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) d[(size_t)k * pitch] = acc[g][k];
+        ex[((size_t)(v - N) * R + r0 + g) * pitch + p] = acc_e[g];
+      }
     }
   }
 };
 
 // root: S_r = sum_i pi_i L[r][i]; combine the R classes in wave 0
-template <typename real, int R, bool FUSED>
-__device__ __forceinline__ void down_root(const DownState<real, R, FUSED>& st, unsigned char* lds_raw,
+template <typename real, int R, int G, bool FUSED>
+__device__ __forceinline__ void down_root(const DownState<real, R, G, FUSED>& st, unsigned char* lds_raw,
                                           const double* __restrict__ pi,
                                           const double* __restrict__ probs,
                                           const uint32_t* __restrict__ weights,
                                           double* __restrict__ site_lnl, double* __restrict__ wave_sums) {
-  const int lane = st.tid & 63, r = st.r;
+  const int lane = st.tid & 63;
   double* rootS = reinterpret_cast<double*>(lds_raw);
   int* rootE = reinterpret_cast<int*>(rootS + R * 64);
-  double S = 0.0;
+  double S[G];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) S += pi[i] * (double)st.acc[i];
+  for (int g = 0; g < G; ++g) {
+    S[g] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) S[g] += pi[i] * (double)st.acc[g][i];
+  }
   __syncthreads();  // every wave is done with its stack rows
-  rootS[r * 64 + lane] = S;
-  rootE[r * 64 + lane] = st.acc_e;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    rootS[(st.r0 + g) * 64 + lane] = S[g];
+    rootE[(st.r0 + g) * 64 + lane] = st.acc_e[g];
+  }
   __syncthreads();
-  if (r == 0) {
+  if (st.r0 == 0) {
     int m = rootE[lane];
 #pragma unroll
     for (int q = 1; q < R; ++q) m = max(m, rootE[q * 64 + lane]);
@@ -377,13 +422,13 @@ __device__ __forceinline__ void down_root(const DownState<real, R, FUSED>& st, u
   }
 }
 
-template <typename real, int R, bool FUSED>
-__device__ __forceinline__ void down_init(DownState<real, R, FUSED>& st, unsigned char* lds_raw,
+template <typename real, int R, int G, bool FUSED>
+__device__ __forceinline__ void down_init(DownState<real, R, G, FUSED>& st, unsigned char* lds_raw,
                                           const int32_t* leaf_ids, const uint64_t* tips,
                                           int64_t stride, int64_t n_pat, int64_t pitch, real* part,
                                           int32_t* ex, int N, int stack_depth) {
   st.tid = threadIdx.x;
-  st.r = __builtin_amdgcn_readfirstlane(st.tid >> 6);  // rate class of this wave
+  st.r0 = __builtin_amdgcn_readfirstlane((st.tid >> 6) * G);  // first rate class of this wave
   st.p = (int64_t)blockIdx.x * 64 + (st.tid & 63);
   st.act = st.p < n_pat;
   st.pc = st.act ? st.p : 0;
@@ -391,16 +436,23 @@ __device__ __forceinline__ void down_init(DownState<real, R, FUSED>& st, unsigne
   st.N = N;
   st.part = part;
   st.ex = ex;
-  // LDS: (fused) stack [depth][4][bd] real + [depth][bd] int; at the root,
-  // when the stack is empty, the same bytes hold the rate exchange
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    st.acc_e[g] = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) st.acc[g][k] = (real)0;
+  }
+  // LDS: (fused) stack [depth][G*4][BD] real + [depth][G][BD] int; at the
+  // root, when the stack is empty, the same bytes hold the rate exchange
   // [R][64] double S + [R][64] int e
   st.stk = reinterpret_cast<real*>(lds_raw);
-  st.stke = reinterpret_cast<int32_t*>(st.stk + (size_t)(FUSED ? stack_depth : 0) * 4 * st.BD);
+  st.stke = reinterpret_cast<int32_t*>(st.stk + (size_t)(FUSED ? stack_depth : 0) * G * 4 * st.BD);
   st.ts.init(tips, stride, st.pc, leaf_ids);
 }
 
-// VALU form: the wave's 4x4 P of both children are wave-uniform (SGPRs)
-template <typename real, int R, bool FUSED>
+// The wave's 4x4 P of both children for each of its G classes are
+// wave-uniform (SGPRs): 16 FMAs with an SGPR operand per product.
+template <typename real, int R, int G, bool FUSED>
 __device__ __forceinline__ void prune_down_body(
     const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids,
     const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
@@ -408,51 +460,42 @@ __device__ __forceinline__ void prune_down_body(
     real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
     double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth) {
   extern __shared__ unsigned char lds_raw[];
-  DownState<real, R, FUSED> st;
+  DownState<real, R, G, FUSED> st;
   down_init(st, lds_raw, leaf_ids, tips, stride, n_pat, pitch, part, ex, N, stack_depth);
-  const int r = st.r;
+  const int r0 = st.r0;
   for_each_entry(prog, n_ops, [&](int, const int4 op) {
-    real PA[16], PB[16];
-    load_mat<real, R>(Pm + ((size_t)op.y * R + r) * 16, PA);
-    load_mat<real, R>(Pm + ((size_t)op.z * R + r) * 16, PB);
-    const int fl = op.w, sa = fl & 3, sb = (fl >> 2) & 3;
+    real PA[G][16], PB[G][16];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      load_mat<real, R>(Pm + ((size_t)op.y * R + r0 + g) * 16, PA[g]);
+      load_mat<real, R>(Pm + ((size_t)op.z * R + r0 + g) * 16, PB[g]);
+    }
+    const int fl = op.w;
     st.spill_if(fl);
-    real src[4], xa[4], xb[4];
-    int ea, eb;
-    if (sa == SRC_ACC) {  // the accumulator is read in place
-      matvec4(PA, st.acc, xa);
-      ea = st.acc_e;
-    } else {
-      st.operand_a(sa, op.y, src, &ea);
-      matvec4(PA, src, xa);
-    }
-    if (sb == SRC_ACC) {
-      matvec4(PB, st.acc, xb);
-      eb = st.acc_e;
-    } else {
-      st.operand_b(sb, src, &eb);
-      matvec4(PB, src, xb);
-    }
-    st.combine(op.x, xa, xb, ea + eb);
+    real xa[G][4], xb[G][4];
+    int ea[G], eb[G];
+    st.product_a(fl & 3, op.y, PA, xa, ea);
+    st.product_b((fl >> 2) & 3, PB, xb, eb);
+    st.combine(op.x, xa, xb, ea, eb);
   });
   down_root(st, lds_raw, pi, probs, weights, site_lnl, wave_sums);
 }
 
-template <typename real, int R, bool FUSED>
+template <typename real, int R, int G, bool FUSED>
 __global__ __launch_bounds__(512) void k_prune_down(
     const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids,
     const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
     const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
     real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
     double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth) {
-  prune_down_body<real, R, FUSED>(prog, n_ops, leaf_ids, tips, stride, n_pat, pitch, Pm, pi, probs,
-                                  part, ex, weights, site_lnl, wave_sums, N, stack_depth);
+  prune_down_body<real, R, G, FUSED>(prog, n_ops, leaf_ids, tips, stride, n_pat, pitch, Pm, pi,
+                                     probs, part, ex, weights, site_lnl, wave_sums, N, stack_depth);
 }
 
 // Many candidate topologies over the same tips and weights (fused form):
 // blockIdx.y selects candidate c, whose program, leaf order, P matrices and
 // per-wave sums sit at fixed strides.
-template <typename real, int R>
+template <typename real, int R, int G>
 __global__ __launch_bounds__(512) void k_prune_down_batch(
     const int4* __restrict__ prog, const int32_t* __restrict__ leaf_ids, int N,
     const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
@@ -464,8 +507,8 @@ __global__ __launch_bounds__(512) void k_prune_down_batch(
   const int4* pg = prog + c * (N - 1);
   const int32_t* li = leaf_ids + c * (int64_t)phy_fifo_padded(N);
   const real* P = Pm + c * (int64_t)M * R * 16;
-  prune_down_body<real, R, true>(pg, N - 1, li, tips, stride, n_pat, pitch, P, pi, probs, nullptr,
-                                 nullptr, weights, nullptr, wave_sums + c * n_waves, N, stack_depth);
+  prune_down_body<real, R, G, true>(pg, N - 1, li, tips, stride, n_pat, pitch, P, pi, probs, nullptr,
+                                    nullptr, weights, nullptr, wave_sums + c * n_waves, N, stack_depth);
 }
 
 // Helpers of the derivative pass, whose threads are (pattern, rate) quads.
@@ -946,12 +989,27 @@ int env_block(const char* name, int dflt) {
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Rate classes per wave of the post-order pass.  Two classes per wave share
+// the tip expansion and the program bookkeeping, and their four 4x4 P fill 64
+// SGPRs, so only single precision can take two.  The fused stack costs the
+// same LDS per 64 patterns either way, so with two classes per wave an
+// LDS-bound launch keeps half as many waves resident: the fused form
+// measures faster with one class per wave, the stored form with two.
+// PHY_PRUNE_RPW=1|2 overrides the choice.
+int rates_per_wave(size_t esz, int R, bool fused) {
+  if (esz != 4 || R % 2) return 1;
+  const char* s = getenv("PHY_PRUNE_RPW");
+  if (s && (atoi(s) == 1 || atoi(s) == 2)) return atoi(s);
+  return fused ? 1 : 2;
+}
+
 // Workspace plan of phy_lk_tree
 struct Plan {
   size_t off_P = 0, off_E = 0, off_tab = 0, off_part = 0, off_ex = 0, off_bsum = 0, off_wsum = 0;
   size_t total = 0;
   int64_t pitch = 0, n_blocks = 0, n_waves = 0, dblocks = 0, n_dwaves = 0;
   int bs = 0, dbs = 0;  // block size of the post-order pass / derivative pass
+  int rpw = 1;          // rate classes per wave of the post-order pass
   size_t esz = 8, lds = 0, dlds = 0;
   bool fused = false;
 };
@@ -959,7 +1017,7 @@ struct Plan {
 constexpr size_t kLdsMax = 64 * 1024;
 
 // LDS of the post-order pass: the (fused) stack, which also holds the rate
-// exchange at the root
+// exchange at the root (the same for any number of rate classes per wave)
 size_t down_lds(int R, size_t esz, bool fused, int depth) {
   const size_t bs = 64 * (size_t)R;
   const size_t root = (size_t)R * 64 * 12;
@@ -984,11 +1042,12 @@ Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t for
   const bool fused = (form & PHY_FUSED) != 0;
   const size_t esz = pl.esz;
   const int depth = t->down_depth;
-  pl.bs = 64 * R;  // one wave per rate class over 64 patterns
   // a fused stack that does not fit in LDS falls back to stored partials
   // (same arithmetic and result, more HBM traffic)
   pl.fused = fused && down_lds(R, esz, true, depth) <= kLdsMax;
   pl.lds = down_lds(R, esz, pl.fused, depth);
+  pl.rpw = rates_per_wave(esz, R, pl.fused);
+  pl.bs = 64 * R / pl.rpw;  // R / rpw waves over 64 patterns
   if (derivs) {
     pl.dbs = fit_block(env_block("PHY_PRUNE_DERIV_BLOCK", 256),
                        [&](int bs) { return deriv_lds(R, esz, bs, t->up_depth); });
@@ -1040,14 +1099,16 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
   real* part = fused ? nullptr : reinterpret_cast<real*>(work + pl.off_part);
   int32_t* ex = fused ? nullptr : reinterpret_cast<int32_t*>(work + pl.off_ex);
   double* bsum = reinterpret_cast<double*>(work + pl.off_bsum);
-  if (fused)
-    hipLaunchKernelGGL((k_prune_down<real, R, true>), dim3((unsigned)pl.n_blocks), dim3(pl.bs), pl.lds,
-                       ctx->stream, t->d_down_prog, N - 1, t->d_down_fifo, tips, stride, n_pat,
-                       pl.pitch, P, d_pi, d_probs, part, ex, weights, site_lnl, bsum, N, t->down_depth);
+  auto down = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)pl.n_blocks), dim3(pl.bs), pl.lds, ctx->stream,
+                       t->d_down_prog, N - 1, t->d_down_fifo, tips, stride, n_pat, pl.pitch, P, d_pi,
+                       d_probs, part, ex, weights, site_lnl, bsum, N, t->down_depth);
+  };
+  constexpr int G2 = (R % 2 == 0 && sizeof(real) == 4) ? 2 : 1;
+  if (pl.rpw == 2 && G2 == 2)
+    fused ? down(k_prune_down<real, R, G2, true>) : down(k_prune_down<real, R, G2, false>);
   else
-    hipLaunchKernelGGL((k_prune_down<real, R, false>), dim3((unsigned)pl.n_blocks), dim3(pl.bs), pl.lds,
-                       ctx->stream, t->d_down_prog, N - 1, t->d_down_fifo, tips, stride, n_pat,
-                       pl.pitch, P, d_pi, d_probs, part, ex, weights, site_lnl, bsum, N, t->down_depth);
+    fused ? down(k_prune_down<real, R, 1, true>) : down(k_prune_down<real, R, 1, false>);
   PHY_LAUNCHED(ctx);
   hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, ctx->stream, bsum, pl.n_waves, (int64_t)1, lnl);
   PHY_LAUNCHED(ctx);
@@ -1219,14 +1280,15 @@ namespace {
 struct SprPlan {
   size_t off_P, off_bl, off_prog, off_fifo, off_ws, total;
   int64_t pitch, n_blocks, n_waves;
-  int bs, depth_cap;
+  int bs, depth_cap, rpw;
 };
 
 SprPlan spr_plan(int32_t N, const phy_model* m, int64_t n_pat, int32_t form, int32_t chunk) {
   SprPlan pl;
   const int M = 2 * N - 1;
   const size_t esz = (form & PHY_F32) ? 4 : 8;
-  pl.bs = 64 * m->R;
+  pl.rpw = rates_per_wave(esz, m->R, true);
+  pl.bs = 64 * m->R / pl.rpw;
   pl.pitch = ((n_pat + 255) / 256) * 256;
   pl.n_blocks = pl.pitch / 64;
   pl.n_waves = pl.pitch / 64;
@@ -1257,10 +1319,17 @@ int spr_launch(phy_ctx* ctx, const phy_model* m, int N, int64_t n_pat, const uin
   const size_t lds = down_lds(R, sizeof(real), true, depth);
   if (lds > kLdsMax) return set_err(ctx, PHY_ERANGE, "spr scores: LDS stack too deep");
   double* ws = reinterpret_cast<double*>(work + pl.off_ws);
-  hipLaunchKernelGGL((k_prune_down_batch<real, R>), dim3((unsigned)(pl.pitch / 64), (unsigned)C),
-                     dim3(64 * R), lds, ctx->stream, reinterpret_cast<const int4*>(work + pl.off_prog),
-                     reinterpret_cast<const int32_t*>(work + pl.off_fifo), N, tips, stride, n_pat,
-                     pl.pitch, P, d_pi, d_pi + K, w, ws, pl.n_waves, depth);
+  auto down = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)(pl.pitch / 64), (unsigned)C), dim3(pl.bs), lds,
+                       ctx->stream, reinterpret_cast<const int4*>(work + pl.off_prog),
+                       reinterpret_cast<const int32_t*>(work + pl.off_fifo), N, tips, stride, n_pat,
+                       pl.pitch, P, d_pi, d_pi + K, w, ws, pl.n_waves, depth);
+  };
+  constexpr int G2 = (R % 2 == 0 && sizeof(real) == 4) ? 2 : 1;
+  if (pl.rpw == 2 && G2 == 2)
+    down(k_prune_down_batch<real, R, G2>);
+  else
+    down(k_prune_down_batch<real, R, 1>);
   PHY_LAUNCHED(ctx);
   hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)C), dim3(256), 0, ctx->stream, ws, pl.n_waves, out);
   PHY_LAUNCHED(ctx);

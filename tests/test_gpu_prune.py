@@ -111,6 +111,32 @@ def test_lnl_and_derivatives_match_oracle(ctx, rng, N, R):
         assert fu64["lnl"] == g["lnl"]
 
 
+@pytest.mark.parametrize("rpw", ["1", "2"])
+@pytest.mark.parametrize("R", [2, 4, 8])
+def test_rate_classes_per_wave(ctx, rng, monkeypatch, R, rpw):
+    # single precision evaluates one or two rate classes per wave: both
+    # layouts, stored and fused, give the oracle's lnL and site lnL
+    monkeypatch.setenv("PHY_PRUNE_RPW", rpw)
+    N = 150
+    _say(f"prune rate classes per wave R={R} rpw={rpw}")
+    ch, root = random_tree(rng, N)
+    P = 1000
+    bl = rng.exponential(0.1, 2 * N - 1) + 1e-3
+    bl[root] = 0.0
+    model = _model(rng, R)
+    tips = _tips(rng, N, P)
+    w = rng.integers(1, 6, P).astype(np.uint32)
+    ov, osl, od1, _ = _oracle(N, ch, root, bl, model, tips, w, True)
+    mask = np.arange(2 * N - 1) != root
+    g32 = _run(ctx, N, ch, root, bl, model, tips, w, F32, True)
+    assert abs(g32["lnl"] - ov) <= 1e-6 * abs(ov), (g32["lnl"], ov)
+    assert np.allclose(g32["site"], osl, rtol=1e-5, atol=1e-5)
+    assert np.all(np.abs(g32["d1"] - od1)[mask] <= 2e-3 * (np.abs(od1[mask]) + float(w.sum())))
+    fu32 = _run(ctx, N, ch, root, bl, model, tips, w, F32 | FUSED, False)
+    assert fu32["lnl"] == g32["lnl"]
+    assert np.array_equal(fu32["site"], g32["site"])
+
+
 def test_stored_partials_match_oracle_pieces(ctx, rng):
     # the partial of a cherry is (P_a t_a) * (P_b t_b): check one node directly
     from phylocaml_amd import device as D
