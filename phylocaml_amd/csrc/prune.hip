@@ -20,6 +20,7 @@
 // per-pattern int32 exponent carries the factor, so FP32 partials cover the
 // FP64 dynamic range without rounding from the scaling itself.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <thread>
 #include <type_traits>
@@ -272,21 +273,36 @@ __device__ __forceinline__ void rescale4(real* v, int* e) {
   }
 }
 
+// compile-time loop: f(integral_constant<int, I>) for I in [B, E)
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
 // State of one wave of the post-order pass: accumulators of its G rate
 // classes, stack, tip stream.  A block is R / G waves over the same 64
 // patterns; wave w evaluates rate classes w*G .. w*G+G-1, each with its own
 // per-(pattern, rate) exponent, so the classes never interact before the
 // root.  With G > 1 a leaf's 0/1 expansion, the program bookkeeping and the
-// stack addressing are shared by G classes.
-template <typename real, int R, int G, bool FUSED>
+// stack addressing are shared by G classes.  The bottom LV levels of the
+// fused stack live in registers (selected by the wave-uniform stack pointer),
+// so the LDS holds LV levels less.
+template <typename real, int R, int G, bool FUSED, int LV = 0>
 struct DownState {
   static_assert(R % G == 0, "rate classes per wave must divide R");
+  static_assert(FUSED || LV == 0, "register stack levels belong to the fused form");
   static constexpr int BD = 64 * (R / G);
+  static constexpr int NB = LV > 0 ? LV : 1;
   real acc[G][4];
   int acc_e[G];
-  int sp = 0;
-  real* stk;       // [depth][G * 4][BD]
-  int32_t* stke;   // [depth][G][BD]
+  real bot[NB][G][4];
+  int bot_e[NB][G];
+  int sp = 0;  // wave-uniform
+  real* stk;       // [depth - LV][G * 4][BD]
+  int32_t* stke;   // [depth - LV][G][BD]
   int tid, r0, N;
   int64_t pitch, pc, p;
   bool act;
@@ -296,11 +312,25 @@ struct DownState {
 
   __device__ __forceinline__ void spill_if(int fl) {
     if (FUSED && (fl & DOWN_SPILL)) {
+      if (sp < LV) {
+        // value selects on the uniform level, so the levels stay registers
+        static_for<0, LV>([&](auto l) {
+          const bool hit = sp == l;
 #pragma unroll
-      for (int g = 0; g < G; ++g) {
+          for (int g = 0; g < G; ++g) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) stk[(sp * G * 4 + g * 4 + k) * BD + tid] = acc[g][k];
-        stke[(sp * G + g) * BD + tid] = acc_e[g];
+            for (int k = 0; k < 4; ++k) bot[l][g][k] = hit ? acc[g][k] : bot[l][g][k];
+            bot_e[l][g] = hit ? acc_e[g] : bot_e[l][g];
+          }
+        });
+      } else {
+        const int lv = sp - LV;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) stk[(lv * G * 4 + g * 4 + k) * BD + tid] = acc[g][k];
+          stke[(lv * G + g) * BD + tid] = acc_e[g];
+        }
       }
       ++sp;
     }
@@ -323,13 +353,32 @@ struct DownState {
       }
     } else if (FUSED) {
       --sp;
+      if (sp < LV) {
 #pragma unroll
-      for (int g = 0; g < G; ++g) {
-        real x[4];
+        for (int g = 0; g < G; ++g) {
+          real x[4];
+          int xe = bot_e[0][g];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) x[k] = stk[(sp * G * 4 + g * 4 + k) * BD + tid];
-        e[g] = stke[(sp * G + g) * BD + tid];
-        matvec4(P[g], x, y[g]);
+          for (int k = 0; k < 4; ++k) x[k] = bot[0][g][k];
+          static_for<1, LV>([&](auto l) {
+            const bool hit = sp == l;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) x[k] = hit ? bot[l][g][k] : x[k];
+            xe = hit ? bot_e[l][g] : xe;
+          });
+          matvec4(P[g], x, y[g]);
+          e[g] = xe;
+        }
+      } else {
+        const int lv = sp - LV;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          real x[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) x[k] = stk[(lv * G * 4 + g * 4 + k) * BD + tid];
+          e[g] = stke[(lv * G + g) * BD + tid];
+          matvec4(P[g], x, y[g]);
+        }
       }
     } else {
 #pragma unroll
@@ -380,8 +429,8 @@ struct DownState {
 };
 
 // root: S_r = sum_i pi_i L[r][i]; combine the R classes in wave 0
-template <typename real, int R, int G, bool FUSED>
-__device__ __forceinline__ void down_root(const DownState<real, R, G, FUSED>& st, unsigned char* lds_raw,
+template <typename real, int R, int G, bool FUSED, int LV>
+__device__ __forceinline__ void down_root(const DownState<real, R, G, FUSED, LV>& st, unsigned char* lds_raw,
                                           const double* __restrict__ pi,
                                           const double* __restrict__ probs,
                                           const uint32_t* __restrict__ weights,
@@ -422,8 +471,8 @@ __device__ __forceinline__ void down_root(const DownState<real, R, G, FUSED>& st
   }
 }
 
-template <typename real, int R, int G, bool FUSED>
-__device__ __forceinline__ void down_init(DownState<real, R, G, FUSED>& st, unsigned char* lds_raw,
+template <typename real, int R, int G, bool FUSED, int LV>
+__device__ __forceinline__ void down_init(DownState<real, R, G, FUSED, LV>& st, unsigned char* lds_raw,
                                           const int32_t* leaf_ids, const uint64_t* tips,
                                           int64_t stride, int64_t n_pat, int64_t pitch, real* part,
                                           int32_t* ex, int N, int stack_depth) {
@@ -442,17 +491,18 @@ __device__ __forceinline__ void down_init(DownState<real, R, G, FUSED>& st, unsi
 #pragma unroll
     for (int k = 0; k < 4; ++k) st.acc[g][k] = (real)0;
   }
-  // LDS: (fused) stack [depth][G*4][BD] real + [depth][G][BD] int; at the
-  // root, when the stack is empty, the same bytes hold the rate exchange
-  // [R][64] double S + [R][64] int e
+  // LDS: (fused) stack [depth - LV][G*4][BD] real + [depth - LV][G][BD]
+  // int; at the root, when the stack is empty, the same bytes hold the rate
+  // exchange [R][64] double S + [R][64] int e
+  const int rows = FUSED ? max(stack_depth - LV, 0) : 0;
   st.stk = reinterpret_cast<real*>(lds_raw);
-  st.stke = reinterpret_cast<int32_t*>(st.stk + (size_t)(FUSED ? stack_depth : 0) * G * 4 * st.BD);
+  st.stke = reinterpret_cast<int32_t*>(st.stk + (size_t)rows * G * 4 * st.BD);
   st.ts.init(tips, stride, st.pc, leaf_ids);
 }
 
 // The wave's 4x4 P of both children for each of its G classes are
 // wave-uniform (SGPRs): 16 FMAs with an SGPR operand per product.
-template <typename real, int R, int G, bool FUSED>
+template <typename real, int R, int G, bool FUSED, int LV>
 __device__ __forceinline__ void prune_down_body(
     const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids,
     const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
@@ -460,7 +510,7 @@ __device__ __forceinline__ void prune_down_body(
     real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
     double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth) {
   extern __shared__ unsigned char lds_raw[];
-  DownState<real, R, G, FUSED> st;
+  DownState<real, R, G, FUSED, LV> st;
   down_init(st, lds_raw, leaf_ids, tips, stride, n_pat, pitch, part, ex, N, stack_depth);
   const int r0 = st.r0;
   for_each_entry(prog, n_ops, [&](int, const int4 op) {
@@ -481,14 +531,14 @@ __device__ __forceinline__ void prune_down_body(
   down_root(st, lds_raw, pi, probs, weights, site_lnl, wave_sums);
 }
 
-template <typename real, int R, int G, bool FUSED>
+template <typename real, int R, int G, bool FUSED, int LV>
 __global__ __launch_bounds__(512) void k_prune_down(
     const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids,
     const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
     const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
     real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
     double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth) {
-  prune_down_body<real, R, G, FUSED>(prog, n_ops, leaf_ids, tips, stride, n_pat, pitch, Pm, pi,
+  prune_down_body<real, R, G, FUSED, LV>(prog, n_ops, leaf_ids, tips, stride, n_pat, pitch, Pm, pi,
                                      probs, part, ex, weights, site_lnl, wave_sums, N, stack_depth);
 }
 
@@ -507,7 +557,7 @@ __global__ __launch_bounds__(512) void k_prune_down_batch(
   const int4* pg = prog + c * (N - 1);
   const int32_t* li = leaf_ids + c * (int64_t)phy_fifo_padded(N);
   const real* P = Pm + c * (int64_t)M * R * 16;
-  prune_down_body<real, R, G, true>(pg, N - 1, li, tips, stride, n_pat, pitch, P, pi, probs, nullptr,
+  prune_down_body<real, R, G, true, 0>(pg, N - 1, li, tips, stride, n_pat, pitch, P, pi, probs, nullptr,
                                     nullptr, weights, nullptr, wave_sums + c * n_waves, N, stack_depth);
 }
 
@@ -540,15 +590,6 @@ __device__ __forceinline__ void rescale_pattern(real* v, int* e) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) v[k] = ldexp(v[k], -ex);
     *e += ex;
-  }
-}
-
-// compile-time loop: f(integral_constant<int, I>) for I in [B, E)
-template <int B, int E, class F>
-__device__ __forceinline__ void static_for(F&& f) {
-  if constexpr (B < E) {
-    f(std::integral_constant<int, B>{});
-    static_for<B + 1, E>(f);
   }
 }
 
@@ -1003,6 +1044,7 @@ int rates_per_wave(size_t esz, int R, bool fused) {
   return fused ? 1 : 2;
 }
 
+
 // Workspace plan of phy_lk_tree
 struct Plan {
   size_t off_P = 0, off_E = 0, off_tab = 0, off_part = 0, off_ex = 0, off_bsum = 0, off_wsum = 0;
@@ -1016,12 +1058,14 @@ struct Plan {
 
 constexpr size_t kLdsMax = 64 * 1024;
 
-// LDS of the post-order pass: the (fused) stack, which also holds the rate
-// exchange at the root (the same for any number of rate classes per wave)
-size_t down_lds(int R, size_t esz, bool fused, int depth) {
+// LDS of the post-order pass: the (fused) stack, less the levels held in
+// registers, which also holds the rate exchange at the root (the same for any
+// number of rate classes per wave)
+size_t down_lds(int R, size_t esz, bool fused, int depth, int reg_levels = 0) {
   const size_t bs = 64 * (size_t)R;
   const size_t root = (size_t)R * 64 * 12;
-  return std::max(root, fused ? (size_t)depth * bs * (4 * esz + 4) : 0);
+  const size_t rows = (size_t)std::max(depth - reg_levels, 0);
+  return std::max(root, fused ? rows * bs * (4 * esz + 4) : 0);
 }
 size_t deriv_lds(int R, size_t esz, int bs, int depth) {
   return (size_t)std::max(depth, 1) * bs * 4 * esz;
@@ -1045,8 +1089,8 @@ Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t for
   // a fused stack that does not fit in LDS falls back to stored partials
   // (same arithmetic and result, more HBM traffic)
   pl.fused = fused && down_lds(R, esz, true, depth) <= kLdsMax;
-  pl.lds = down_lds(R, esz, pl.fused, depth);
   pl.rpw = rates_per_wave(esz, R, pl.fused);
+  pl.lds = down_lds(R, esz, pl.fused, depth);  // upper bound: no level in registers
   pl.bs = 64 * R / pl.rpw;  // R / rpw waves over 64 patterns
   if (derivs) {
     pl.dbs = fit_block(env_block("PHY_PRUNE_DERIV_BLOCK", 256),
@@ -1078,6 +1122,42 @@ Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t for
   return pl;
 }
 
+// Fused stack levels held in registers (0, 1 or 2) for one rate class per
+// wave: the choice that keeps the most waves resident per CU (registers per
+// thread against LDS per block), the fewest levels on a tie.  Cached per
+// stack depth; PHY_PRUNE_REG=0|1|2 overrides it.
+template <typename real, int R>
+int register_levels(int depth, int bs) {
+  if (depth < 1) return 0;
+  const char* s = getenv("PHY_PRUNE_REG");
+  if (s && s[0] >= '0' && s[0] <= '2' && s[1] == 0) return std::min(s[0] - '0', depth);
+  static std::atomic<int> cache[64];
+  static std::atomic<int> cache_dev[64];
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const int slot = std::min(depth, 63);
+  if (depth < 64 && cache[slot].load() > 0 && cache_dev[slot].load() == dev)
+    return cache[slot].load() - 1;
+  const void* kern[3] = {reinterpret_cast<const void*>(&k_prune_down<real, R, 1, true, 0>),
+                         reinterpret_cast<const void*>(&k_prune_down<real, R, 1, true, 1>),
+                         reinterpret_cast<const void*>(&k_prune_down<real, R, 1, true, 2>)};
+  int best = 0, best_waves = -1;
+  for (int lv = 0; lv <= std::min(2, depth); ++lv) {
+    const size_t lds = down_lds(R, sizeof(real), true, depth, lv);
+    int nb = 0;
+    if (lds > kLdsMax ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern[lv], bs, lds) != hipSuccess)
+      continue;
+    const int waves = nb * (bs / 64);
+    if (waves > best_waves) best = lv, best_waves = waves;
+  }
+  if (depth < 64) {
+    cache_dev[slot].store(dev);
+    cache[slot].store(best + 1);
+  }
+  return best;
+}
+
 template <typename real, int R>
 int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused, bool derivs,
                 int64_t n_pat, const uint64_t* tips, int64_t stride, const uint32_t* weights,
@@ -1099,16 +1179,27 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
   real* part = fused ? nullptr : reinterpret_cast<real*>(work + pl.off_part);
   int32_t* ex = fused ? nullptr : reinterpret_cast<int32_t*>(work + pl.off_ex);
   double* bsum = reinterpret_cast<double*>(work + pl.off_bsum);
-  auto down = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3((unsigned)pl.n_blocks), dim3(pl.bs), pl.lds, ctx->stream,
+  auto down = [&](auto kern, size_t lds) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)pl.n_blocks), dim3(pl.bs), lds, ctx->stream,
                        t->d_down_prog, N - 1, t->d_down_fifo, tips, stride, n_pat, pl.pitch, P, d_pi,
                        d_probs, part, ex, weights, site_lnl, bsum, N, t->down_depth);
   };
   constexpr int G2 = (R % 2 == 0 && sizeof(real) == 4) ? 2 : 1;
-  if (pl.rpw == 2 && G2 == 2)
-    fused ? down(k_prune_down<real, R, G2, true>) : down(k_prune_down<real, R, G2, false>);
-  else
-    fused ? down(k_prune_down<real, R, 1, true>) : down(k_prune_down<real, R, 1, false>);
+  if (!fused) {
+    pl.rpw == 2 ? down(k_prune_down<real, R, G2, false, 0>, pl.lds)
+                : down(k_prune_down<real, R, 1, false, 0>, pl.lds);
+  } else if (pl.rpw == 2) {
+    down(k_prune_down<real, R, G2, true, 0>, pl.lds);
+  } else {
+    const int lv = register_levels<real, R>(t->down_depth, pl.bs);
+    const size_t lds = down_lds(R, sizeof(real), true, t->down_depth, lv);
+    if (lv == 2)
+      down(k_prune_down<real, R, 1, true, 2>, lds);
+    else if (lv == 1)
+      down(k_prune_down<real, R, 1, true, 1>, lds);
+    else
+      down(k_prune_down<real, R, 1, true, 0>, lds);
+  }
   PHY_LAUNCHED(ctx);
   hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, ctx->stream, bsum, pl.n_waves, (int64_t)1, lnl);
   PHY_LAUNCHED(ctx);

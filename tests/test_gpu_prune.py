@@ -111,14 +111,16 @@ def test_lnl_and_derivatives_match_oracle(ctx, rng, N, R):
         assert fu64["lnl"] == g["lnl"]
 
 
-@pytest.mark.parametrize("rpw", ["1", "2"])
+@pytest.mark.parametrize("rpw,reg", [("1", "0"), ("1", "1"), ("1", "2"), ("2", "0")])
 @pytest.mark.parametrize("R", [2, 4, 8])
-def test_rate_classes_per_wave(ctx, rng, monkeypatch, R, rpw):
-    # single precision evaluates one or two rate classes per wave: both
-    # layouts, stored and fused, give the oracle's lnL and site lnL
+def test_rate_classes_per_wave(ctx, rng, monkeypatch, R, rpw, reg):
+    # single precision evaluates one or two rate classes per wave, and the
+    # fused stack keeps 0, 1 or 2 of its levels in registers: every layout,
+    # stored and fused, gives the oracle's lnL and site lnL
     monkeypatch.setenv("PHY_PRUNE_RPW", rpw)
+    monkeypatch.setenv("PHY_PRUNE_REG", reg)
     N = 150
-    _say(f"prune rate classes per wave R={R} rpw={rpw}")
+    _say(f"prune rate classes per wave R={R} rpw={rpw} register levels={reg}")
     ch, root = random_tree(rng, N)
     P = 1000
     bl = rng.exponential(0.1, 2 * N - 1) + 1e-3
@@ -135,6 +137,11 @@ def test_rate_classes_per_wave(ctx, rng, monkeypatch, R, rpw):
     fu32 = _run(ctx, N, ch, root, bl, model, tips, w, F32 | FUSED, False)
     assert fu32["lnl"] == g32["lnl"]
     assert np.array_equal(fu32["site"], g32["site"])
+    g64 = _run(ctx, N, ch, root, bl, model, tips, w, F64, False)
+    fu64 = _run(ctx, N, ch, root, bl, model, tips, w, F64 | FUSED, False)
+    assert abs(fu64["lnl"] - ov) <= 1e-10 * abs(ov)
+    assert fu64["lnl"] == g64["lnl"]
+    assert np.array_equal(fu64["site"], g64["site"])
 
 
 def test_stored_partials_match_oracle_pieces(ctx, rng):
