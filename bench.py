@@ -297,7 +297,7 @@ def main():
                    "n_taxa": N, "n_patterns": P, "n_rates": 4, "form": "f32_fused"},
         "roofline": {"bound": "hbm", "achieved": alg_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg_gbs / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("k_prune_down<float, 4, true>"),
+                     "traffic": measured_traffic("k_prune_down<float, 4, 1, true,"),
                      "compute_tflops": tflops, "compute_peak_tflops": FP32_PEAK_TF,
                      "note": "achieved = 136 B per internal pattern.node (the unfused "
                              "algorithmic traffic) x 511 nodes x 200,000 patterns / step "

@@ -8,6 +8,7 @@ compose_gtr, ...) served by libphylo_hip.so instead of LAPACK.
 from __future__ import annotations
 
 import ctypes as C
+import math
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -120,21 +121,23 @@ def _priors(spec: Spec) -> np.ndarray:
     return p if abs(s - 1.0) <= 2.220446049250313e-16 * 4 else p / s
 
 
+def _site_rates(spec: Spec):
+    sv = spec.site_variation
+    if sv[0] == "Constant":
+        return np.array([1.0]), np.array([1.0])
+    if sv[0] == "DiscreteGamma":
+        k, alpha = int(sv[1]), float(sv[2])
+        return gamma_rates(alpha, alpha, k, spec.gamma_mode), np.full(k, 1.0 / k)
+    if sv[0] == "DiscreteCustom":
+        return (np.array([r for r, _ in sv[1]], np.float64),
+                np.array([p for _, p in sv[1]], np.float64))
+    raise Error(f"unsupported site variation {sv[0]}")
+
+
 def create(spec: Spec) -> Model:
     """MlModel.create (lib/mlModel.ml:664-746) for JC69 / F81 / GTR."""
     n = spec.n_states
-    sv = spec.site_variation
-    if sv[0] == "Constant":
-        rates, probs = np.array([1.0]), np.array([1.0])
-    elif sv[0] == "DiscreteGamma":
-        k, alpha = int(sv[1]), float(sv[2])
-        probs = np.full(k, 1.0 / k)
-        rates = gamma_rates(alpha, alpha, k, spec.gamma_mode)
-    elif sv[0] == "DiscreteCustom":
-        rates = np.array([r for r, _ in sv[1]], np.float64)
-        probs = np.array([p for _, p in sv[1]], np.float64)
-    else:
-        raise Error(f"unsupported site variation {sv[0]}")
+    rates, probs = _site_rates(spec)
     priors = _priors(spec)
     sub = spec.substitution
     if sub[0] == "JC69":
@@ -154,5 +157,84 @@ def create(spec: Spec) -> Model:
     return Model(spec=spec, priors=priors, rates=rates, probs=probs, q=q, u=u, d=d, ui=ui)
 
 
+def _same(a, b) -> bool:
+    if isinstance(a, (tuple, list)) and isinstance(b, (tuple, list)):
+        return len(a) == len(b) and all(_same(x, y) for x, y in zip(a, b))
+    if isinstance(a, np.ndarray) or isinstance(b, np.ndarray):
+        return np.array_equal(np.asarray(a), np.asarray(b))
+    return a == b
+
+
+def update(model: Model, spec: Spec) -> Model:
+    """MlModel.update (lib/mlModel.ml:196, TODO there): the model of spec with
+    minimal work -- when only the site-rate variation changed, the rate matrix
+    and its eigen-decomposition are kept and only the rate categories are
+    recomputed; otherwise the model is created anew."""
+    if (spec.n_states == model.spec.n_states and _same(spec.substitution, model.spec.substitution)
+            and _same(spec.base_priors, model.spec.base_priors)):
+        rates, probs = _site_rates(spec)
+        return Model(spec=spec, priors=model.priors, rates=rates, probs=probs, q=model.q,
+                     u=model.u, d=model.d, ui=model.ui)
+    return create(spec)
+
+
+def _with(spec: Spec, **kw) -> Spec:
+    d = dict(spec.__dict__)
+    d.update(kw)
+    return Spec(**d)
+
+
+def gen_subst_opt_func(m: Model):
+    """MlModel.gen_subst_opt_func (lib/mlModel.ml:821-822, TODO there): the
+    GTR exchangeabilities (the last one fixed at 1) as a log-scale vector, and
+    the function from such a vector to the model.  Models without free
+    substitution parameters give an empty vector."""
+    sub = m.spec.substitution
+    if sub[0] != "GTR":
+        return np.zeros(0), lambda v: m
+    n = m.spec.n_states
+    co = np.asarray(sub[1], np.float64)
+    if co.size == 0:
+        co = np.ones((n + 1) * (n - 2) // 2)
+    return np.log(co), lambda v: update(m, _with(m.spec, substitution=("GTR", np.exp(v))))
+
+
+def gen_rates_opt_func(m: Model):
+    """MlModel.gen_rates_opt_func (lib/mlModel.ml:824-825, TODO there): the
+    Gamma shape as [log alpha]; other site-rate models give an empty vector."""
+    sv = m.spec.site_variation
+    if sv[0] != "DiscreteGamma":
+        return np.zeros(0), lambda v: m
+    k = int(sv[1])
+    return (np.array([math.log(float(sv[2]))]),
+            lambda v: update(m, _with(m.spec, site_variation=("DiscreteGamma", k,
+                                                                 float(np.exp(v[0]))))))
+
+
+def gen_prior_opt_func(m: Model):
+    """MlModel.gen_prior_opt_func (lib/mlModel.ml:827-828, TODO there): the
+    empirical base frequencies as n - 1 log ratios log(pi_i / pi_n); equal
+    priors give an empty vector."""
+    if m.spec.base_priors[0] != "Empirical":
+        return np.zeros(0), lambda v: m
+    pi = m.priors
+
+    def make(v):
+        x = np.append(np.asarray(v, np.float64), 0.0)
+        e = np.exp(x - x.max())
+        return update(m, _with(m.spec, base_priors=("Empirical", e / e.sum())))
+
+    return np.log(pi[:-1] / pi[-1]), make
+
+
+def num_parameters(m: Model) -> int:
+    """MlModel.num_parameters (lib/mlModel.ml:193, TODO there): free model
+    parameters (substitution, site rates, priors), as used by the
+    information criteria; branch lengths are not counted."""
+    return sum(g(m)[0].size for g in (gen_subst_opt_func, gen_rates_opt_func, gen_prior_opt_func))
+
+
 __all__ = ["Spec", "Model", "Error", "create", "compose", "gamma_rates", "gtr_q",
-           "diagonalize_gtr", "substitution_matrix", "PHY_GAMMA_REF", "PHY_GAMMA_MEAN"]
+           "diagonalize_gtr", "substitution_matrix", "update", "gen_subst_opt_func",
+           "gen_rates_opt_func", "gen_prior_opt_func", "num_parameters", "PHY_GAMMA_REF",
+           "PHY_GAMMA_MEAN"]
