@@ -273,3 +273,18 @@ def test_pack_dna_matches_numpy(rng):
     st[1, 3] = 255
     sets = np.where(st < 4, 1 << np.minimum(st, 3), 15).astype(np.uint8)
     assert np.array_equal(synth.pack_dna(st), D.pack_nibbles(sets))
+
+
+def test_ptopology_store_is_functional():
+    from phylocaml_amd.diagnosis import Ptopology, RootData
+
+    t0 = Ptopology()
+    t1 = t0.add_node_data(3, "a").add_root_data((1, 2), "r")
+    t2 = t1.add_comp_root(0, RootData({}, 5.0, 4.0)).add_comp_root_i(0, 7, ((1, 2), "r"))
+    assert t0.node_data == {} and t1.get_node_data(3) == "a"
+    assert t2.get_comp_root_i(0, 7) == ((1, 2), "r")
+    assert t2.get_comp_root(0).component_cost == 5.0
+    t3 = t2.remove_comp_root_i(0, 7).remove_node_data(3).remove_root_data((1, 2))
+    assert t3.get_comp_root(0).assignment == {} and 7 in t2.get_comp_root(0).assignment
+    assert t3.node_data == {} and t3.root_data == {} and t2.get_node_data(3) == "a"
+    assert t3.remove_comp_root(0).comp_root == {}
