@@ -212,6 +212,24 @@ int phy_fitch_downpass(phy_ctx* ctx, const phy_topo* topo, uint64_t* d_sets,
 int phy_fitch_uppass(phy_ctx* ctx, const phy_topo* topo, const uint64_t* d_sets,
                      uint64_t* d_final, int64_t stride, int64_t n_sites);
 
+/* Whole-tree Sankoff (1975) parsimony under an integer transformation cost
+ * matrix (the PTcm.X spec of lib/pTcm.ml:5-50, costs as in the median search
+ * of lib/costMatrix.ml:52-65: a child in state j under a parent in state k
+ * costs tcm[j][k]; no Sankoff NodeData exists in the reference), in one
+ * launch: C_leaf[k] = 0 for k in the leaf's set, else infinite;
+ * C_v[k] = sum over the children c of min_j (C_c[j] + tcm[j][k]);
+ * site cost = min_k C_root[k].
+ *   n_states     K, 2..8;  d_tcm K x K int32 (device), entries >= 0 with
+ *                (2N - 2) * max(tcm) < 2^31 (the int32 sums cannot overflow)
+ *   d_leaves     N rows x stride bytes: the state set of leaf i at site s is
+ *                bit k of d_leaves[i * stride + s]; bits >= K are ignored
+ *                and an empty set means every state (missing data)
+ *   d_site_cost  n_sites uint32;  d_total 1 uint64: sum_s w_s site_cost_s */
+int phy_sankoff_downpass(phy_ctx* ctx, const phy_topo* topo, int32_t n_states,
+                         const int32_t* d_tcm, const uint8_t* d_leaves, int64_t stride,
+                         int64_t n_sites, const uint32_t* d_weights, uint32_t* d_site_cost,
+                         uint64_t* d_total);
+
 /* ------------------------------------------------------------------------- */
 /* MlModel native stubs (host, FP64, synchronous; no LAPACK).                */
 /* ------------------------------------------------------------------------- */

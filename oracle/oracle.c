@@ -189,6 +189,60 @@ void orc_fitch_uppass(int32_t N, const int32_t* ch, int32_t root, const uint8_t*
   free(post);
 }
 
+/* Sankoff (1975) over an integer cost matrix, in the cost convention of the
+ * median search of lib/costMatrix.ml:52-65 (a child in state j under a parent
+ * in state k costs tcm[j][k]):  C_leaf[k] = 0 for k in the leaf set (an empty
+ * set means every state), else infinite;  C_v[k] = sum over children of
+ * min_j (C_c[j] + tcm[j][k]);  site cost = min_k C_root[k].  int64 throughout.
+ * Returns sum_s w_s site_cost_s. */
+uint64_t orc_sankoff(int32_t N, const int32_t* ch, int32_t root, int32_t K, const int32_t* tcm,
+                     const uint8_t* leaves, int64_t S, const uint32_t* w, uint32_t* site_cost,
+                     int32_t n_threads) {
+  const int32_t M = 2 * N - 1;
+  const int64_t INF = INT64_MAX / 4;
+  int32_t* post = post_order(N, ch, root);
+  uint64_t total = 0;
+#ifdef _OPENMP
+  if (n_threads < 1) n_threads = 1;
+#pragma omp parallel for num_threads(n_threads) schedule(static) reduction(+ : total)
+#endif
+  for (int64_t s = 0; s < S; ++s) {
+    int64_t* C = (int64_t*)malloc(sizeof(int64_t) * (size_t)M * K);
+    for (int32_t q = 0; q < M; ++q) {
+      const int32_t v = post[q];
+      int64_t* cv = C + (int64_t)v * K;
+      if (v < N) {
+        uint8_t code = leaves[(int64_t)v * S + s] & (uint8_t)((1u << K) - 1u);
+        if (code == 0) code = (uint8_t)((1u << K) - 1u);
+        for (int32_t k = 0; k < K; ++k) cv[k] = ((code >> k) & 1) ? 0 : INF;
+        continue;
+      }
+      for (int32_t k = 0; k < K; ++k) cv[k] = 0;
+      for (int c = 0; c < 2; ++c) {
+        const int64_t* cc = C + (int64_t)ch[2 * (v - N) + c] * K;
+        for (int32_t k = 0; k < K; ++k) {
+          int64_t best = INF;
+          for (int32_t j = 0; j < K; ++j) {
+            if (cc[j] >= INF) continue;
+            const int64_t x = cc[j] + tcm[j * K + k];
+            if (x < best) best = x;
+          }
+          cv[k] += best;
+        }
+      }
+    }
+    const int64_t* cr = C + (int64_t)root * K;
+    int64_t best = cr[0];
+    for (int32_t k = 1; k < K; ++k)
+      if (cr[k] < best) best = cr[k];
+    if (site_cost) site_cost[s] = (uint32_t)best;
+    total += (uint64_t)best * (w ? w[s] : 1u);
+    free(C);
+  }
+  free(post);
+  return total;
+}
+
 int64_t orc_fitch_brute(int32_t N, const int32_t* ch, int32_t root, const uint8_t* leaves,
                         int32_t K, uint8_t* mpr) {
   const int32_t nint = N - 1;

@@ -15,6 +15,9 @@ Parity status per function (the test that pins it):
                              invariants, scipy goldens)
   lnl                        tests/test_oracle_lnl.py (brute force, 2-taxon
                              closed form, finite differences)
+  sankoff                    tests/test_oracle_sankoff.py (brute force over all
+                             assignments, unit costs = Fitch, cherry = the
+                             reference median search)
 """
 from __future__ import annotations
 
@@ -56,6 +59,9 @@ _lib.orc_fitch_downpass.argtypes = [C.c_int32, _i32, C.c_int32, _u8, C.c_int64, 
                                     _u64, C.c_int32]
 _lib.orc_fitch_uppass.restype = None
 _lib.orc_fitch_uppass.argtypes = [C.c_int32, _i32, C.c_int32, _u8, _u8, C.c_int64, C.c_int32]
+_lib.orc_sankoff.restype = C.c_uint64
+_lib.orc_sankoff.argtypes = [C.c_int32, _i32, C.c_int32, C.c_int32, _i32, _u8, C.c_int64, _u32,
+                             _u32, C.c_int32]
 _lib.orc_fitch_brute.restype = C.c_int64
 _lib.orc_fitch_brute.argtypes = [C.c_int32, _i32, C.c_int32, _u8, C.c_int32, _u8]
 _lib.orc_gtr_q.restype = None
@@ -122,6 +128,55 @@ def fitch_downpass(n_leaves, children, root, leaves, weights=None, threads=1):
     tot = _lib.orc_fitch_downpass(N, _p(ch, _i32), int(root), _p(sets, _u8), S, _p(w, _u32),
                                   _p(sc, _u32), _p(nc, _u64), int(threads))
     return sets, sc, nc, int(tot)
+
+
+def sankoff(n_leaves, children, root, tcm, leaves, weights=None, threads=1):
+    """leaves: [N, S] uint8 state-set bitmasks; tcm K x K ints.  Returns
+    (site_cost, total)."""
+    N = int(n_leaves)
+    tcm = _c(tcm, np.int32)
+    K = tcm.shape[0]
+    leaves = _c(leaves, np.uint8)
+    S = leaves.shape[1]
+    ch = _c(children, np.int32)
+    sc = np.zeros(S, dtype=np.uint32)
+    w = _c(weights, np.uint32)
+    tot = _lib.orc_sankoff(N, _p(ch, _i32), int(root), K, _p(tcm, _i32), _p(leaves, _u8), S,
+                           _p(w, _u32), _p(sc, _u32), int(threads))
+    return sc, int(tot)
+
+
+def sankoff_brute(n_leaves, children, root, tcm, leaves):
+    """Per-site minimum over every assignment of states to all nodes (leaf
+    states within their sets) of sum over edges tcm[child][parent]; tiny
+    trees only."""
+    import itertools
+
+    N = int(n_leaves)
+    tcm = np.asarray(tcm, np.int64)
+    K = tcm.shape[0]
+    leaves = np.asarray(leaves, np.uint8)
+    ch = np.asarray(children)
+    M = 2 * N - 1
+    out = np.zeros(leaves.shape[1], np.int64)
+    for s in range(leaves.shape[1]):
+        opts = []
+        for v in range(M):
+            if v < N:
+                code = int(leaves[v, s]) & ((1 << K) - 1)
+                code = code or (1 << K) - 1
+                opts.append([k for k in range(K) if (code >> k) & 1])
+            else:
+                opts.append(list(range(K)))
+        best = None
+        for a in itertools.product(*opts):
+            c = 0
+            for v in range(N, M):
+                for x in (ch[2 * (v - N)], ch[2 * (v - N) + 1]):
+                    c += tcm[a[x], a[v]]
+            best = c if best is None or c < best else best
+        out[s] = best
+    return out
 
 
 def fitch_uppass(n_leaves, children, root, sets):

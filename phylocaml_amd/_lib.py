@@ -66,6 +66,7 @@ SIGNATURES = {
     "phy_fitch_adjust3": (C.c_int, [vp, vp, vp, vp, vp, vp, i64, vp]),
     "phy_fitch_downpass": (C.c_int, [vp, vp, vp, i64, i64, vp, vp, vp, vp]),
     "phy_fitch_uppass": (C.c_int, [vp, vp, vp, vp, i64, i64]),
+    "phy_sankoff_downpass": (C.c_int, [vp, vp, i32, vp, vp, i64, i64, vp, vp, vp]),
     "phy_fitch_insertion_costs": (C.c_int, [vp, vp, vp, i64, i64, vp, i64, vp, vp]),
     "phy_diagonalize_gtr": (C.c_int, [i32, P_f64, P_f64, P_f64, P_f64]),
     "phy_diagonalize_sym": (C.c_int, [i32, P_f64, P_f64]),

@@ -227,6 +227,19 @@ def gen_prior_opt_func(m: Model):
     return np.log(pi[:-1] / pi[-1]), make
 
 
+def integerized_model(model: Model, t: float, sigma: int = 4) -> np.ndarray:
+    """MlModel.integerized_model (lib/mlModel.ml:639-660): the integer cost
+    matrix of P(t) scaled by 10^sigma, -int(10^sigma log P_ij) for JC69 and
+    -int(10^sigma log(pi_i P_ij)) otherwise (OCaml's int_of_float truncates
+    toward zero)."""
+    s = 10.0 ** int(sigma)
+    p = compose(model, t)
+    with np.errstate(divide="ignore"):
+        x = np.log(p) if model.spec.substitution[0] == "JC69" else \
+            np.log(model.priors[:, None] * p)
+    return -np.trunc(s * x).astype(np.int64)
+
+
 def num_parameters(m: Model) -> int:
     """MlModel.num_parameters (lib/mlModel.ml:193, TODO there): free model
     parameters (substitution, site rates, priors), as used by the
@@ -235,6 +248,6 @@ def num_parameters(m: Model) -> int:
 
 
 __all__ = ["Spec", "Model", "Error", "create", "compose", "gamma_rates", "gtr_q",
-           "diagonalize_gtr", "substitution_matrix", "update", "gen_subst_opt_func",
+           "diagonalize_gtr", "substitution_matrix", "integerized_model", "update", "gen_subst_opt_func",
            "gen_rates_opt_func", "gen_prior_opt_func", "num_parameters", "PHY_GAMMA_REF",
            "PHY_GAMMA_MEAN"]
