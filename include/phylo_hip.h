@@ -370,6 +370,33 @@ int phy_spr_moves(int32_t n_leaves, const int32_t* children, int32_t root,
 int phy_spr_apply(int32_t n_leaves, const int32_t* children, int32_t root,
                   const double* blen, int32_t s, int32_t v, int32_t* children_out,
                   int32_t* root_out, double* blen_out);
+/* TBR and NNI (TreeNeighborhoods.generate_tbr / generate_nni, lib/
+ * treeNeighborhoods.ml:27-75, never compiled: NNI is TBR with join distance
+ * 1 and no reroot, SPR is TBR with no reroot).  A TBR move (s, w, v) reroots
+ * the subtree of s on the edge above w (a node of that subtree below the
+ * children of s; w = s keeps its root), then regrafts it as the SPR move
+ * (s, v).  Rerooting puts the subtree's root at the middle of the edge above
+ * w, reverses the path from s with its lengths and suppresses the former
+ * root (its two edges add).  v = -1 keeps the subtree where it is (only
+ * the reroot acts).  Join distance: edges between the edge the prune point
+ * lies on and the edge above v in the unrooted pruned tree (the root's two
+ * edges count as one; v = -1 is distance 0, other targets at distance 0
+ * give the same tree and are left out).  Reroot distance: edges between the
+ * subtree root's two edges and the edge above w (w = s: 0).  The moves are
+ * every (s, w, v) within max_join / max_reroot (< 0: unlimited) except the
+ * identity (s, s, -1).  The unrooted SPR neighbourhood is the union of the
+ * moves with reroot distance 0 and those with join distance 0 (the two
+ * sides of each cut edge). */
+int phy_tbr_moves(int32_t n_leaves, const int32_t* children, int32_t root, int32_t max_join,
+                  int32_t max_reroot, int64_t* n_moves, int32_t* moves, int64_t capacity);
+int phy_tbr_apply(int32_t n_leaves, const int32_t* children, int32_t root,
+                  const double* blen, int32_t s, int32_t w, int32_t v, int32_t* children_out,
+                  int32_t* root_out, double* blen_out);
+/* The 2(N - 3) NNI neighbours, one SPR move (s, v) each: for each internal
+ * unrooted edge, the swaps of either child of its lower end with the
+ * subtree across it. */
+int phy_nni_moves(int32_t n_leaves, const int32_t* children, int32_t root, int64_t* n_moves,
+                  int32_t* moves, int64_t capacity);
 /* Exact lnL of every SPR neighbour, many candidate trees per fused launch
  * (chunk candidates at a time; form PHY_F32 or PHY_F64, fused implied):
  * d_lnl[i] = lnL of the tree after moves[i] (branch lengths as above, not
@@ -378,6 +405,13 @@ int phy_spr_apply(int32_t n_leaves, const int32_t* children, int32_t root,
 int phy_lk_spr_work_size(int32_t n_leaves, const phy_model* model, int64_t n_patterns,
                          int32_t form, int32_t chunk, size_t* bytes);
 int phy_lk_spr_scores(phy_ctx* ctx, const phy_model* model, int32_t form,
+                      int64_t n_patterns, const uint64_t* d_tips, int64_t stride,
+                      const uint32_t* d_weights, int32_t n_leaves, const int32_t* children,
+                      int32_t root, const double* blen, int64_t n_moves,
+                      const int32_t* moves, int32_t chunk, double* d_lnl, void* d_work,
+                      size_t work_bytes);
+/* The same for TBR moves (s, w, v), 3 ints each (workspace as for SPR). */
+int phy_lk_tbr_scores(phy_ctx* ctx, const phy_model* model, int32_t form,
                       int64_t n_patterns, const uint64_t* d_tips, int64_t stride,
                       const uint32_t* d_weights, int32_t n_leaves, const int32_t* children,
                       int32_t root, const double* blen, int64_t n_moves,

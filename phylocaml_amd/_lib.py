@@ -85,8 +85,13 @@ SIGNATURES = {
     "phy_lk_root": (C.c_int, [vp, vp, i32, i64, vp, vp, i64, vp, vp]),
     "phy_spr_moves": (C.c_int, [i32, P_i32, i32, P_i64, P_i32, i64]),
     "phy_spr_apply": (C.c_int, [i32, P_i32, i32, P_f64, i32, i32, P_i32, P_i32, P_f64]),
+    "phy_tbr_moves": (C.c_int, [i32, P_i32, i32, i32, i32, P_i64, P_i32, i64]),
+    "phy_tbr_apply": (C.c_int, [i32, P_i32, i32, P_f64, i32, i32, i32, P_i32, P_i32, P_f64]),
+    "phy_nni_moves": (C.c_int, [i32, P_i32, i32, P_i64, P_i32, i64]),
     "phy_lk_spr_work_size": (C.c_int, [i32, vp, i64, i32, i32, C.POINTER(sz)]),
     "phy_lk_spr_scores": (C.c_int, [vp, vp, i32, i64, vp, i64, vp, i32, P_i32, i32, P_f64, i64,
+                                    P_i32, i32, vp, vp, sz]),
+    "phy_lk_tbr_scores": (C.c_int, [vp, vp, i32, i64, vp, i64, vp, i32, P_i32, i32, P_f64, i64,
                                     P_i32, i32, vp, vp, sz]),
     "phy_mlp_create": (C.c_int, [vp, i32, i32, u64, PP]),
     "phy_mlp_destroy": (None, [vp]),

@@ -1438,19 +1438,19 @@ int phy_lk_spr_work_size(int32_t N, const phy_model* m, int64_t n_pat, int32_t f
   return PHY_OK;
 }
 
-int phy_lk_spr_scores(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat,
-                      const uint64_t* tips, int64_t stride, const uint32_t* weights, int32_t N,
-                      const int32_t* children, int32_t root, const double* blen, int64_t n_moves,
-                      const int32_t* moves, int32_t chunk, double* d_lnl, void* d_work,
-                      size_t work_bytes) {
-  if (!ctx || !m) return PHY_EINVAL;
-  if (m->K != 4) return set_err(ctx, PHY_EINVAL, "spr scores: 4-state models only");
-  if (N < 3 || n_pat < 1 || chunk < 1 || n_moves < 0 || !children || !blen || !tips || !weights ||
-      !d_lnl || !d_work || (n_moves > 0 && !moves))
-    return set_err(ctx, PHY_EINVAL, "spr scores: bad argument");
-  if (stride < phy_words(n_pat)) return set_err(ctx, PHY_EINVAL, "spr scores: tip stride < words");
+}  // extern "C"
+
+namespace {
+
+// Score n_moves candidate trees of the base tree; apply(i, ch, &root, bl)
+// builds candidate i (PHY_OK or an error).
+template <class Apply>
+int score_candidates(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat,
+                     const uint64_t* tips, int64_t stride, const uint32_t* weights, int32_t N,
+                     int64_t n_moves, int32_t chunk, double* d_lnl, void* d_work,
+                     size_t work_bytes, const char* what, Apply apply) {
   const SprPlan pl = spr_plan(N, m, n_pat, form, chunk);
-  if (work_bytes < pl.total) return set_err(ctx, PHY_EINVAL, "spr scores: workspace too small");
+  if (work_bytes < pl.total) return set_err(ctx, PHY_EINVAL, std::string(what) + ": workspace too small");
   const int M = 2 * N - 1;
   char* work = static_cast<char*>(d_work);
   std::vector<int32_t> prog, fifo;
@@ -1469,8 +1469,7 @@ int phy_lk_spr_scores(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_
         std::vector<int32_t> ch((size_t)2 * (N - 1));
         for (int c = (int)k; c < C; c += (int)nt) {
           int32_t r = -1;
-          const int32_t s = moves[2 * (base + c)], v = moves[2 * (base + c) + 1];
-          if (phy_spr_apply(N, children, root, blen, s, v, ch.data(), &r, &bl[(size_t)c * M]) != PHY_OK) {
+          if (apply(base + c, ch.data(), &r, &bl[(size_t)c * M]) != PHY_OK) {
             bad[c] = 1;
             continue;
           }
@@ -1488,7 +1487,7 @@ int phy_lk_spr_scores(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_
       });
     for (auto& x : th) x.join();
     for (int c = 0; c < C; ++c)
-      if (bad[c]) return set_err(ctx, PHY_EINVAL, "spr scores: invalid move " + std::to_string(base + c));
+      if (bad[c]) return set_err(ctx, PHY_EINVAL, std::string(what) + ": invalid move " + std::to_string(base + c));
     const int dmax = *std::max_element(depth.begin(), depth.end());
     PHY_HIP(ctx, hipMemcpyAsync(work + pl.off_prog, prog.data(), prog.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     PHY_HIP(ctx, hipMemcpyAsync(work + pl.off_fifo, fifo.data(), fifo.size() * 4, hipMemcpyHostToDevice, ctx->stream));
@@ -1499,13 +1498,62 @@ int phy_lk_spr_scores(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_
       case 4: rc = spr_launch<double, 4>(ctx, m, N, n_pat, tips, stride, weights, C, dmax, work, pl, d_lnl + base); break;
       case 11: rc = spr_launch<float, 1>(ctx, m, N, n_pat, tips, stride, weights, C, dmax, work, pl, d_lnl + base); break;
       case 14: rc = spr_launch<float, 4>(ctx, m, N, n_pat, tips, stride, weights, C, dmax, work, pl, d_lnl + base); break;
-      default: rc = set_err(ctx, PHY_ERANGE, "spr scores: n_rates must be 1 or 4");
+      default: rc = set_err(ctx, PHY_ERANGE, std::string(what) + ": n_rates must be 1 or 4");
     }
     if (rc) return rc;
     // the host buffers are reused by the next chunk: wait for the copies
     PHY_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   return PHY_OK;
+}
+
+int check_scores_args(phy_ctx* ctx, const phy_model* m, int64_t n_pat, const uint64_t* tips,
+                      int64_t stride, const uint32_t* weights, int32_t N, const int32_t* children,
+                      const double* blen, int64_t n_moves, const int32_t* moves, int32_t chunk,
+                      double* d_lnl, void* d_work, const char* what) {
+  if (!ctx || !m) return PHY_EINVAL;
+  if (m->K != 4) return set_err(ctx, PHY_EINVAL, std::string(what) + ": 4-state models only");
+  if (N < 3 || n_pat < 1 || chunk < 1 || n_moves < 0 || !children || !blen || !tips || !weights ||
+      !d_lnl || !d_work || (n_moves > 0 && !moves))
+    return set_err(ctx, PHY_EINVAL, std::string(what) + ": bad argument");
+  if (stride < phy_words(n_pat)) return set_err(ctx, PHY_EINVAL, std::string(what) + ": tip stride < words");
+  return PHY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phy_lk_spr_scores(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat,
+                      const uint64_t* tips, int64_t stride, const uint32_t* weights, int32_t N,
+                      const int32_t* children, int32_t root, const double* blen, int64_t n_moves,
+                      const int32_t* moves, int32_t chunk, double* d_lnl, void* d_work,
+                      size_t work_bytes) {
+  int rc = check_scores_args(ctx, m, n_pat, tips, stride, weights, N, children, blen, n_moves,
+                             moves, chunk, d_lnl, d_work, "spr scores");
+  if (rc) return rc;
+  return score_candidates(ctx, m, form, n_pat, tips, stride, weights, N, n_moves, chunk, d_lnl,
+                          d_work, work_bytes, "spr scores",
+                          [&](int64_t i, int32_t* ch, int32_t* r, double* bl) {
+                            return phy_spr_apply(N, children, root, blen, moves[2 * i],
+                                                 moves[2 * i + 1], ch, r, bl);
+                          });
+}
+
+int phy_lk_tbr_scores(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat,
+                      const uint64_t* tips, int64_t stride, const uint32_t* weights, int32_t N,
+                      const int32_t* children, int32_t root, const double* blen, int64_t n_moves,
+                      const int32_t* moves, int32_t chunk, double* d_lnl, void* d_work,
+                      size_t work_bytes) {
+  int rc = check_scores_args(ctx, m, n_pat, tips, stride, weights, N, children, blen, n_moves,
+                             moves, chunk, d_lnl, d_work, "tbr scores");
+  if (rc) return rc;
+  return score_candidates(ctx, m, form, n_pat, tips, stride, weights, N, n_moves, chunk, d_lnl,
+                          d_work, work_bytes, "tbr scores",
+                          [&](int64_t i, int32_t* ch, int32_t* r, double* bl) {
+                            return phy_tbr_apply(N, children, root, blen, moves[3 * i],
+                                                 moves[3 * i + 1], moves[3 * i + 2], ch, r, bl);
+                          });
 }
 
 }  // extern "C"
