@@ -454,7 +454,7 @@ int phy_mlp_step(phy_ctx* ctx, phy_mlp* mlp, const float* d_x, const float* d_t,
 /*   tips       sum_t N_t x P_t bytes, row-major per tree: state index       */
 /*              0..19, >= 20 = missing (all states)                          */
 /*   weights    sum_t P_t pattern weights                                    */
-/* waves: waves per workgroup (1, 2 or 4); a workgroup owns 16*waves         */
+/* waves: waves per workgroup (1, 2, 4 or 8); a workgroup owns 16*waves     */
 /* patterns of one tree.  The host inputs are copied to the device here.     */
 /* ------------------------------------------------------------------------- */
 typedef struct phy_aa_batch phy_aa_batch;

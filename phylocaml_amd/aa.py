@@ -134,7 +134,7 @@ def tips_bitmask(tips: np.ndarray) -> np.ndarray:
 # Autotuner: waves per workgroup (tile = 16 * waves patterns) per size bucket
 # ---------------------------------------------------------------------------
 BUCKETS = ((2, 64), (64, 128), (128, 1 << 30))  # taxa ranges [lo, hi)
-WAVE_CHOICES = (1, 2, 4)
+WAVE_CHOICES = (1, 2, 4, 8)
 
 
 def bucket_of(n_leaves: int) -> int:

@@ -25,6 +25,7 @@
 // exponents FP32 / int32, the root log-sum FP64.  P(t) = U diag(exp(lambda t))
 // Ui is formed in FP64 from the host eigen-decomposition of each tree's Q.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <numeric>
 
@@ -131,7 +132,7 @@ __device__ __forceinline__ bf16x8 tip_operand(uint32_t code, int q) {
   return r;
 }
 
-template <int WAVES, int R>
+template <int WAVES, int R, int LV>
 __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
   constexpr int NT = 64 * WAVES;
   constexpr int STAGE = 2 * R * 2 * OPB;  // uint4 per node: [c][r][blk][lane]
@@ -141,8 +142,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
   __shared__ int s_item;
   extern __shared__ uint4 dyn[];
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, q = l >> 4, pl = l & 15;
-  // per-wave stack: depth x (R operands of 64 lanes) + exponents
-  uint4* stk = dyn + (size_t)wave * a.depth * (R * 64 + 16);
+  // per-wave stack: the bottom LV levels in registers (selected by the
+  // uniform stack pointer with value selects), the rest in LDS:
+  // (depth - LV) x (R operands of 64 lanes) + exponents
+  const int rows = a.depth > LV ? a.depth - LV : 0;
+  uint4* stk = dyn + (size_t)wave * rows * (R * 64 + 16);
+  uint4 rs[LV > 0 ? LV : 1][R];
+  int rse[LV > 0 ? LV : 1];
   // the 21 tip operands (a state 0..19, or missing = all 20 states) for each
   // quarter of the lanes, built once; a tip child is then one LDS read
   for (int e = tid; e < (KAA + 1) * 4; e += NT) {
@@ -209,13 +215,33 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
       const uint4* cur = stage[buf];
       const int fl = op.w;
       if (fl & DOWN_SPILL) {
-        uint4* s = stk + (size_t)sp * (R * 64 + 16);
+        if (sp < LV) {
+          uint4 u[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          bf16x8 b = to_bf16x8(acc[r]);
-          s[r * 64 + l] = *reinterpret_cast<uint4*>(&b);
+          for (int r = 0; r < R; ++r) {
+            bf16x8 b = to_bf16x8(acc[r]);
+            u[r] = *reinterpret_cast<uint4*>(&b);
+          }
+          static_for<0, LV>([&](auto lv) {
+            const bool hit = sp == lv;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              rs[lv][r].x = hit ? u[r].x : rs[lv][r].x;
+              rs[lv][r].y = hit ? u[r].y : rs[lv][r].y;
+              rs[lv][r].z = hit ? u[r].z : rs[lv][r].z;
+              rs[lv][r].w = hit ? u[r].w : rs[lv][r].w;
+            }
+            rse[lv] = hit ? acc_e : rse[lv];
+          });
+        } else {
+          uint4* s = stk + (size_t)(sp - LV) * (R * 64 + 16);
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            bf16x8 b = to_bf16x8(acc[r]);
+            s[r * 64 + l] = *reinterpret_cast<uint4*>(&b);
+          }
+          if (q == 0) reinterpret_cast<int*>(s + R * 64)[pl] = acc_e;
         }
-        if (q == 0) reinterpret_cast<int*>(s + R * 64)[pl] = acc_e;
         ++sp;
       }
       float xa[R][8];
@@ -236,13 +262,33 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
           e = acc_e;
         } else {
           --sp;
-          const uint4* s = stk + (size_t)sp * (R * 64 + 16);
+          if (sp < LV) {
+            uint4 u[R];
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const uint4 u = s[r * 64 + l];
-            bop[r] = *reinterpret_cast<const bf16x8*>(&u);
+            for (int r = 0; r < R; ++r) u[r] = rs[0][r];
+            e = rse[0];
+            static_for<1, LV>([&](auto lv) {
+              const bool hit = sp == lv;
+#pragma unroll
+              for (int r = 0; r < R; ++r) {
+                u[r].x = hit ? rs[lv][r].x : u[r].x;
+                u[r].y = hit ? rs[lv][r].y : u[r].y;
+                u[r].z = hit ? rs[lv][r].z : u[r].z;
+                u[r].w = hit ? rs[lv][r].w : u[r].w;
+              }
+              e = hit ? rse[lv] : e;
+            });
+#pragma unroll
+            for (int r = 0; r < R; ++r) bop[r] = *reinterpret_cast<const bf16x8*>(&u[r]);
+          } else {
+            const uint4* s = stk + (size_t)(sp - LV) * (R * 64 + 16);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              const uint4 u = s[r * 64 + l];
+              bop[r] = *reinterpret_cast<const bf16x8*>(&u);
+            }
+            e = reinterpret_cast<const int*>(s + R * 64)[pl];
           }
-          e = reinterpret_cast<const int*>(s + R * 64)[pl];
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -387,11 +433,20 @@ int upload(phy_aa_batch* b, T** dst, const std::vector<T>& src) {
   return PHY_OK;
 }
 
-size_t lds_bytes(int waves, int R, int depth) {
-  return (size_t)waves * depth * (R * 64 + 16) * sizeof(uint4);
+size_t lds_bytes(int waves, int R, int depth, int lv) {
+  return (size_t)waves * (depth > lv ? depth - lv : 0) * (R * 64 + 16) * sizeof(uint4);
 }
 
-template <int WAVES, int R>
+// stack levels kept in registers: 3 for stacks of 3 or more levels (frees
+// LDS for a second workgroup per CU), else none (fewer VGPRs).
+// PHY_AA_REG=0|3 overrides.
+int aa_register_levels(int depth) {
+  const char* s = getenv("PHY_AA_REG");
+  if (s && (s[0] == '0' || s[0] == '3') && s[1] == 0) return s[0] - '0';
+  return depth >= 3 ? 3 : 0;
+}
+
+template <int WAVES, int R, int LV>
 int launch(phy_ctx* ctx, phy_aa_batch* b) {
   AADev a;
   a.prog = b->prog;
@@ -414,15 +469,15 @@ int launch(phy_ctx* ctx, phy_aa_batch* b) {
   a.part_off = b->part_off;
   a.counter = b->counter;
   a.depth = std::max(1, b->depth);
-  const size_t lds = lds_bytes(WAVES, R, a.depth);
+  const size_t lds = lds_bytes(WAVES, R, a.depth, LV);
   int dev = 0, n_cu = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_aa_prune<WAVES, R>, 64 * WAVES, lds) != hipSuccess || per_cu < 1)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_aa_prune<WAVES, R, LV>, 64 * WAVES, lds) != hipSuccess || per_cu < 1)
     per_cu = 1;
   const int64_t grid = std::min<int64_t>((int64_t)n_cu * per_cu, b->n_items);
-  hipLaunchKernelGGL((k_aa_prune<WAVES, R>), dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(64 * WAVES), lds,
+  hipLaunchKernelGGL((k_aa_prune<WAVES, R, LV>), dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(64 * WAVES), lds,
                      ctx->stream, a);
   PHY_LAUNCHED(ctx);
   return PHY_OK;
@@ -430,11 +485,13 @@ int launch(phy_ctx* ctx, phy_aa_batch* b) {
 
 template <int R>
 int launch_w(phy_ctx* ctx, phy_aa_batch* b) {
+  const bool reg = aa_register_levels(std::max(1, b->depth)) == 3;
   switch (b->waves) {
-    case 1: return launch<1, R>(ctx, b);
-    case 2: return launch<2, R>(ctx, b);
-    case 4: return launch<4, R>(ctx, b);
-    default: return set_err(ctx, PHY_ERANGE, "aa_batch: waves per workgroup must be 1, 2 or 4");
+    case 1: return reg ? launch<1, R, 3>(ctx, b) : launch<1, R, 0>(ctx, b);
+    case 2: return reg ? launch<2, R, 3>(ctx, b) : launch<2, R, 0>(ctx, b);
+    case 4: return reg ? launch<4, R, 3>(ctx, b) : launch<4, R, 0>(ctx, b);
+    case 8: return reg ? launch<8, R, 3>(ctx, b) : launch<8, R, 0>(ctx, b);
+    default: return set_err(ctx, PHY_ERANGE, "aa_batch: waves per workgroup must be 1, 2, 4 or 8");
   }
 }
 
@@ -453,7 +510,8 @@ int phy_aa_batch_create(phy_ctx* ctx, int32_t T, const int32_t* n_leaves, const 
       !probs || !tips || !weights)
     return set_err(ctx, PHY_EINVAL, "aa_batch: null argument");
   if (!(R == 1 || R == 2 || R == 4)) return set_err(ctx, PHY_ERANGE, "aa_batch: n_rates must be 1, 2 or 4");
-  if (!(waves == 1 || waves == 2 || waves == 4)) return set_err(ctx, PHY_ERANGE, "aa_batch: waves must be 1, 2 or 4");
+  if (!(waves == 1 || waves == 2 || waves == 4 || waves == 8))
+    return set_err(ctx, PHY_ERANGE, "aa_batch: waves must be 1, 2, 4 or 8");
   phy_aa_batch* b = new phy_aa_batch();
   b->ctx = ctx;
   b->T = T;
@@ -547,7 +605,8 @@ int phy_aa_batch_create(phy_ctx* ctx, int32_t T, const int32_t* n_leaves, const 
   int lds_max = 64 * 1024, dev = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  if (lds_bytes(waves, R, std::max(1, b->depth)) + (size_t)2 * 2 * R * 2 * OPB * sizeof(uint4) > (size_t)lds_max) {
+  const int lv = aa_register_levels(std::max(1, b->depth));
+  if (lds_bytes(waves, R, std::max(1, b->depth), lv) + (size_t)2 * 2 * R * 2 * OPB * sizeof(uint4) > (size_t)lds_max) {
     delete b;
     return set_err(ctx, PHY_ERANGE, "aa_batch: LDS stack too deep for this many waves per workgroup");
   }

@@ -273,15 +273,6 @@ __device__ __forceinline__ void rescale4(real* v, int* e) {
   }
 }
 
-// compile-time loop: f(integral_constant<int, I>) for I in [B, E)
-template <int B, int E, class F>
-__device__ __forceinline__ void static_for(F&& f) {
-  if constexpr (B < E) {
-    f(std::integral_constant<int, B>{});
-    static_for<B + 1, E>(f);
-  }
-}
-
 // State of one wave of the post-order pass: accumulators of its G rate
 // classes, stack, tip stream.  A block is R / G waves over the same 64
 // patterns; wave w evaluates rate classes w*G .. w*G+G-1, each with its own

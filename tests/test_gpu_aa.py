@@ -34,7 +34,7 @@ def _oracle(t):
     return v
 
 
-@pytest.mark.parametrize("waves", [1, 2, 4])
+@pytest.mark.parametrize("waves", [1, 2, 4, 8])
 def test_batch_matches_oracle(ctx, waves):
     from phylocaml_amd import aa
 
@@ -61,6 +61,28 @@ def test_batch_matches_oracle(ctx, waves):
     b.run()
     ctx.sync()
     assert np.array_equal(b.lnl.cpu().numpy(), got)
+
+
+@pytest.mark.parametrize("waves", [4, 8])
+def test_register_stack_levels_give_the_same_bits(ctx, monkeypatch, waves):
+    # the bottom stack levels held in registers carry the same BF16 operands
+    # as the LDS rows: identical results with and without them
+    from phylocaml_amd import aa
+
+    rng = np.random.default_rng(11)
+    trees = [aa.synth_tree(2000 + k, int(rng.integers(40, 140)), int(rng.integers(100, 700)),
+                           n_sim=200) for k in range(6)]
+    out = {}
+    for reg in ("0", "3"):
+        monkeypatch.setenv("PHY_AA_REG", reg)
+        print(f"aa register levels {reg} waves={waves}", flush=True)
+        b = aa.AABatch(ctx, trees, waves=waves)
+        b.run()
+        ctx.sync()
+        out[reg] = b.lnl.cpu().numpy()
+    assert np.array_equal(out["0"], out["3"])
+    ov = _oracle(trees[0])
+    assert abs(out["3"][0] - ov) <= 2e-3 * abs(ov)
 
 
 def test_tiny_exact_cases(ctx):
