@@ -127,8 +127,20 @@ __device__ __forceinline__ void vecmat4(const real* __restrict__ P, const real* 
 
 template <typename real>
 __device__ __forceinline__ void tipvec(uint32_t code, real* x) {
+  if constexpr (sizeof(real) == 4) {
+    // bit j of the 4-bit code to byte j (the four shifted copies of the
+    // multiply do not overlap, so no carries), then one byte conversion each
+    uint32_t spread = ((code & 15u) * 0x00204081u) & 0x01010101u;
+    // hide the known-zero bits, so that each byte converts in one instruction
+    asm("" : "+v"(spread));
+    x[0] = (real)(spread & 255u);
+    x[1] = (real)((spread >> 8) & 255u);
+    x[2] = (real)((spread >> 16) & 255u);
+    x[3] = (real)(spread >> 24);
+  } else {
 #pragma unroll
-  for (int j = 0; j < 4; ++j) x[j] = (real)((code >> j) & 1u);
+    for (int j = 0; j < 4; ++j) x[j] = (real)((code >> j) & 1u);
+  }
 }
 
 __device__ __forceinline__ uint32_t tip_code(const uint64_t* __restrict__ tips, int64_t stride,
