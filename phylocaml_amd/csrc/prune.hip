@@ -858,57 +858,60 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
 }
 
 // Neumaier sum of n doubles with one block (deterministic order).
-__global__ void k_reduce_sum(const double* __restrict__ x, int64_t n, int64_t stride_x,
-                             double* __restrict__ out) {
-  __shared__ double ss[256], cc[256];
+// Compensated sums as (s, c) pairs: Neumaier within a thread, then pairs
+// combined by an error-free two-sum in a fixed butterfly over the wave and a
+// fixed order over the waves (deterministic, launch-shape independent).
+struct CSum {
   double s = 0.0, c = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-    const double v = x[i * stride_x];
+  __device__ __forceinline__ void add(double v) {
     const double t = s + v;
     if (fabs(s) >= fabs(v)) c += (s - t) + v;
     else c += (v - t) + s;
     s = t;
   }
-  ss[threadIdx.x] = s;
-  cc[threadIdx.x] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double S = 0.0, Cc = 0.0;
-    for (int k = 0; k < (int)blockDim.x; ++k) {
-      const double v = ss[k];
-      const double t = S + v;
-      if (fabs(S) >= fabs(v)) Cc += (S - t) + v;
-      else Cc += (v - t) + S;
-      S = t;
-      Cc += cc[k];
-    }
-    *out = S + Cc;
-  }
-}
-
-// one block per row: out[row] = Neumaier sum of x[row * n .. row * n + n)
-__global__ void k_reduce_rows(const double* __restrict__ x, int64_t n, double* __restrict__ out) {
-  __shared__ double ss[256], cc[256];
-  const double* r = x + (int64_t)blockIdx.x * n;
-  double s = 0.0, c = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-    const double v = r[i], t = s + v;
-    if (fabs(s) >= fabs(v)) c += (s - t) + v; else c += (v - t) + s;
+  __device__ __forceinline__ void merge(double s2, double c2) {
+    const double t = s + s2, bb = t - s;
+    c += c2 + ((s - (t - bb)) + (s2 - bb));
     s = t;
   }
-  ss[threadIdx.x] = s;
-  cc[threadIdx.x] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double S = 0.0, C = 0.0;
-    for (int k = 0; k < (int)blockDim.x; ++k) {
-      const double v = ss[k], t = S + v;
-      if (fabs(S) >= fabs(v)) C += (S - t) + v; else C += (v - t) + S;
-      S = t;
-      C += cc[k];
-    }
-    out[blockIdx.x] = S + C;
+};
+
+// block-wide compensated sum (blockDim.x a multiple of 64, at most 1024);
+// thread 0 gets the total
+__device__ __forceinline__ double block_csum(CSum a) {
+  __shared__ double ws[16], wc[16];
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) a.merge(__shfl_xor(a.s, off, 64), __shfl_xor(a.c, off, 64));
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    ws[w] = a.s;
+    wc[w] = a.c;
   }
+  __syncthreads();
+  double out = 0.0;
+  if (threadIdx.x == 0) {
+    CSum t;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t.merge(ws[k], wc[k]);
+    out = t.s + t.c;
+  }
+  return out;
+}
+
+__global__ void k_reduce_sum(const double* __restrict__ x, int64_t n, int64_t stride_x,
+                             double* __restrict__ out) {
+  CSum a;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) a.add(x[i * stride_x]);
+  const double t = block_csum(a);
+  if (threadIdx.x == 0) *out = t;
+}
+
+// one block per row: out[row] = compensated sum of x[row * n .. row * n + n)
+__global__ void k_reduce_rows(const double* __restrict__ x, int64_t n, double* __restrict__ out) {
+  const double* r = x + (int64_t)blockIdx.x * n;
+  CSum a;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) a.add(r[i]);
+  const double t = block_csum(a);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
 // one block per branch: d1[c], d2[c] = sums over waves; root -> 0; the second
@@ -917,42 +920,22 @@ __global__ void k_reduce_derivs(const double* __restrict__ ws, int64_t n_waves, 
                                 int ra, int rb, double* __restrict__ d1, double* __restrict__ d2) {
   // ws[(edge * 2 + q) * n_waves + wave]: q = 0 first, 1 second derivative
   const int c = blockIdx.x;
-  __shared__ double s1[256], c1[256], s2[256], c2[256];
   const int src = (c == rb) ? ra : c;
-  double a = 0, ac = 0, b = 0, bc = 0;
+  CSum a, b;
   if (c != root) {
     const double* w1 = ws + (size_t)src * 2 * n_waves;
     const double* w2 = w1 + n_waves;
     for (int64_t w = threadIdx.x; w < n_waves; w += blockDim.x) {
-      const double v1 = w1[w], v2 = w2[w];
-      double t = a + v1;
-      if (fabs(a) >= fabs(v1)) ac += (a - t) + v1; else ac += (v1 - t) + a;
-      a = t;
-      t = b + v2;
-      if (fabs(b) >= fabs(v2)) bc += (b - t) + v2; else bc += (v2 - t) + b;
-      b = t;
+      a.add(w1[w]);
+      b.add(w2[w]);
     }
   }
-  s1[threadIdx.x] = a;
-  c1[threadIdx.x] = ac;
-  s2[threadIdx.x] = b;
-  c2[threadIdx.x] = bc;
-  __syncthreads();
+  const double ta = block_csum(a);
+  __syncthreads();  // block_csum's shared slots are reused
+  const double tb = block_csum(b);
   if (threadIdx.x == 0) {
-    double A = 0, AC = 0, B = 0, BC = 0;
-    for (int k = 0; k < (int)blockDim.x; ++k) {
-      double v = s1[k], t = A + v;
-      if (fabs(A) >= fabs(v)) AC += (A - t) + v; else AC += (v - t) + A;
-      A = t;
-      AC += c1[k];
-      v = s2[k];
-      t = B + v;
-      if (fabs(B) >= fabs(v)) BC += (B - t) + v; else BC += (v - t) + B;
-      B = t;
-      BC += c2[k];
-    }
-    d1[c] = A + AC;
-    d2[c] = B + BC;
+    d1[c] = ta;
+    d2[c] = tb;
   }
 }
 
