@@ -141,7 +141,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
   __shared__ uint4 tipop[(KAA + 1) * 4];  // B operand of a tip: [code 0..20][quarter]
   __shared__ int s_item;
   extern __shared__ uint4 dyn[];
-  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, q = l >> 4, pl = l & 15;
+  // the wave index is uniform: readfirstlane keeps the operand DMA loop scalar
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63,
+            q = l >> 4, pl = l & 15;
   // per-wave stack: the bottom LV levels in registers (selected by the
   // uniform stack pointer with value selects), the rest in LDS:
   // (depth - LV) x (R operands of 64 lanes) + exponents
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
     const int32_t* fifo = a.fifo + a.fifo_off[t];
     const uint8_t* tips = a.tips + a.tips_off[t];
     const uint4* pops = a.pops + a.pop_off[t];
-    int fpos = 0, sp = 0;
+    int sp = 0;
     float acc[R][8];
     int acc_e = 0;
 #pragma unroll
@@ -182,10 +184,19 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
 #pragma unroll
       for (int v = 0; v < 8; ++v) acc[r][v] = 0.f;
 
+    // Every global load issued while node i computes (the operand DMA of node
+    // i+1, the tip codes of node i+1, fifo id refills) is consumed at node i+1
+    // or later, after the node's closing vmcnt(0) and barrier.  Nothing loaded
+    // from global memory is waited on in the middle of a node, so the DMA
+    // latency overlaps a whole node of MFMA and VALU work.
+    //
     // operands of a node's two children straight into LDS (global_load_lds,
     // 16 B per lane: one 1 KiB block per wave instruction), no registers
     auto dma_stage = [&](const int4 o, int buf) {
-      for (int bk = wave; bk < NBLK; bk += WAVES) {
+#pragma unroll
+      for (int k = 0; k < (NBLK + WAVES - 1) / WAVES; ++k) {
+        const int bk = wave + k * WAVES;  // uniform
+        if (NBLK % WAVES != 0 && bk >= NBLK) break;
         const int c = bk / (R * 2), rem = bk % (R * 2);
         const int child = c ? o.z : o.y;
         const uint4* src = pops + ((size_t)child * R * 2 + rem) * OPB + l;
@@ -194,26 +205,45 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
                                          16, 0, 0);
       }
     };
-    // tip codes of the next two leaves of the stream (the fifo is padded)
-    auto tip_at = [&](int k) -> uint32_t { return tips[(size_t)fifo[k] * pitch + p]; };
-    uint32_t t0 = tip_at(0), t1 = tip_at(1);
-    auto next_tip = [&]() -> uint32_t {
-      const uint32_t v = t0;
-      t0 = t1;
-      ++fpos;
-      t1 = tip_at(fpos + 1);
-      return v;
+    // leaf ids of the stream, 128 at a time in two lane vectors (entries
+    // fbase .. fbase+127; the tree's fifo is padded by 128 ids)
+    int fbase = 0, fpos = 0;
+    int fcur = fifo[l], fnx = fifo[64 + l];
+    auto fid = [&](int pos) -> int {
+      const int k = pos - fbase;  // uniform, 0 <= k < 128
+      return k < 64 ? __builtin_amdgcn_readlane(fcur, k) : __builtin_amdgcn_readlane(fnx, k - 64);
     };
-    dma_stage(prog[0], 0);
+    auto tip_at = [&](int pos) -> uint32_t { return tips[(size_t)fid(pos) * pitch + p]; };
+    // tip codes of a node's leaf children: the first leaf of the stream goes
+    // to child a when a is a leaf, else to child b
+    auto leaf_codes = [&](const int4 o, int pos, uint32_t& ca, uint32_t& cb) {
+      ca = tip_at(pos);
+      cb = tip_at(pos + ((o.w & 3) == SRC_LEAF ? 1 : 0));
+    };
+    uint32_t code_a, code_b;
+    {
+      const int4 o0 = prog[0];
+      leaf_codes(o0, 0, code_a, code_b);
+      dma_stage(o0, 0);
+    }
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the DMA has landed
     __syncthreads();
 
     for_each_entry_ahead(prog, N - 1, [&](int i, const int4 op, const int4 on) {
       const int buf = i & 1;
       const bool more = i + 1 < N - 1;
-      if (more) dma_stage(on, buf ^ 1);  // in flight while node i computes
-      const uint4* cur = stage[buf];
       const int fl = op.w;
+      const int nl = ((fl & 3) == SRC_LEAF ? 1 : 0) + (((fl >> 2) & 3) == SRC_LEAF ? 1 : 0);
+      if (fpos - fbase >= 64) {  // fnx landed at least one node ago
+        fbase += 64;
+        fcur = fnx;
+        fnx = fifo[fbase + 64 + l];
+      }
+      if (more) dma_stage(on, buf ^ 1);  // in flight while node i computes
+      const uint32_t ca = code_a, cb = code_b;
+      leaf_codes(on, fpos + nl, code_a, code_b);  // node i+1's tips, used there
+      fpos += nl;
+      const uint4* cur = stage[buf];
       if (fl & DOWN_SPILL) {
         if (sp < LV) {
           uint4 u[R];
@@ -252,7 +282,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
         bf16x8 bop[R];
         int e = 0;
         if (src == SRC_LEAF) {
-          const uint4 tu = tipop[min(next_tip(), (uint32_t)KAA) * 4 + q];
+          const uint4 tu = tipop[min(c ? cb : ca, (uint32_t)KAA) * 4 + q];
           const bf16x8 tb = *reinterpret_cast<const bf16x8*>(&tu);
 #pragma unroll
           for (int r = 0; r < R; ++r) bop[r] = tb;
@@ -338,7 +368,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_aa_prune(AADev a) {
           for (int v = 0; v < 8; ++v) acc[r][v] = ldexpf(acc[r][v], -ex);
         acc_e += ex;
       }
-      if (more) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): node i+1 operands landed
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): node i+1 operands and tips landed
       __syncthreads();
     });
     // root: L_p = sum_r prob_r sum_i pi_i L[r][i]
@@ -545,7 +575,7 @@ int phy_aa_batch_create(phy_ctx* ctx, int32_t T, const int32_t* n_leaves, const 
                                topo->down_prog[k + 3]));
     fifo_off.push_back((int64_t)fifo.size());
     fifo.insert(fifo.end(), topo->down_fifo.begin(), topo->down_fifo.end());
-    fifo.insert(fifo.end(), 2, 0);  // the kernel reads two ids ahead
+    fifo.insert(fifo.end(), 128, 0);  // the kernel holds ids fbase .. fbase+127, fbase <= N
     phy_topo_destroy(topo);
     nl.push_back(N);
     const int64_t pp = (P + tile - 1) / tile * tile;

@@ -95,3 +95,24 @@ def test_tiny_exact_cases(ctx):
     ctx.sync()
     ov = _oracle(t)
     assert abs(b.lnl.cpu().numpy()[0] - ov) <= 2e-3 * abs(ov)
+
+
+@pytest.mark.parametrize("waves", [1, 4, 8])
+def test_large_trees_cross_leaf_id_chunks(ctx, waves):
+    # the kernel holds the leaf-id stream 128 ids at a time and advances it by
+    # 64: trees of 129-300 leaves cross several such refills
+    from phylocaml_amd import aa
+
+    print(f"aa large trees waves={waves}", flush=True)
+    trees = [aa.synth_tree(3000 + k, N, P, n_sim=150)
+             for k, (N, P) in enumerate([(129, 70), (200, 130), (300, 40), (257, 17)])]
+    trees[1].tips[:, ::5] = 255
+    b = aa.AABatch(ctx, trees, waves=waves)
+    b.run()
+    ctx.sync()
+    got = b.lnl.cpu().numpy()
+    for k, t in enumerate(trees):
+        ov = _oracle(t)
+        rel = abs(got[k] - ov) / abs(ov)
+        print(f"tree {k}: N={t.n_leaves} P={t.n_patterns} rel {rel:.2e}", flush=True)
+        assert rel <= 2e-3, (k, got[k], ov)
