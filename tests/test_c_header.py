@@ -1,0 +1,34 @@
+"""The public headers are plain C: a C99 translation unit including both
+compiles with -Wall -Wextra -Werror -pedantic, and so does the C consumer of
+the ABI (tests/c_abi/abi_demo.c)."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FLAGS = ["-std=c99", "-Wall", "-Wextra", "-Werror", "-pedantic", "-fsyntax-only",
+         "-I", os.path.join(ROOT, "include")]
+
+
+@pytest.fixture
+def gcc():
+    g = shutil.which("gcc")
+    if not g:
+        pytest.skip("no C compiler")
+    return g
+
+
+def test_headers_compile_as_strict_c99(gcc, tmp_path):
+    src = tmp_path / "h.c"
+    src.write_text('#include "phylo_hip.h"\n#include "phylo_synth.h"\n'
+                   "int main(void) { return PHY_OK; }\n")
+    r = subprocess.run([gcc, *FLAGS, str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_c_consumer_compiles(gcc):
+    r = subprocess.run([gcc, *FLAGS, os.path.join(ROOT, "tests", "c_abi", "abi_demo.c")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
