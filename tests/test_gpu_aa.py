@@ -6,7 +6,6 @@ with a relative tolerance of 2e-3 (the measured error is reported by
 bench.py); every tree in a batch of uneven trees must meet it, which also
 checks the persistent work queue and the per-tree bookkeeping."""
 import os
-import sys
 
 import numpy as np
 import pytest

@@ -3,7 +3,6 @@ pattern, all-missing tips (lnL 0), zero weights, very short and very long
 branches, a perfectly balanced 1,024-taxon tree (deepest stack) and a
 2,048-taxon caterpillar, in every pruning form; the same shapes for Fitch
 and Sankoff."""
-import math
 import os
 import sys
 

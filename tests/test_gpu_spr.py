@@ -4,8 +4,6 @@ training) against the FP64 numpy oracle.
 
 BF16 MFMA operands: forward outputs within 2e-2 relative to the output
 scale, gradients within 5e-2 of the gradient norm."""
-import ctypes as C
-import os
 
 import numpy as np
 import pytest

@@ -2,7 +2,6 @@
 exact-ΔlnL pairs of other instances ranks moves so that hill climbing with a
 top-k exact re-score needs far fewer exact evaluations than exhaustive
 scoring, and both stop at a tree with no improving SPR move."""
-import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu

@@ -1,5 +1,4 @@
 """Pins the MLP oracle's gradients by central finite differences."""
-import numpy as np
 
 from oracle import mlp_oracle as M
 
