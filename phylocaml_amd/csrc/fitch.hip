@@ -18,6 +18,9 @@
 // batch ahead through registers into an LDS ring.  Trees of >= 128 leaves
 // run split: parts under cut nodes side by side (grid.y = part), plus a top
 // program, which multiplies the waves in flight by the number of parts.
+#include <algorithm>
+#include <cstdlib>
+
 #include "common.hpp"
 #include "stream.hpp"
 
@@ -245,6 +248,9 @@ struct SetLoad {
 // Pass modes: the whole tree in one program, or a tree split at cut nodes:
 // the parts side by side (grid.y = part; per-site counts of a part go to
 // scratch[part][lane][8V]) and then the top, which adds the parts' counts.
+// A part's count at a site is at most its number of internal nodes, so when
+// every part has at most 255 of them (narrow) the counts are stored as bytes
+// (a quarter of the scratch traffic).
 enum : int { MODE_WHOLE = 0, MODE_PART = 1, MODE_TOP = 2 };
 
 // program k of a split program: meta = prog_off | n_ops | fifo_off, each
@@ -260,7 +266,7 @@ enum : int { MODE_WHOLE = 0, MODE_PART = 1, MODE_TOP = 2 };
 template <int BATCH, bool NODE_COST, int MODE, int V, bool NT>
 __global__ __launch_bounds__(256) void k_fitch_down(
     const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ fifo_ids,
-    const int32_t* __restrict__ meta, int n_parts, uint4* __restrict__ scratch,
+    const int32_t* __restrict__ meta, int n_parts, uint4* __restrict__ scratch, int narrow,
     uint32_t* __restrict__ sets32, int64_t stride, int64_t W, int64_t n_sites, int N,
     const uint32_t* __restrict__ wt, uint32_t* __restrict__ site_cost,
     unsigned long long* __restrict__ node_cost, unsigned long long* __restrict__ total) {
@@ -343,24 +349,44 @@ __global__ __launch_bounds__(256) void k_fitch_down(
     for (int k = 0; k < 8; ++k) site[8 * i + k] += (cnt[i] >> (4 * k)) & 15u;
   if (MODE == MODE_PART) {
     if (act) {
-      uint4* dst = scratch + ((size_t)part * L + w) * 2 * V;
+      if (narrow) {  // 4 counts per word, 8V bytes per lane
+        uint32_t* dst = reinterpret_cast<uint32_t*>(scratch) + ((size_t)part * L + w) * 2 * V;
 #pragma unroll
-      for (int q = 0; q < 2 * V; ++q)
-        dst[q] = make_uint4(site[4 * q], site[4 * q + 1], site[4 * q + 2], site[4 * q + 3]);
+        for (int q = 0; q < 2 * V; ++q)
+          dst[q] = site[4 * q] | site[4 * q + 1] << 8 | site[4 * q + 2] << 16 | site[4 * q + 3] << 24;
+      } else {
+        uint4* dst = scratch + ((size_t)part * L + w) * 2 * V;
+#pragma unroll
+        for (int q = 0; q < 2 * V; ++q)
+          dst[q] = make_uint4(site[4 * q], site[4 * q + 1], site[4 * q + 2], site[4 * q + 3]);
+      }
     }
     return;
   }
   if (MODE == MODE_TOP && act) {
+    if (narrow) {
 #pragma unroll 4
-    for (int pq = 0; pq < n_parts; ++pq) {
-      const uint4* src = scratch + ((size_t)pq * L + w) * 2 * V;
+      for (int pq = 0; pq < n_parts; ++pq) {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(scratch) + ((size_t)pq * L + w) * 2 * V;
 #pragma unroll
-      for (int q = 0; q < 2 * V; ++q) {
-        const uint4 x = src[q];
-        site[4 * q] += x.x;
-        site[4 * q + 1] += x.y;
-        site[4 * q + 2] += x.z;
-        site[4 * q + 3] += x.w;
+        for (int q = 0; q < 2 * V; ++q) {
+          const uint32_t x = src[q];
+#pragma unroll
+          for (int b = 0; b < 4; ++b) site[4 * q + b] += (x >> (8 * b)) & 255u;
+        }
+      }
+    } else {
+#pragma unroll 4
+      for (int pq = 0; pq < n_parts; ++pq) {
+        const uint4* src = scratch + ((size_t)pq * L + w) * 2 * V;
+#pragma unroll
+        for (int q = 0; q < 2 * V; ++q) {
+          const uint4 x = src[q];
+          site[4 * q] += x.x;
+          site[4 * q + 1] += x.y;
+          site[4 * q + 2] += x.z;
+          site[4 * q + 3] += x.w;
+        }
       }
     }
   }
@@ -592,11 +618,11 @@ int downpass_launch(phy_ctx* ctx, const phy_topo* topo, uint32_t* s32, int64_t s
   if (!split) {
     if (node_cost)
       hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_WHOLE, V, NT>), dim3(grid), dim3(bs), lds, ctx->stream,
-                         topo->d_down_prog, N - 1, topo->d_down_fifo, nullptr, 0, nullptr, s32, stride32, W32,
+                         topo->d_down_prog, N - 1, topo->d_down_fifo, nullptr, 0, nullptr, 0, s32, stride32, W32,
                          n_sites, N, wt, site_cost, node_cost, total);
     else
       hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_WHOLE, V, NT>), dim3(grid), dim3(bs), lds, ctx->stream,
-                         topo->d_down_prog, N - 1, topo->d_down_fifo, nullptr, 0, nullptr, s32, stride32, W32,
+                         topo->d_down_prog, N - 1, topo->d_down_fifo, nullptr, 0, nullptr, 0, s32, stride32, W32,
                          n_sites, N, wt, site_cost, node_cost, total);
     PHY_LAUNCHED(ctx);
     return PHY_OK;
@@ -615,21 +641,26 @@ int downpass_launch(phy_ctx* ctx, const phy_topo* topo, uint32_t* s32, int64_t s
     topo->scratch_bytes = need;
   }
   uint4* scr = reinterpret_cast<uint4*>(topo->d_scratch);
+  int max_ops = 0;  // internal nodes of the largest part
+  for (int k = 0; k < P; ++k) max_ops = std::max(max_ops, (int)sp.n_ops[k]);
+  // PHY_FITCH_NARROW=0 keeps 32-bit counts
+  const char* ne = getenv("PHY_FITCH_NARROW");
+  const int narrow = max_ops <= 255 && !(ne && ne[0] == '0');
   if (node_cost) {
     hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_PART, V, NT>), dim3(grid, (unsigned)P), dim3(bs), lds,
-                       ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites,
+                       ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, narrow, s32, stride32, W32, n_sites,
                        N, wt, site_cost, node_cost, total);
     PHY_LAUNCHED(ctx);
     hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_TOP, V, NT>), dim3(grid), dim3(bs), lds, ctx->stream,
-                       sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites, N, wt,
+                       sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, narrow, s32, stride32, W32, n_sites, N, wt,
                        site_cost, node_cost, total);
   } else {
     hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_PART, V, NT>), dim3(grid, (unsigned)P), dim3(bs), lds,
-                       ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites,
+                       ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, narrow, s32, stride32, W32, n_sites,
                        N, wt, site_cost, node_cost, total);
     PHY_LAUNCHED(ctx);
     hipLaunchKernelGGL((k_fitch_down<BATCH, false, MODE_TOP, V, NT>), dim3(grid), dim3(bs), lds, ctx->stream,
-                       sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, s32, stride32, W32, n_sites, N, wt,
+                       sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, narrow, s32, stride32, W32, n_sites, N, wt,
                        site_cost, node_cost, total);
   }
   PHY_LAUNCHED(ctx);

@@ -114,6 +114,21 @@ def test_prefetch_depth_and_split_forms(ctx, rng, monkeypatch, batch, split, wor
     _tree_case(ctx, N, ch, root, leaves, w)
 
 
+@pytest.mark.parametrize("N,narrow", [(1000, "1"), (1000, "0"), (8192, "1")])
+def test_split_part_counts_as_bytes_or_words(ctx, rng, monkeypatch, N, narrow):
+    # the parts hand their per-site counts to the top as bytes when every part
+    # has at most 255 internal nodes (1,000 leaves: parts of <= 103), else as
+    # 32-bit words (8,192 leaves: parts of up to ~1,500); PHY_FITCH_NARROW=0
+    # forces words
+    monkeypatch.setenv("PHY_FITCH_NARROW", narrow)
+    S = 517
+    _say(f"fitch split counts N={N} narrow={narrow}")
+    ch, root = random_tree(rng, N)
+    leaves = rng.integers(1, 16, (N, S)).astype(np.uint8)
+    w = rng.integers(1, 5, S).astype(np.uint32)
+    _tree_case(ctx, N, ch, root, leaves, w)
+
+
 def test_config1_full_size_bit_exact(ctx):
     """1,024-taxon Yule tree, 10^6 IUPAC sites (the parsimony benchmark
     workload): every preliminary and final set, every site cost and the total
