@@ -28,6 +28,45 @@ from . import device as D
 from ._lib import check, lib, ptr
 
 
+def num_edges(n: int) -> int:
+    """Edges of an unrooted binary tree with n leaves: Tree.num_edges
+    (lib/tree.ml:552-559): 0 for n <= 1, else 2n - 3."""
+    if n < 0:
+        raise ValueError("num_edges: negative leaf count")
+    return 0 if n <= 1 else 2 * n - 3
+
+
+def num_nodes(n: int) -> int:
+    """Nodes of an unrooted binary tree with n leaves: Tree.num_nodes
+    (lib/tree.ml:561-568): n for n <= 1, else 2n - 2."""
+    if n < 0:
+        raise ValueError("num_nodes: negative leaf count")
+    return n if n <= 1 else 2 * n - 2
+
+
+def num_unrooted_trees(n: int) -> int:
+    """Distinct unrooted binary topologies on n labelled leaves:
+    Tree.num_unrooted_trees (lib/tree.ml:570-586): 0 for n = 0, 1 for
+    n < 3, else (2n - 5)!!."""
+    if n < 0:
+        raise ValueError("num_unrooted_trees: negative leaf count")
+    if n == 0:
+        return 0
+    out = 1
+    for k in range(2 * n - 5, 1, -2):
+        out *= k
+    return out
+
+
+def num_rooted_trees(n: int) -> int:
+    """Distinct rooted binary topologies on n labelled leaves:
+    Tree.num_rooted_trees (lib/tree.ml:588-595): n for n <= 1, else
+    num_edges(n) * num_unrooted_trees(n) = (2n - 3)!!."""
+    if n < 0:
+        raise ValueError("num_rooted_trees: negative leaf count")
+    return n if n <= 1 else num_edges(n) * num_unrooted_trees(n)
+
+
 def nni_moves(n_leaves: int, children, root: int) -> np.ndarray:
     """The 2(N - 3) NNI neighbours as SPR pairs, int32 [n, 2]."""
     ch = np.ascontiguousarray(children, np.int32)
@@ -238,7 +277,7 @@ def main(argv=None):
 
 
 __all__ = ["nni_moves", "tbr_moves", "apply", "Deferred", "Neighborhood", "Scorer",
-           "hill_climb"]
+           "hill_climb", "num_edges", "num_nodes", "num_unrooted_trees", "num_rooted_trees"]
 
 
 if __name__ == "__main__":

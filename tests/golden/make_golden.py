@@ -11,6 +11,8 @@ reference_kats.json -- known-answer data transcribed from the reference's own
                                    pair of Alphabet.dna codes (listed here as
                                    explicit (x, y, median, cost) rows, each
                                    computed with Python sets of state names)
+      test/treeTest.ml:221-255     Tree.num_unrooted_trees, num_rooted_trees,
+                                   num_nodes and num_edges for n = 0 .. 12 / 10
 model_goldens.json -- scipy reference values:
       scipy.linalg.expm(Q t) for GTR rate matrices built here from the
       textbook definition (not from the oracle), and Gamma quantiles /
@@ -49,6 +51,17 @@ def kats():
                   "H": ["G", "T", "A"], "D": ["C", "T", "A"], "B": ["G", "C", "A"],
                   "N": ["A", "C", "G", "T"]},
         "fitch_pairs": rows,
+        "tree_counts": {
+            "unrooted": [[0, 0], [1, 1], [2, 1], [3, 1], [4, 3], [5, 15], [6, 105], [7, 945],
+                         [8, 10395], [9, 135135], [10, 2027025], [11, 34459425],
+                         [12, 654729075]],
+            "rooted": [[0, 0], [1, 1], [2, 1], [3, 3], [4, 15], [5, 105], [6, 945], [7, 10395],
+                       [8, 135135], [9, 2027025], [10, 34459425]],
+            "nodes": [[0, 0], [1, 1], [2, 2], [3, 4], [4, 6], [5, 8], [6, 10], [7, 12], [8, 14],
+                      [9, 16], [10, 18]],
+            "edges": [[0, 0], [1, 0], [2, 1], [3, 3], [4, 5], [5, 7], [6, 9], [7, 11], [8, 13],
+                      [9, 15], [10, 17]],
+        },
     }
 
 

@@ -238,3 +238,92 @@ def test_neighborhood_interface_forms(rng):
         assert seen == full
         parts = nb.deferred_partition(t, 5)
         assert sum((list(nb.lazy_of_deferred(t, p)) for p in parts), []) == full
+
+
+def _clades(N, ch, root):
+    """Rooted topology key: the leaf set below every internal non-root node."""
+    below = {i: 1 << i for i in range(N)}
+    order, st = [], [root]
+    while st:
+        v = st.pop()
+        order.append(v)
+        if v >= N:
+            st += [int(ch[2 * (v - N)]), int(ch[2 * (v - N) + 1])]
+    for v in reversed(order):
+        if v >= N:
+            below[v] = below[int(ch[2 * (v - N)])] | below[int(ch[2 * (v - N) + 1])]
+    return frozenset(below[v] for v in range(N, 2 * N - 1) if v != root)
+
+
+def _walk(N, start, neighbours, key):
+    """Breadth-first walk over the trees reachable by repeated moves."""
+    seen = {key(N, *start): start}
+    frontier = [start]
+    while frontier:
+        nxt = []
+        for ch, root in frontier:
+            for ch2, r2 in neighbours(ch, root):
+                k = key(N, ch2, r2)
+                if k not in seen:
+                    seen[k] = (ch2, r2)
+                    nxt.append((ch2, r2))
+        frontier = nxt
+    return seen
+
+
+def test_tree_counts_match_reference_fixtures():
+    # Tree.num_unrooted_trees / num_rooted_trees / num_nodes / num_edges
+    # against the values of test/treeTest.ml:221-255
+    import json
+    import os
+
+    from conftest import GOLDEN
+    from phylocaml_amd import neighborhood as NB
+
+    kat = json.load(open(os.path.join(GOLDEN, "reference_kats.json")))["tree_counts"]
+    fns = {"unrooted": NB.num_unrooted_trees, "rooted": NB.num_rooted_trees,
+           "nodes": NB.num_nodes, "edges": NB.num_edges}
+    for name, fn in fns.items():
+        for n, want in kat[name]:
+            assert fn(n) == want, (name, n)
+        with pytest.raises(ValueError):
+            fn(-1)
+    # the rooted layout of N leaves has 2N - 1 nodes and 2N - 2 branches; its
+    # unrooted view suppresses the root (one node, and its two branches are
+    # one edge)
+    for N in (2, 3, 10, 1024):
+        assert (2 * N - 1) - 1 == NB.num_nodes(N) and (2 * N - 2) - 1 == NB.num_edges(N)
+
+
+@pytest.mark.parametrize("N", [4, 5, 6, 7])
+def test_nni_walk_reaches_every_unrooted_topology(N):
+    # NNI connects the space of unrooted trees: walking from a caterpillar
+    # reaches exactly (2N - 5)!! distinct split sets
+    from phylocaml_amd import neighborhood as NB
+
+    bl = np.full(2 * N - 1, 0.1)
+
+    def nbrs(ch, root):
+        for m in NB.nni_moves(N, ch, root):
+            c2, r2, _ = NB.apply(N, ch, root, bl, m)
+            yield c2, r2
+
+    seen = _walk(N, caterpillar(N), nbrs, _rooted_splits)
+    assert len(seen) == NB.num_unrooted_trees(N)
+
+
+@pytest.mark.parametrize("N", [3, 4, 5])
+def test_rooted_spr_walk_reaches_every_rooted_topology(N):
+    # rooted SPR (every (s, v) of the rooted layout) connects the rooted
+    # trees: the walk reaches exactly (2N - 3)!! distinct clade sets
+    from phylocaml_amd import neighborhood as NB, spr
+
+    bl = np.full(2 * N - 1, 0.1)
+
+    def nbrs(ch, root):
+        for s, v in spr.moves(N, ch, root):
+            c2, r2, _ = spr.apply(N, ch, root, bl, int(s), int(v))
+            yield c2, r2
+
+    seen = _walk(N, caterpillar(N), nbrs, _clades)
+    assert len(seen) == NB.num_rooted_trees(N)
