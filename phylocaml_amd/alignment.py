@@ -3,11 +3,16 @@ site-pattern compression, and the leaf formats of the GPU passes.
 
 * Alphabets follow lib/alphabet.ml:301-343.  ``dna`` and ``nucleotides``
   are bit-flag alphabets (A = 1, C = 2, G = 4, T = 8, gap = 16; the code
-  table of test/alphabetTest.ml:12-18); ``nucleotides`` adds the IUPAC and
-  indel polymorphism codes of lib/alphabet.ml:313-323; the "all" / missing
-  symbol is the union of every state.  ``aminoacids`` is sequential: the 20
-  residues in the order of lib/alphabet.ml:329-341 are 0..19, X (all) is 20
-  and the gap 21.  Symbols are case-insensitive (``~case:false``).
+  table of test/alphabetTest.ml:12-18).  ``dna`` has X as a sixth atomic
+  state (32, its "all" and missing symbol) and the equates 0..4 for
+  A C G T -; ``nucleotides`` has five atomic states plus the IUPAC and indel
+  polymorphism codes of lib/alphabet.ml:313-323, "?" (all, missing) being
+  the union of the five.  ``aminoacids`` is sequential: the 20 residues in
+  the order of lib/alphabet.ml:329-341 are 0..19, X (all) is 20 and the gap
+  21.  Symbols are case-insensitive (``~case:false``).
+* Polymorphisms (lib/alphabet.ml:418-466): ``max_polymorphism``,
+  ``get_combination``, ``get_state_combination`` (``_exn``),
+  ``explode_polymorphisms``, ``compress_polymorphisms``, ``is_atomic``.
 * ``of_string`` is Sequence.of_string (lib/sequence.ml:209-227) for these
   one-character alphabets: an unknown symbol raises ``InvalidSequence``.
 * ``read_fasta`` / ``read_phylip`` (sequential or interleaved, relaxed
@@ -30,12 +35,77 @@ class InvalidSequence(ValueError):
     """Sequence.Invalid_Sequence (lib/sequence.ml:209-227)."""
 
 
+class AlphabetError(ValueError):
+    """Alphabet.Error (lib/alphabet.ml): an illegal code, or a set of states
+    the alphabet cannot represent as one code."""
+
+
 @dataclass(frozen=True)
 class Alphabet:
     name: str
     codes: dict          # upper-case symbol -> code
     kind: str            # "bitflag" or "sequential"
     n_states: int        # atomic states (the gap included for DNA)
+
+    @property
+    def atomic(self) -> list:
+        """Codes of the atomic states (t.atomic)."""
+        if self.kind == "bitflag":
+            return [1 << i for i in range(self.n_states)]
+        return list(range(self.n_states))
+
+    def max_polymorphism(self) -> int:
+        """lib/alphabet.ml:420-424: all states for a bit-flag alphabet, one
+        for a sequential one."""
+        return self.n_states if self.kind == "bitflag" else 1
+
+    def is_atomic(self, code: int) -> bool:
+        return code in self.atomic
+
+    def get_combination(self, code: int) -> set:
+        """The atomic states of one code (lib/alphabet.ml:427-434)."""
+        if self.kind == "bitflag":
+            return {a for a in self.atomic if code & a}
+        if code in self.atomic:
+            return {code}
+        raise AlphabetError(f"illegal code {code} for alphabet {self.name}")
+
+    def get_state_combination_exn(self, states) -> int:
+        """The code of a set of atomic states (lib/alphabet.ml:437-449): their
+        sum for a bit-flag alphabet; a sequential alphabet represents single
+        states only."""
+        states = set(int(x) for x in states)
+        if self.kind == "bitflag":
+            return sum(states)
+        if len(states) != 1:
+            raise AlphabetError(f"{self.name}: one state per code, got {sorted(states)}")
+        (c,) = states
+        if c not in self.atomic:
+            raise AlphabetError(f"illegal code {c} for alphabet {self.name}")
+        return c
+
+    def get_state_combination(self, states):
+        """As ``get_state_combination_exn``, None where the states cannot be
+        represented (lib/alphabet.ml:452-456)."""
+        try:
+            return self.get_state_combination_exn(states)
+        except AlphabetError:
+            if self.kind == "sequential" and len(set(states)) == 1:
+                raise
+            return None
+
+    def explode_polymorphisms(self, codes) -> list:
+        """Sorted atomic states of the union of codes (lib/alphabet.ml:459-461)."""
+        out = set()
+        for c in codes:
+            out |= self.get_combination(int(c))
+        return sorted(out)
+
+    def compress_polymorphisms(self, codes) -> int:
+        """The code of the union of codes, truncated to the first
+        ``max_polymorphism`` atomic states (lib/alphabet.ml:464-466)."""
+        return self.get_state_combination_exn(
+            self.explode_polymorphisms(codes)[: self.max_polymorphism()])
 
     def get_code(self, sym: str) -> int:
         try:
@@ -58,7 +128,8 @@ def _bitflag(extra: dict) -> dict:
     return out
 
 
-dna = Alphabet("dna", _bitflag({"X": "ACGT-"}), "bitflag", 5)
+dna = Alphabet("dna", {"A": 1, "C": 2, "G": 4, "T": 8, GAP: 16, "X": 32,
+                      "0": 1, "1": 2, "2": 4, "3": 8, "4": 16}, "bitflag", 6)
 nucleotides = Alphabet("nucleotides", _bitflag({
     "M": "AC", "R": "AG", "W": "AT", "S": "GC", "Y": "TC", "K": "GT",
     "V": "GTC", "H": "GTA", "D": "CTA", "B": "GCA", "N": "ACGT", "X": "ACGT",
@@ -157,8 +228,9 @@ def compress(codes: np.ndarray):
 
 
 def fitch_sets(codes: np.ndarray) -> np.ndarray:
-    """4-bit Fitch / pruning sets of bit-flag DNA codes: the gap reads as
-    missing data (all four bases); a base set with a gap keeps its bases."""
+    """4-bit Fitch / pruning sets of bit-flag DNA codes: the gap, and the
+    ``dna`` alphabet's X (32), read as missing data (all four bases); a base
+    set with a gap keeps its bases."""
     c = np.asarray(codes, np.uint8) & 15
     return np.where(c == 0, 15, c).astype(np.uint8)
 
@@ -167,7 +239,8 @@ tip_sets = fitch_sets
 
 
 def sankoff_sets(codes: np.ndarray) -> np.ndarray:
-    """5-state Sankoff sets of bit-flag DNA codes, the gap as the fifth state."""
+    """5-state Sankoff sets of bit-flag DNA codes, the gap as the fifth state
+    (``dna``'s X, 32, becomes the empty set: every state)."""
     return (np.asarray(codes, np.uint8) & 31).astype(np.uint8)
 
 
@@ -176,6 +249,6 @@ def aa_states(codes: np.ndarray) -> np.ndarray:
     return np.asarray(codes, np.uint8).copy()
 
 
-__all__ = ["Alphabet", "InvalidSequence", "dna", "nucleotides", "aminoacids", "of_string",
+__all__ = ["Alphabet", "AlphabetError", "InvalidSequence", "dna", "nucleotides", "aminoacids", "of_string",
            "to_string", "read_fasta", "read_phylip", "compress", "fitch_sets", "tip_sets",
            "sankoff_sets", "aa_states"]

@@ -80,3 +80,47 @@ def test_compression_keeps_fitch_cost_and_lnl(rng):
     # Sankoff sets keep the gap as the fifth state
     assert AL.sankoff_sets(AL.of_string("A-R1", AL.nucleotides)).tolist() == [1, 16, 5, 24]
     assert AL.fitch_sets(AL.of_string("A-R1", AL.nucleotides)).tolist() == [1, 15, 5, 8]
+
+
+def test_polymorphisms_follow_the_reference_properties():
+    # the properties of test/alphabetTest.ml:120-169 over Alphabet.dna, whose
+    # atomic states are A C G T - X (lib/alphabet.ml:301-307)
+    import itertools
+    import json
+    import os
+
+    from conftest import GOLDEN
+    from phylocaml_amd import alignment as AL
+
+    kat = json.load(open(os.path.join(GOLDEN, "reference_kats.json")))
+    dna = AL.dna
+    assert {s: dna.get_code(s) for s in kat["dna_states"]} == kat["dna_states"]
+    assert dna.atomic == sorted(kat["dna_states"].values())
+    assert [dna.get_code(e) for e in "01234"] == [1, 2, 4, 8, 16]  # the equates
+    for x in dna.atomic:
+        assert dna.explode_polymorphisms([x, x]) == [x]
+    for x, y in itertools.permutations(dna.atomic, 2):
+        xy = dna.compress_polymorphisms([x, y])
+        assert xy == x + y and dna.explode_polymorphisms([xy]) == sorted([x, y])
+    acc, total = set(), 0
+    for e in dna.atomic:
+        acc.add(e)
+        total += e
+        assert dna.get_state_combination_exn(acc) == total
+    for x, y, z in itertools.permutations(dna.atomic, 3):
+        assert dna.explode_polymorphisms([x + y + z]) == sorted([x, y, z])
+    # IUPAC codes explode to their members
+    nuc = AL.nucleotides
+    assert nuc.explode_polymorphisms([nuc.get_code("V")]) == sorted(
+        nuc.get_code(c) for c in "GTC")
+    assert nuc.compress_polymorphisms([nuc.get_code("M"), nuc.get_code("K")]) == nuc.get_code("N")
+    # a sequential alphabet represents one state per code
+    aa = AL.aminoacids
+    assert aa.max_polymorphism() == 1 and aa.get_state_combination({5}) == 5
+    assert aa.get_state_combination({1, 2}) is None
+    assert aa.compress_polymorphisms([7, 3]) == 3  # truncated to the first state
+    with pytest.raises(AL.AlphabetError):
+        aa.get_combination(25)
+    # dna's X reads as missing data in the leaf formats
+    assert AL.fitch_sets(np.array([32, 16, 5], np.uint8)).tolist() == [15, 15, 5]
+    assert AL.sankoff_sets(np.array([32, 17], np.uint8)).tolist() == [0, 17]
