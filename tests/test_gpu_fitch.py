@@ -98,7 +98,7 @@ def test_tree_passes_match_oracle(ctx, rng, N, S):
 
 
 @pytest.mark.parametrize("words", ["1", "2"])
-@pytest.mark.parametrize("batch", ["8", "16", "32"])
+@pytest.mark.parametrize("batch", ["4", "8", "16", "32"])
 @pytest.mark.parametrize("split", ["0", "1"])
 def test_prefetch_depth_and_split_forms(ctx, rng, monkeypatch, batch, split, words):
     # the whole-tree and the split (parts side by side, then the top) forms,
