@@ -15,7 +15,7 @@ is the adjust_3 branch-length update built on them.
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass, replace
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -63,6 +63,7 @@ class Likelihood:
     weights: object = None  # device uint32 [n_patterns] (pattern weights)
     form: int = PHY_F64
     lnl: float | None = None
+    codes: list = field(default_factory=list)  # character code of each pattern
 
 
 def _dtype(form):
@@ -117,6 +118,83 @@ def root_cost(ctx, dm: DeviceModel, x: Likelihood) -> float:
 
 def leaf_cost(x: Likelihood) -> float:
     return 0.0
+
+
+# Character bookkeeping of NodeData.S (lib/nodeData.ml:9-16; TODO in
+# lib/likelihood_c.ml:5-12), as Node.Make1D calls it (lib/node.ml:93-145):
+# one character per pattern, identified by its code (``codes``, the pattern
+# index when not given).
+def _codes(x: Likelihood) -> list:
+    return list(x.codes) if len(x.codes) == x.n_patterns else list(range(x.n_patterns))
+
+
+def cardinal(x: Likelihood) -> int:
+    return x.n_patterns
+
+
+def get_codes(x: Likelihood) -> set:
+    return set(_codes(x))
+
+
+def mem(codes, x: Likelihood) -> bool:
+    return True if codes is None else bool(set(codes) & get_codes(x))
+
+
+def recode(f, x: Likelihood) -> Likelihood:
+    return replace(x, codes=[f(c) for c in _codes(x)])
+
+
+def _select(x: Likelihood, keep: list):
+    torch = __import__("torch")
+    if not keep:
+        return None
+    dev = x.partial.device
+    idx = torch.as_tensor(keep, dtype=torch.int64, device=dev)
+    n = len(keep)
+    pitch = ((n + 63) // 64) * 64
+    part = torch.ones((x.partial.shape[0], pitch), dtype=x.partial.dtype, device=dev)
+    part[:, :n] = x.partial.index_select(1, idx)
+    ex = torch.zeros((pitch,), dtype=x.exps.dtype, device=dev)
+    ex[:n] = x.exps.index_select(0, idx)
+    w = None if x.weights is None else \
+        x.weights.view(torch.int32).index_select(0, idx).view(torch.uint32)
+    codes = _codes(x)
+    return replace(x, partial=part, exps=ex, n_patterns=n, weights=w, lnl=None,
+                   codes=[codes[i] for i in keep])
+
+
+def filter_codes(codes, x: Likelihood):
+    """The patterns of x whose code is in codes (None when there is none):
+    their partials, exponents and weights; padding columns hold ones."""
+    cs = set(codes)
+    return _select(x, [i for i, c in enumerate(_codes(x)) if c in cs])
+
+
+def filter_codes_comp(codes, x: Likelihood):
+    cs = set(codes)
+    return _select(x, [i for i, c in enumerate(_codes(x)) if c not in cs])
+
+
+def compare(a: Likelihood, b: Likelihood) -> int:
+    """A total order: codes, then branch length, then the scaled partials
+    pattern by pattern (exponent first, then the values)."""
+    ka, kb = _codes(a), _codes(b)
+    if ka != kb:
+        return -1 if ka < kb else 1
+    if a.blen != b.blen:
+        return -1 if a.blen < b.blen else 1
+    n = a.n_patterns
+    ea, eb = a.exps[:n], b.exps[:n]
+    pa, pb = a.partial[:, :n], b.partial[:, :n]
+    diff = (ea != eb) | (pa != pb).any(0)
+    nz = diff.nonzero()
+    if not nz.numel():
+        return 0
+    p = int(nz[0, 0])
+    if int(ea[p]) != int(eb[p]):
+        return -1 if int(ea[p]) < int(eb[p]) else 1
+    col_a, col_b = pa[:, p].cpu().tolist(), pb[:, p].cpu().tolist()
+    return -1 if col_a < col_b else 1
 
 
 class Evaluator:
@@ -180,4 +258,5 @@ def newton_step(blen: np.ndarray, d1: np.ndarray, d2: np.ndarray, root: int,
 
 
 __all__ = ["DeviceModel", "Likelihood", "Evaluator", "leaf", "median_2", "root_lnl",
-           "root_cost", "leaf_cost", "newton_step", "PHY_F32", "PHY_F64", "PHY_FUSED"]
+           "root_cost", "leaf_cost", "cardinal", "get_codes", "mem", "recode", "filter_codes",
+           "filter_codes_comp", "compare", "newton_step", "PHY_F32", "PHY_F64", "PHY_FUSED"]

@@ -132,6 +132,87 @@ def leaf_cost(x: NonAdditive) -> float:
 
 
 # ---------------------------------------------------------------------------
+# Character bookkeeping of NodeData.S (lib/nodeData.ml:9-16; TODO in
+# lib/nonAdditive_c.ml:9-16), as Node.Make1D calls it (lib/node.ml:93-145):
+# a value holds one character per site, identified by its code (``codes``,
+# the site index when not given).
+# ---------------------------------------------------------------------------
+def _codes(x: NonAdditive) -> list:
+    return list(x.codes) if len(x.codes) == x.n_sites else list(range(x.n_sites))
+
+
+def cardinal(x: NonAdditive) -> int:
+    """Number of characters."""
+    return x.n_sites
+
+
+def get_codes(x: NonAdditive) -> set:
+    return set(_codes(x))
+
+
+def mem(codes, x: NonAdditive) -> bool:
+    """Whether x holds any of codes (None: any code)."""
+    return True if codes is None else bool(set(codes) & get_codes(x))
+
+
+def recode(f, x: NonAdditive) -> NonAdditive:
+    """The same states under codes f(code)."""
+    return replace(x, codes=[f(c) for c in _codes(x)])
+
+
+def _select(x: NonAdditive, keep: list):
+    torch = __import__("torch")
+    if not keep:
+        return None
+    dev = x.states.device
+    b = x.states.view(torch.uint8)
+    sites = torch.stack([b & 15, b >> 4], 1).reshape(-1)[: x.n_sites]
+    idx = torch.as_tensor(keep, dtype=torch.int64, device=dev)
+    n = len(keep)
+    pad = torch.zeros((D.words(n) * 16,), dtype=torch.uint8, device=dev)
+    pad[:n] = sites.index_select(0, idx)
+    packed = (pad[0::2] | (pad[1::2] << 4)).contiguous().view(torch.uint64)
+    # uint32 weights gathered through an int32 view (same bits)
+    w = None if x.weights is None else \
+        x.weights.view(torch.int32).index_select(0, idx).view(torch.uint32)
+    codes = _codes(x)
+    return NonAdditive(codes=[codes[i] for i in keep], weights=w, states=packed, n_sites=n,
+                       cost=0.0)
+
+
+def filter_codes(codes, x: NonAdditive):
+    """The characters of x whose code is in codes, None when there is none.
+    The node cost is not carried over (median_2 recomputes it)."""
+    cs = set(codes)
+    return _select(x, [i for i, c in enumerate(_codes(x)) if c in cs])
+
+
+def filter_codes_comp(codes, x: NonAdditive):
+    """The characters of x whose code is not in codes (None when none is left)."""
+    cs = set(codes)
+    return _select(x, [i for i, c in enumerate(_codes(x)) if c not in cs])
+
+
+def compare(a: NonAdditive, b: NonAdditive) -> int:
+    """A total order: codes, then the state sets site by site, then the cost."""
+    torch = __import__("torch")
+    ka, kb = _codes(a), _codes(b)
+    if ka != kb:
+        return -1 if ka < kb else 1
+    sa, sb = a.states.view(torch.int64), b.states.view(torch.int64)  # same bits
+    ne = (sa != sb).nonzero()
+    if ne.numel():
+        k = int(ne[0, 0])
+        wa = int(sa[k].cpu()) & (2 ** 64 - 1)
+        wb = int(sb[k].cpu()) & (2 ** 64 - 1)
+        for j in range(16):
+            na, nb = (wa >> (4 * j)) & 15, (wb >> (4 * j)) & 15
+            if na != nb:
+                return -1 if na < nb else 1
+    return (a.cost > b.cost) - (a.cost < b.cost)
+
+
+# ---------------------------------------------------------------------------
 # Whole-tree passes (Node.Make1D over a Tree, in one launch each)
 # ---------------------------------------------------------------------------
 @dataclass
