@@ -149,6 +149,16 @@ int phy_fitch_median2(phy_ctx* ctx, const uint64_t* d_x, const uint64_t* d_y,
                       uint64_t* d_out, int64_t n_sites, const uint32_t* d_weights,
                       uint64_t* d_cost);
 
+/* filter_codes / filter_codes_comp of NodeData.S (lib/nodeData.ml:9-10, TODO
+ * in lib/nonAdditive_c.ml:9-10): the characters h_idx[0 .. n_idx-1] (host
+ * array of site indices, each in 0 .. n_sites-1, checked) of a nibble block,
+ * in that order, into a new block of words(n_idx) words, and their weights
+ * when d_weights is given (d_weights_out n_idx uint32; both null or both
+ * given).  Synchronous on the context's stream. */
+int phy_fitch_select(phy_ctx* ctx, const uint64_t* d_states, int64_t n_sites,
+                     const uint32_t* d_weights, const int64_t* h_idx, int64_t n_idx,
+                     uint64_t* d_states_out, uint32_t* d_weights_out);
+
 /* median_3 a b c (lib/nonAdditive_c.ml signature slot, TODO in the reference):
  * the 3-way Fitch median: per site the states contained in the largest number
  * of {a,b,c}; cost = sum_s w_s (3 - that number). */

@@ -64,6 +64,7 @@ SIGNATURES = {
     "phy_fitch_distance1": (C.c_int, [vp, vp, vp, i64, vp, vp]),
     "phy_fitch_distance2": (C.c_int, [vp, vp, vp, vp, i64, vp, vp]),
     "phy_fitch_adjust3": (C.c_int, [vp, vp, vp, vp, vp, vp, i64, vp]),
+    "phy_fitch_select": (C.c_int, [vp, vp, i64, vp, P_i64, i64, vp, vp]),
     "phy_fitch_downpass": (C.c_int, [vp, vp, vp, i64, i64, vp, vp, vp, vp]),
     "phy_fitch_uppass": (C.c_int, [vp, vp, vp, vp, i64, i64]),
     "phy_sankoff_downpass": (C.c_int, [vp, vp, i32, vp, vp, i64, i64, vp, vp, vp]),

@@ -594,6 +594,64 @@ int phy_fitch_adjust3(phy_ctx* ctx, const uint64_t* pv, const uint64_t* pa, cons
 
 namespace {
 
+// one thread per output word: the 16 selected sites' nibbles (and weights)
+__global__ __launch_bounds__(256) void k_fitch_select(const uint64_t* __restrict__ src,
+                                                      const uint32_t* __restrict__ w_src,
+                                                      const int64_t* __restrict__ idx, int64_t n_idx,
+                                                      uint64_t* __restrict__ dst,
+                                                      uint32_t* __restrict__ w_dst) {
+  const int64_t word = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (word >= (n_idx + 15) / 16) return;
+  uint64_t out = 0;
+  for (int j = 0; j < 16; ++j) {
+    const int64_t s = word * 16 + j;
+    if (s >= n_idx) break;
+    const int64_t k = idx[s];
+    out |= ((src[k >> 4] >> (4 * (k & 15))) & 15ull) << (4 * j);
+    if (w_dst) w_dst[s] = w_src[k];
+  }
+  dst[word] = out;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phy_fitch_select(phy_ctx* ctx, const uint64_t* d_states, int64_t n_sites,
+                     const uint32_t* d_weights, const int64_t* h_idx, int64_t n_idx,
+                     uint64_t* d_states_out, uint32_t* d_weights_out) {
+  if (!ctx) return PHY_EINVAL;
+  if (n_sites < 0 || n_idx < 0) return set_err(ctx, PHY_EINVAL, "select: negative size");
+  if (n_idx == 0) return PHY_OK;
+  if (!d_states || !h_idx || !d_states_out) return set_err(ctx, PHY_EINVAL, "select: null argument");
+  if ((d_weights == nullptr) != (d_weights_out == nullptr))
+    return set_err(ctx, PHY_EINVAL, "select: weights in and out must both be given or both be null");
+  for (int64_t s = 0; s < n_idx; ++s)
+    if (h_idx[s] < 0 || h_idx[s] >= n_sites) return set_err(ctx, PHY_EINVAL, "select: site index out of range");
+  int64_t* d_idx = nullptr;
+  if (hipMalloc((void**)&d_idx, (size_t)n_idx * sizeof(int64_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_err(ctx, PHY_ENOMEM, "select: index buffer");
+  }
+  int rc = PHY_OK;
+  if (hipMemcpyAsync(d_idx, h_idx, (size_t)n_idx * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+    rc = set_err(ctx, PHY_EDEVICE, "select: index upload");
+  } else {
+    const int64_t words = (n_idx + 15) / 16;
+    hipLaunchKernelGGL(k_fitch_select, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, ctx->stream,
+                       d_states, d_weights, d_idx, n_idx, d_states_out, d_weights_out);
+    if (hipGetLastError() != hipSuccess) rc = set_err(ctx, PHY_EDEVICE, "select: launch");
+  }
+  // the index buffer is released once the stream has used it
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess && rc == PHY_OK) rc = set_err(ctx, PHY_EDEVICE, "select: sync");
+  (void)hipFree(d_idx);
+  return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
 // Split passes (parts side by side, then the top) unless disabled; the
 // whole-tree program otherwise.
 bool use_split(const phy_topo* t) {

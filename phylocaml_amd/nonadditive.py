@@ -15,12 +15,13 @@ whole-tree functions at the end keep everything on the device.
 """
 from __future__ import annotations
 
+import ctypes as C
 from dataclasses import dataclass, field, replace
 
 import numpy as np
 
 from . import device as D
-from ._lib import lib
+from ._lib import lib, ptr
 
 
 @dataclass
@@ -160,37 +161,33 @@ def recode(f, x: NonAdditive) -> NonAdditive:
     return replace(x, codes=[f(c) for c in _codes(x)])
 
 
-def _select(x: NonAdditive, keep: list):
+def _select(ctx: D.Context, x: NonAdditive, keep: list):
     torch = __import__("torch")
     if not keep:
         return None
-    dev = x.states.device
-    b = x.states.view(torch.uint8)
-    sites = torch.stack([b & 15, b >> 4], 1).reshape(-1)[: x.n_sites]
-    idx = torch.as_tensor(keep, dtype=torch.int64, device=dev)
     n = len(keep)
-    pad = torch.zeros((D.words(n) * 16,), dtype=torch.uint8, device=dev)
-    pad[:n] = sites.index_select(0, idx)
-    packed = (pad[0::2] | (pad[1::2] << 4)).contiguous().view(torch.uint64)
-    # uint32 weights gathered through an int32 view (same bits)
-    w = None if x.weights is None else \
-        x.weights.view(torch.int32).index_select(0, idx).view(torch.uint32)
+    idx = np.ascontiguousarray(keep, np.int64)
+    out = torch.empty((D.words(n),), dtype=torch.uint64, device=x.states.device)
+    w = None if x.weights is None else torch.empty((n,), dtype=torch.uint32,
+                                                    device=x.weights.device)
+    ctx.check(lib.phy_fitch_select(ctx.handle, D.dptr(x.states), x.n_sites, D.dptr(x.weights),
+                                   ptr(idx, C.c_int64), n, D.dptr(out), D.dptr(w)), "fitch_select")
     codes = _codes(x)
-    return NonAdditive(codes=[codes[i] for i in keep], weights=w, states=packed, n_sites=n,
-                       cost=0.0)
+    return NonAdditive(codes=[codes[i] for i in keep], weights=w, states=out, n_sites=n, cost=0.0)
 
 
-def filter_codes(codes, x: NonAdditive):
-    """The characters of x whose code is in codes, None when there is none.
-    The node cost is not carried over (median_2 recomputes it)."""
+def filter_codes(ctx: D.Context, codes, x: NonAdditive):
+    """The characters of x whose code is in codes (phy_fitch_select), None
+    when there is none.  The node cost is not carried over (median_2
+    recomputes it)."""
     cs = set(codes)
-    return _select(x, [i for i, c in enumerate(_codes(x)) if c in cs])
+    return _select(ctx, x, [i for i, c in enumerate(_codes(x)) if c in cs])
 
 
-def filter_codes_comp(codes, x: NonAdditive):
+def filter_codes_comp(ctx: D.Context, codes, x: NonAdditive):
     """The characters of x whose code is not in codes (None when none is left)."""
     cs = set(codes)
-    return _select(x, [i for i, c in enumerate(_codes(x)) if c not in cs])
+    return _select(ctx, x, [i for i, c in enumerate(_codes(x)) if c not in cs])
 
 
 def compare(a: NonAdditive, b: NonAdditive) -> int:

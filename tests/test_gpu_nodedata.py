@@ -38,17 +38,32 @@ def test_fitch_values_filter_recode_compare(ctx, rng):
     assert NA.mem(None, x) and NA.mem([codes[3], -1], x) and not NA.mem([-1], x)
     keep = set(codes[i] for i in rng.choice(S, 300, replace=False))
     sel = [i for i, c in enumerate(codes) if c in keep]
-    fx, fy = NA.filter_codes(keep, x), NA.filter_codes(keep, y)
+    fx, fy = NA.filter_codes(ctx, keep, x), NA.filter_codes(ctx, keep, y)
     assert NA.to_sets(fx).tolist() == a[sel].tolist() and NA.cardinal(fx) == len(sel)
     assert [codes[i] for i in sel] == fx.codes
     assert fx.weights.view(torch.int32).cpu().numpy().tolist() == w[sel].astype(np.int32).tolist()
-    comp = NA.filter_codes_comp(keep, x)
+    comp = NA.filter_codes_comp(ctx, keep, x)
     assert NA.cardinal(comp) + NA.cardinal(fx) == S and not (NA.get_codes(comp) & keep)
-    assert NA.filter_codes([-1], x) is None and NA.filter_codes_comp(codes, x) is None
+    assert NA.filter_codes(ctx, [-1], x) is None and NA.filter_codes_comp(ctx, codes, x) is None
+    # an unweighted block, and an index order that is not sorted
+    u = NA.of_sets(ctx, a, None, codes)
+    rev = list(reversed(sel))
+    fu = NA._select(ctx, u, rev)
+    assert NA.to_sets(fu).tolist() == a[rev].tolist() and fu.weights is None
+    # an index out of range is refused before anything reaches the GPU
+    import ctypes as C
+
+    from phylocaml_amd import device as D
+    from phylocaml_amd._lib import lib, ptr
+
+    bad = np.array([0, S], np.int64)
+    out = torch.empty((1,), dtype=torch.uint64, device=x.states.device)
+    assert lib.phy_fitch_select(ctx.handle, D.dptr(x.states), S, None, ptr(bad, C.c_int64), 2,
+                                D.dptr(out), None) == 1  # PHY_EINVAL
     # filtering commutes with the per-site median; the cost is over the kept sites
     m_f = NA.median_2(ctx, None, None, fx, fy)
     m = NA.median_2(ctx, None, None, x, y)
-    assert NA.to_sets(m_f).tolist() == NA.to_sets(NA.filter_codes(keep, m)).tolist()
+    assert NA.to_sets(m_f).tolist() == NA.to_sets(NA.filter_codes(ctx, keep, m)).tolist()
     om, oc = O.fitch_median2(a[sel], b[sel], w[sel])
     assert NA.to_sets(m_f).tolist() == np.asarray(om).tolist() and m_f.cost == oc
     # recode keeps the states
