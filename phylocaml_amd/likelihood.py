@@ -108,12 +108,21 @@ def root_lnl(ctx, dm: DeviceModel, x: Likelihood) -> float:
     ctx.check(lib.phy_lk_root(ctx.handle, dm.handle, x.form, x.n_patterns, D.dptr(x.partial),
                               D.dptr(x.exps), x.partial.shape[1], D.dptr(x.weights),
                               D.dptr(out)), "lk_root")
-    return float(out.cpu().numpy()[0])
+    x.lnl = float(out.cpu().numpy()[0])  # kept for cost
+    return x.lnl
 
 
 def root_cost(ctx, dm: DeviceModel, x: Likelihood) -> float:
     """-lnL of the tree rooted at x (NodeData costs are minimised)."""
     return -root_lnl(ctx, dm, x)
+
+
+def cost(x: Likelihood) -> float:
+    """NodeData.cost: -lnL of the tree rooted at x, as the last root_lnl /
+    root_cost of x evaluated it (median_2 results start unevaluated)."""
+    if x.lnl is None:
+        raise ValueError("cost: evaluate root_cost(ctx, dm, x) first")
+    return -x.lnl
 
 
 def leaf_cost(x: Likelihood) -> float:
@@ -258,5 +267,5 @@ def newton_step(blen: np.ndarray, d1: np.ndarray, d2: np.ndarray, root: int,
 
 
 __all__ = ["DeviceModel", "Likelihood", "Evaluator", "leaf", "median_2", "root_lnl",
-           "root_cost", "leaf_cost", "cardinal", "get_codes", "mem", "recode", "filter_codes",
+           "root_cost", "cost", "leaf_cost", "cardinal", "get_codes", "mem", "recode", "filter_codes",
            "filter_codes_comp", "compare", "newton_step", "PHY_F32", "PHY_F64", "PHY_FUSED"]

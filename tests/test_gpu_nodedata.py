@@ -102,7 +102,11 @@ def test_likelihood_values_filter_patterns(ctx, rng):
         fx, fy = f(keep, x), f(keep, y)
         root = LK.median_2(ctx, dm, None, fx, fy)
         parts.append(LK.root_lnl(ctx, dm, root))
-    full = LK.root_lnl(ctx, dm, LK.median_2(ctx, dm, None, x, y))
+    top = LK.median_2(ctx, dm, None, x, y)
+    with pytest.raises(ValueError):
+        LK.cost(top)  # not evaluated yet
+    full = LK.root_lnl(ctx, dm, top)
+    assert LK.cost(top) == -full == LK.root_cost(ctx, dm, top)
     assert abs(parts[0] + parts[1] - full) <= 1e-11 * abs(full)
     # the same as the oracle's two-taxon tree on the kept patterns
     sel = sorted(keep)
