@@ -105,3 +105,54 @@ def test_plain_c_consumer_matches_oracle(tmp_path, seed, N, S, P):
     mask = np.arange(2 * N - 1) != c["root"]
     assert np.all(np.abs(g["d1"] - od1)[mask] <= 1e-8 * (np.abs(od1[mask]) + scale))
     assert np.all(np.abs(g["d2"] - od2)[mask] <= 1e-8 * (np.abs(od2[mask]) + scale))
+
+
+AA_EXE = os.path.join(HERE, "c_abi", "aa_demo")
+
+
+def test_plain_c_batched_20_state_matches_oracle(tmp_path):
+    # the batched 20-state pruning driven from C (tests/c_abi/aa_demo.c):
+    # per-tree lnL within the BF16 tolerance of the FP64 oracle, and a second
+    # run of the same batch bit-identical to the first
+    from phylocaml_amd import aa
+
+    if torch.cuda.device_count() == 0:
+        pytest.skip("no GPU")
+    if not os.path.exists(AA_EXE):
+        pytest.skip("tests/c_abi/aa_demo not built (make -C tests/c_abi)")
+    if torch.cuda.is_initialized():
+        pytest.skip("start this module before anything in the process initialises the GPU")
+    rng = np.random.default_rng(17)
+    shapes = [(2, 40), (33, 900), (70, 317), (129, 200), (17, 1)]
+    trees = [aa.synth_tree(4000 + k, N, P, n_sim=150) for k, (N, P) in enumerate(shapes)]
+    trees[2].tips[:, ::6] = 255
+    R = len(trees[0].rates)
+    cat = np.concatenate
+    blob = b"".join([
+        struct.pack("<3q", len(trees), R, 4),
+        np.array([t.n_leaves for t in trees], "<i4").tobytes(),
+        np.array([t.n_patterns for t in trees], "<i8").tobytes(),
+        np.array([t.root for t in trees], "<i4").tobytes(),
+        cat([np.asarray(t.children, "<i4") for t in trees]).tobytes(),
+        cat([np.asarray(t.blen, "<f8") for t in trees]).tobytes(),
+        cat([np.asarray(t.pi, "<f8") for t in trees]).tobytes(),
+        cat([np.asarray(t.q, "<f8").ravel() for t in trees]).tobytes(),
+        cat([np.asarray(t.rates, "<f8") for t in trees]).tobytes(),
+        cat([np.asarray(t.probs, "<f8") for t in trees]).tobytes(),
+        cat([np.ascontiguousarray(t.tips, np.uint8).ravel() for t in trees]).tobytes(),
+        cat([np.asarray(t.weights, "<u4") for t in trees]).tobytes()])
+    fin, fout = tmp_path / "aa_in.bin", tmp_path / "aa_out.bin"
+    fin.write_bytes(blob)
+    print(f"aa_demo trees={len(trees)} seed=17", flush=True)
+    r = subprocess.run([AA_EXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    b = fout.read_bytes()
+    T = len(trees)
+    pn = int(np.frombuffer(b, "<i8", 1)[0])
+    lnl = np.frombuffer(b, "<f8", 2 * T, 8).reshape(2, T)
+    assert pn == sum(t.n_patterns * (t.n_leaves - 1) for t in trees)
+    assert np.array_equal(lnl[0], lnl[1])
+    for k, t in enumerate(trees):
+        ov, _ = O.lnl(t.n_leaves, t.children, t.root, t.blen, t.pi, t.q, t.rates, t.probs,
+                      aa.tips_bitmask(t.tips), t.weights)
+        assert abs(lnl[0, k] - ov) <= 2e-3 * abs(ov), (k, lnl[0, k], ov)
