@@ -101,14 +101,29 @@ def bench_aa(torch, ctx, n_trees, steps):
     tb = aa.TunedBatch(ctx, trees, tune)
     ms = time_steps(torch, tb.run, steps, 1)
     got = tb.lnl.cpu().numpy()
-    # error against the FP64 oracle on the smallest trees (CPU time bounded)
+    # error against the FP64 oracle on the 8 smallest trees and, below, the 3
+    # nearest the median size (CPU time bounded)
     order = np.argsort([t.n_leaves * t.n_patterns for t in trees])[:8]
     errs = []
+    threads = min(16, os.cpu_count() or 1)
     for i in order:
         t = trees[i]
         ov, _ = O.lnl(t.n_leaves, t.children, t.root, t.blen, t.pi, t.q, t.rates, t.probs,
-                      aa.tips_bitmask(t.tips), t.weights, threads=min(16, os.cpu_count() or 1))
+                      aa.tips_bitmask(t.tips), t.weights, threads=threads)
         errs.append(abs(got[i] - ov) / abs(ov))
+    # CPU baseline: the oracle on the 3 trees nearest the median size
+    size = np.array([t.n_patterns * (t.n_leaves - 1) for t in trees])
+    mid = np.argsort(np.abs(size - np.median(size)))[:3]
+    t_cpu = 0.0
+    for i in mid:
+        t = trees[i]
+        tips = aa.tips_bitmask(t.tips)
+        c0 = time.perf_counter()
+        ov, _ = O.lnl(t.n_leaves, t.children, t.root, t.blen, t.pi, t.q, t.rates, t.probs,
+                      tips, t.weights, threads=threads)
+        t_cpu += time.perf_counter() - c0
+        errs.append(abs(got[i] - ov) / abs(ov))
+    pn_cpu = int(size[mid].sum())
     pn = tb.pattern_nodes
     useful_tf = 6400.0 * pn / (ms / 1e3) / 1e12
     executed_tf = 16384.0 * pn / (ms / 1e3) / 1e12
@@ -118,7 +133,12 @@ def bench_aa(torch, ctx, n_trees, steps):
            "pattern_nodes": pn, "ms": ms, "pattern_node_per_s": pn / (ms / 1e3),
            "autotune": tune,
            "rel_err_vs_fp64_oracle": {"max": float(max(errs)), "mean": float(np.mean(errs)),
-                                      "trees": [int(i) for i in order]},
+                                      "trees": [int(i) for i in order] + [int(i) for i in mid]},
+           "cpu_baseline": {"value": pn_cpu / t_cpu, "unit": "pattern.node updates/s",
+                            "cores": threads, "kind": "port",
+                            "sample": f"oracle FP64 20-state pruning of the 3 trees nearest "
+                                      f"the batch's median size ({pn_cpu:.3e} pattern.nodes, "
+                                      f"{t_cpu:.2f}s), {threads} OpenMP threads"},
            "roofline": {"bound": "mfma", "traffic": None, "achieved": useful_tf, "peak": 2500.0,
                         "unit": "TFLOP/s", "frac": useful_tf / 2500.0,
                         "executed_tflops": executed_tf,
@@ -235,6 +255,28 @@ def main():
         }
         log(f"  fitch down {ms_down:.3f} ms ({gbs_down:.0f} GB/s alg), "
             f"up {ms_up:.3f} ms ({gbs_up:.0f} GB/s alg), total {total}")
+        if not args.no_cpu_baseline:
+            from oracle import oracle as O
+
+            # the same leaves through the oracle
+            S_cpu = S
+            thr = min(16, os.cpu_count() or 1)
+            leaves = D.unpack_nibbles(lv[:, : D.words(S_cpu)], S_cpu)
+            c0 = time.perf_counter()
+            osets, _, _, _ = O.fitch_downpass(NF, ftree.children, ftree.root, leaves, threads=thr)
+            t_down = time.perf_counter() - c0
+            c0 = time.perf_counter()
+            O.fitch_uppass(NF, ftree.children, ftree.root, osets)
+            t_up = time.perf_counter() - c0
+            fitch["cpu_baseline"] = {
+                "value": S_cpu * (NF - 1) / t_down, "unit": "site.node/s (down-pass)",
+                "cores": thr, "kind": "port",
+                "uppass_value": S_cpu * (2 * NF - 1) / t_up, "uppass_cores": 1,
+                "sample": f"oracle Fitch on the whole workload ({S_cpu:,} sites): "
+                          f"down-pass {t_down:.2f}s on {thr} OpenMP threads, up-pass "
+                          f"{t_up:.2f}s on 1 thread"}
+            log(f"  fitch cpu: down {fitch['cpu_baseline']['value']:.3e}/s ({thr} thr), "
+                f"up {fitch['cpu_baseline']['uppass_value']:.3e}/s (1 thr)")
         del ft
         torch.cuda.empty_cache()
 
