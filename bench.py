@@ -85,6 +85,62 @@ def time_steps(torch, fn, steps, warmup):
     return s.elapsed_time(e) / steps  # ms per step
 
 
+def bench_spr(torch, ctx, cpu_baseline: bool):
+    """Exact scores of SPR neighbours (the learned-proposal workload: 256-taxon
+    tree, 50,000 GTR+G4 patterns): GPU exact evaluations per second over an
+    even sample of 4,096 of the round's moves, FP64 and FP32-scaled; beside
+    it the FP64 oracle on 4 of the moved trees, which must agree."""
+    from phylocaml_amd import spr, spr_search as SS
+
+    inst = SS.make_instance(7, 256, 50_000, 10)
+    c = SS.Context(ctx, inst)
+    mv = spr.moves(inst.N, inst.children, inst.root)
+    sample = mv[:: max(1, len(mv) // 4096)][:4096]
+    rates, scores = {}, {}
+    for name, fp64 in (("f64", True), ("f32", False)):
+        c.exact(inst.children, inst.root, inst.blen, sample[:256], fp64=fp64)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        scores[name] = c.exact(inst.children, inst.root, inst.blen, sample, fp64=fp64)
+        torch.cuda.synchronize()
+        rates[name] = len(sample) / (time.perf_counter() - t0)
+    res = {"workload": "256-taxon Yule tree, 50,000 unique GTR+G4 patterns; exact lnL of "
+                       f"{len(sample)} of the {len(mv)} rooted SPR neighbours (even sample)",
+           "moves_per_round": int(len(mv)), "exact_evaluations_per_s": rates}
+    log(f"  spr: {len(mv)} moves; exact evaluations/s f64 {rates['f64']:.0f}, "
+        f"f32 {rates['f32']:.0f}")
+    if cpu_baseline:
+        from oracle import oracle as O
+
+        threads = min(16, os.cpu_count() or 1)
+        tips = D_unpack_dna(inst)
+        t_cpu, worst = 0.0, 0.0
+        for k in range(0, len(sample), len(sample) // 4)[:4]:
+            s_, v_ = (int(x) for x in sample[k])
+            ch, r, bl = spr.apply(inst.N, inst.children, inst.root, inst.blen, s_, v_)
+            m = inst.model
+            c0 = time.perf_counter()
+            ov, _ = O.lnl(inst.N, ch, r, bl, m.priors, m.q, m.rates, m.probs, tips,
+                          inst.weights, threads=threads)
+            t_cpu += time.perf_counter() - c0
+            worst = max(worst, abs(scores["f64"][k] - ov) / abs(ov))
+        res["rel_err_f64_vs_oracle"] = worst
+        res["cpu_baseline"] = {"value": 4 / t_cpu, "unit": "exact evaluations/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"oracle FP64 lnL of 4 moved trees ({t_cpu:.2f}s), "
+                                         f"{threads} OpenMP threads"}
+        log(f"  spr cpu: {4 / t_cpu:.2f} exact evaluations/s ({threads} thr), "
+            f"max rel err {worst:.1e}")
+    return res
+
+
+def D_unpack_dna(inst):
+    """uint32 state bitmasks [N, P] of an instance's packed 4-bit tips."""
+    from phylocaml_amd import device as D
+
+    return D.unpack_nibbles(inst.tips_np, inst.P).astype(np.uint32)
+
+
 def bench_aa(torch, ctx, n_trees, steps):
     """Batched 20-state pruning: n_trees trees, taxa ~ U{16..256}, patterns
     ~ U{1000..50000}, one persistent BF16-MFMA launch per size bucket with
@@ -156,7 +212,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["all", "prune", "fitch", "aa"], default="all")
+    ap.add_argument("--only", choices=["all", "prune", "fitch", "aa", "spr"], default="all")
     ap.add_argument("--aa-trees", type=int, default=2048)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scale for HBM counters)")
@@ -285,6 +341,11 @@ def main():
     if args.only in ("all", "aa"):
         aa_res = bench_aa(torch, ctx, args.aa_trees, max(2, steps // 5))
 
+    # ------------------------------------------------------------- exact SPR scores
+    spr_res = None
+    if args.only in ("all", "spr"):
+        spr_res = bench_spr(torch, ctx, not args.no_cpu_baseline)
+
     # ------------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu_baseline:
@@ -359,6 +420,7 @@ def main():
         },
         "fitch": fitch,
         "aa": aa_res,
+        "spr": spr_res,
     }
     print(json.dumps(line), flush=True)
 
