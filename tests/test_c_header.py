@@ -1,6 +1,6 @@
 """The public headers are plain C: a C99 translation unit including both
 compiles with -Wall -Wextra -Werror -pedantic, and so do the C consumers of
-the ABI (tests/c_abi/abi_demo.c, tests/c_abi/aa_demo.c)."""
+the ABI (tests/c_abi/*.c)."""
 import os
 import shutil
 import subprocess
@@ -28,7 +28,7 @@ def test_headers_compile_as_strict_c99(gcc, tmp_path):
     assert r.returncode == 0, r.stderr
 
 
-@pytest.mark.parametrize("name", ["abi_demo.c", "aa_demo.c"])
+@pytest.mark.parametrize("name", ["abi_demo.c", "aa_demo.c", "sankoff_demo.c"])
 def test_c_consumers_compile(gcc, name):
     r = subprocess.run([gcc, *FLAGS, os.path.join(ROOT, "tests", "c_abi", name)],
                        capture_output=True, text=True)

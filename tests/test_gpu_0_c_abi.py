@@ -156,3 +156,37 @@ def test_plain_c_batched_20_state_matches_oracle(tmp_path):
         ov, _ = O.lnl(t.n_leaves, t.children, t.root, t.blen, t.pi, t.q, t.rates, t.probs,
                       aa.tips_bitmask(t.tips), t.weights)
         assert abs(lnl[0, k] - ov) <= 2e-3 * abs(ov), (k, lnl[0, k], ov)
+
+
+SK_EXE = os.path.join(HERE, "c_abi", "sankoff_demo")
+
+
+def test_plain_c_sankoff_matches_oracle(tmp_path):
+    # the whole-tree Sankoff pass driven from C (tests/c_abi/sankoff_demo.c):
+    # per-site costs and the weighted total bit for bit
+    if torch.cuda.device_count() == 0:
+        pytest.skip("no GPU")
+    if not os.path.exists(SK_EXE):
+        pytest.skip("tests/c_abi/sankoff_demo not built (make -C tests/c_abi)")
+    if torch.cuda.is_initialized():
+        pytest.skip("start this module before anything in the process initialises the GPU")
+    rng = np.random.default_rng(23)
+    N, S, K = 150, 777, 5
+    ch, root = random_tree(rng, N)
+    m = rng.integers(1, 20, (K, K))
+    tcm = (np.triu(m, 1) + np.tril(m, -1) * 2).astype(np.int32)  # asymmetric, zero diagonal
+    leaves = (1 << rng.integers(0, K, (N, S))).astype(np.uint8)
+    leaves[rng.random((N, S)) < 0.05] |= 3
+    leaves[rng.random((N, S)) < 0.01] = 0  # missing: every state
+    w = rng.integers(1, 6, S).astype(np.uint32)
+    fin, fout = tmp_path / "sk_in.bin", tmp_path / "sk_out.bin"
+    fin.write_bytes(struct.pack("<4q", N, root, S, K) + np.ascontiguousarray(ch, "<i4").tobytes()
+                    + tcm.astype("<i4").tobytes() + leaves.tobytes() + w.astype("<u4").tobytes())
+    print(f"sankoff_demo N={N} S={S} K={K}", flush=True)
+    r = subprocess.run([SK_EXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    b = fout.read_bytes()
+    total = int(np.frombuffer(b, "<u8", 1)[0])
+    sc = np.frombuffer(b, "<u4", S, 8)
+    osc, otot = O.sankoff(N, ch, root, tcm, leaves, w)
+    assert total == otot and np.array_equal(sc, osc)
