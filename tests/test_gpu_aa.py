@@ -115,3 +115,28 @@ def test_large_trees_cross_leaf_id_chunks(ctx, waves):
         rel = abs(got[k] - ov) / abs(ov)
         print(f"tree {k}: N={t.n_leaves} P={t.n_patterns} rel {rel:.2e}", flush=True)
         assert rel <= 2e-3, (k, got[k], ov)
+
+
+@pytest.mark.parametrize("R", [1, 2])
+@pytest.mark.parametrize("waves", [1, 4, 8])
+def test_one_and_two_rate_classes(ctx, R, waves):
+    # the R = 1 and R = 2 kernels (the other tests use four Gamma classes)
+    from dataclasses import replace
+
+    from phylocaml_amd import aa
+
+    print(f"aa rate classes R={R} waves={waves}", flush=True)
+    rates = {1: [1.0], 2: [0.35, 1.65]}[R]
+    rng = np.random.default_rng(31 + R)
+    trees = []
+    for k in range(6):
+        t = aa.synth_tree(5000 + 10 * R + k, int(rng.integers(2, 150)), int(rng.integers(1, 600)),
+                          n_sim=120)
+        trees.append(replace(t, rates=np.array(rates), probs=np.full(R, 1.0 / R)))
+    b = aa.AABatch(ctx, trees, waves=waves)
+    b.run()
+    ctx.sync()
+    got = b.lnl.cpu().numpy()
+    for k, t in enumerate(trees):
+        ov = _oracle(t)
+        assert abs(got[k] - ov) <= 2e-3 * abs(ov), (k, got[k], ov)
