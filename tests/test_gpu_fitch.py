@@ -97,7 +97,7 @@ def test_tree_passes_match_oracle(ctx, rng, N, S):
         _tree_case(ctx, N, ch, root, leaves, w)
 
 
-@pytest.mark.parametrize("words", ["1", "2"])
+@pytest.mark.parametrize("words", ["1", "2", "4"])
 @pytest.mark.parametrize("batch", ["4", "8", "16", "32"])
 @pytest.mark.parametrize("split", ["0", "1"])
 def test_prefetch_depth_and_split_forms(ctx, rng, monkeypatch, batch, split, words):
@@ -111,6 +111,15 @@ def test_prefetch_depth_and_split_forms(ctx, rng, monkeypatch, batch, split, wor
     ch, root = random_tree(rng, N)
     leaves = rng.integers(1, 16, (N, S)).astype(np.uint8)
     w = rng.integers(1, 5, S).astype(np.uint32)
+    if batch == "32" and words == "4":
+        # (2 x 32 + depth) sets of 16 B per thread exceed one block's LDS even
+        # at 64 threads: the launch is refused with PHY_ERANGE
+        from phylocaml_amd._lib import PhyError
+
+        with pytest.raises(PhyError) as e:
+            _tree_case(ctx, N, ch, root, leaves, w)
+        assert e.value.code == 5
+        return
     _tree_case(ctx, N, ch, root, leaves, w)
 
 

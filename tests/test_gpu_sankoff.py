@@ -42,7 +42,7 @@ def _run(ctx, N, ch, root, tcm, leaves, w, stride=None):
     return sc.cpu().numpy()[:S], int(tot.cpu().numpy()[0])
 
 
-@pytest.mark.parametrize("K", [2, 3, 4, 5, 8])
+@pytest.mark.parametrize("K", [2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("N,S", [(2, 1), (3, 63), (17, 64), (64, 65), (300, 5000)])
 def test_sankoff_matches_oracle(ctx, rng, K, N, S):
     for kind in ("random", "caterpillar"):
