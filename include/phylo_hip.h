@@ -282,7 +282,9 @@ int phy_gtr_q(int32_t n, const double* pi, const double* co, double* Q);
 /* ------------------------------------------------------------------------- */
 /* Likelihood model on the device: MlModel.t (lib/mlModel.ml:53-63) reduced  */
 /* to what pruning needs: pi, the eigen-decomposition of Q, Gamma rates and  */
-/* their probabilities.  n_states K in {4, 20}; n_rates R in 1..8.           */
+/* their probabilities.  n_states K in {4, 20}; n_rates R in 1..8 (the      */
+/* pruning passes take R in {1, 2, 4, 8}, the 20-state batch R in {1, 2, 4};  */
+/* other counts give PHY_ERANGE there).                                      */
 /* ------------------------------------------------------------------------- */
 int phy_model_create(phy_ctx* ctx, int32_t n_states, int32_t n_rates,
                      const double* pi, const double* Q, const double* rates,
