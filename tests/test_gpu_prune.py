@@ -237,3 +237,21 @@ def test_argument_errors(ctx, rng):
                               D.dptr(buf), D.dptr(buf), D.dptr(ev.lnl), None, None, None,
                               D.dptr(ev.work), 16)
     assert rc == 1  # workspace too small
+
+
+def test_unsupported_rate_class_count_is_refused(ctx, rng):
+    # three rate classes: the model is valid (1..8), the pruning passes take
+    # 1, 2, 4 or 8 and refuse the evaluation with PHY_ERANGE before launching
+    from phylocaml_amd import _lib, device as D, likelihood as LK
+
+    N, P = 5, 10
+    ch, root = caterpillar(N)
+    topo = D.Topology(N, ch, root, ctx)
+    dm = LK.DeviceModel(ctx, _model(rng, 3))
+    tips = D.to_device(D.pack_nibbles(_tips(rng, N, P)), ctx.device)
+    w = D.to_device(np.ones(P, np.uint32), ctx.device)
+    bl = D.to_device(np.full(2 * N - 1, 0.1), ctx.device)
+    with pytest.raises(_lib.PhyError) as e:
+        ev = LK.Evaluator(ctx, topo, dm, P, F64)
+        ev.run(tips, D.words(P), w, bl)
+    assert e.value.code == 5
