@@ -222,8 +222,9 @@ def main():
                     help="only run a 1 GiB device-to-device copy (scale for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): every path "
+              "here runs on a single GPU; sharding across GPUs is not implemented",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
