@@ -49,7 +49,7 @@ class Instance:
 
 def make_instance(seed: int, N: int = 256, P: int = 50_000, n_perturb: int = 10,
                   mean_bl: float = 0.05) -> Instance:
-    """A config-2-style instance: Yule tree, GTR+G4 (alpha 0.5), P unique
+    """A pruning-benchmark-style instance: Yule tree, GTR+G4 (alpha 0.5), P unique
     simulated patterns with weights Poisson(3)+1; the start tree is the true
     tree after n_perturb random SPR moves."""
     rng = np.random.default_rng(seed)

@@ -138,14 +138,14 @@ def test_split_part_counts_as_bytes_or_words(ctx, rng, monkeypatch, N, narrow):
     _tree_case(ctx, N, ch, root, leaves, w)
 
 
-def test_config1_full_size_bit_exact(ctx):
+def test_1024_taxa_1e6_sites_bit_exact(ctx):
     """1,024-taxon Yule tree, 10^6 IUPAC sites (the parsimony benchmark
     workload): every preliminary and final set, every site cost and the total
     against the oracle."""
     from phylocaml_amd import device as D, nonadditive as NA, synth
 
     N, S = 1024, 1_000_000
-    _say("config-1 full size: generating")
+    _say("1,024 taxa x 10^6 sites: generating")
     tree = synth.yule(N, 1, 0.1)
     lv = synth.fitch_leaves(N, S, 1)
     topo = D.Topology(N, tree.children, tree.root, ctx)
@@ -153,7 +153,7 @@ def test_config1_full_size_bit_exact(ctx):
     NA.downpass(ctx, topo, ft)
     NA.uppass(ctx, topo, ft)
     ctx.sync()
-    _say("config-1 full size: oracle")
+    _say("1,024 taxa x 10^6 sites: oracle")
     t0 = time.time()
     leaves = D.unpack_nibbles(lv, S)
     osets, osc, onc, otot = O.fitch_downpass(N, tree.children, tree.root, leaves,
@@ -166,7 +166,7 @@ def test_config1_full_size_bit_exact(ctx):
     assert np.array_equal(ft.sets.cpu().numpy()[N:], D.pack_nibbles(osets[N:]))
     ofin = O.fitch_uppass(N, tree.children, tree.root, osets)
     assert np.array_equal(ft.final.cpu().numpy(), D.pack_nibbles(ofin))
-    _say(f"config-1 total cost {otot}")
+    _say(f"1,024 taxa x 10^6 sites: total cost {otot}")
 
 
 def test_empty_and_bad_arguments(ctx):

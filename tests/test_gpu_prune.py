@@ -193,14 +193,14 @@ def test_node_api_matches_tree(ctx, rng):
         assert LK.root_cost(ctx, dm, rt) == -v
 
 
-def test_config2_full_size(ctx):
+def test_512_taxa_200k_patterns_full_size(ctx):
     """512-taxon Yule tree, 200,000 unique GTR+G4 patterns (the pruning
     benchmark workload): FP64 lnL and derivatives vs the oracle, FP32-scaled
     lnL error vs FP64."""
     from phylocaml_amd import device as D, likelihood as LK, mlmodel as MM, synth
 
     N, P = 512, 200_000
-    _say("config-2: generating")
+    _say("512 taxa x 200,000 patterns: generating")
     tree = synth.yule(N, 2, 0.05)
     pi, co = synth.gtr_params(4, 5.0, 2)
     spec = MM.Spec(4, ("GTR", co), ("DiscreteGamma", 4, 0.5), ("Empirical", pi))
@@ -208,10 +208,10 @@ def test_config2_full_size(ctx):
     st = synth.simulate(tree, model.priors, model.q, model.rates, P, 2, unique=True)
     w = synth.poisson_weights(P, 3.0, 2)
     tips = np.where(st < 4, 1 << np.minimum(st, 3), 15).astype(np.uint8)
-    _say("config-2: GPU")
+    _say("512 taxa x 200,000 patterns: GPU")
     g64 = _run(ctx, N, tree.children, tree.root, tree.blen, model, tips, w, F64, True)
     g32 = _run(ctx, N, tree.children, tree.root, tree.blen, model, tips, w, F32 | FUSED, False)
-    _say("config-2: oracle")
+    _say("512 taxa x 200,000 patterns: oracle")
     ov, osl, od1, od2 = _oracle(N, tree.children, tree.root, tree.blen, model, tips, w, True)
     _say(f"lnL oracle {ov!r} fp64 {g64['lnl']!r} fp32 {g32['lnl']!r}")
     assert abs(g64["lnl"] - ov) <= 1e-10 * abs(ov)
