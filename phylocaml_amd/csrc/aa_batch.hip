@@ -70,35 +70,41 @@ __host__ __device__ __forceinline__ int rho(int b, int i) {
   return 8 * q + 4 * b + v;
 }
 
-// --- P operands: one workgroup of 64 threads per (tree, branch, rate) ---
-__global__ __launch_bounds__(64) void k_aa_pops(const double* __restrict__ eig,  // [T][U 400|lam 20|Ui 400]
-                                                const double* __restrict__ rates,  // [T][R]
-                                                const double* __restrict__ blen,   // flat
-                                                const int64_t* __restrict__ blen_off,
-                                                const int32_t* __restrict__ br_tree,  // flat branch -> tree
-                                                int R, const int64_t* __restrict__ pop_off,
-                                                uint4* __restrict__ pops) {
-  const int64_t gb = blockIdx.x / R;  // flat branch index
-  const int r = blockIdx.x % R;
+// --- P operands: one wave per (tree, branch, rate), four to a workgroup ---
+constexpr int kPopsPerBlock = 4;
+
+__global__ __launch_bounds__(64 * kPopsPerBlock) void k_aa_pops(
+    const double* __restrict__ eig,  // [T][U 400|lam 20|Ui 400]
+    const double* __restrict__ rates,  // [T][R]
+    const double* __restrict__ blen,   // flat
+    const int64_t* __restrict__ blen_off,
+    const int32_t* __restrict__ br_tree,  // flat branch -> tree
+    int R, int64_t n_pairs, const int64_t* __restrict__ pop_off, uint4* __restrict__ pops) {
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int64_t pair = (int64_t)blockIdx.x * kPopsPerBlock + wv;
+  const bool valid = pair < n_pairs;
+  const int64_t pc = valid ? pair : n_pairs - 1;  // an idle wave reads in bounds, writes nothing
+  const int64_t gb = pc / R;  // flat branch index
+  const int r = (int)(pc % R);
   const int t = br_tree[gb];
   const int64_t b = gb - blen_off[t];
   const double* U = eig + (size_t)t * (2 * KAA * KAA + KAA);
   const double* lam = U + KAA * KAA;
   const double* Ui = lam + KAA;
   const double rt = rates[(size_t)t * R + r] * blen[gb];
-  __shared__ double ex[KAA];
-  __shared__ double P[KAA * KAA];
-  const int l = threadIdx.x;
-  if (l < KAA) ex[l] = exp(lam[l] * rt);
+  __shared__ double ex[kPopsPerBlock][KAA];
+  __shared__ double P[kPopsPerBlock][KAA * KAA];
+  if (l < KAA) ex[wv][l] = exp(lam[l] * rt);
   __syncthreads();
   for (int e = l; e < KAA * KAA; e += 64) {
     const int i = e / KAA, j = e % KAA;
     double s = 0.0;
-    for (int m = 0; m < KAA; ++m) s += U[i * KAA + m] * ex[m] * Ui[m * KAA + j];
+    for (int m = 0; m < KAA; ++m) s += U[i * KAA + m] * ex[wv][m] * Ui[m * KAA + j];
     if (!(rt >= 1e-10)) s = (i == j) ? 1.0 : 0.0;  // compose rule, lib/mlmodel.c:339-341
-    P[e] = s;
+    P[wv][e] = s;
   }
   __syncthreads();
+  if (!valid) return;
   uint4* dst = pops + pop_off[t] + ((b * R + r) * 2) * OPB;
   for (int blk = 0; blk < 2; ++blk) {
     // lane l: output row l%16 -> state rho(blk, l%16); k = 8*(l/16) + v
@@ -107,7 +113,7 @@ __global__ __launch_bounds__(64) void k_aa_pops(const double* __restrict__ eig, 
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
       const int k = 8 * (l >> 4) + v;
-      const float val = (srow < KAA && k < KAA) ? (float)P[srow * KAA + k] : 0.0f;
+      const float val = (srow < KAA && k < KAA) ? (float)P[wv][srow * KAA + k] : 0.0f;
       a[v] = (__bf16)val;
     }
     dst[blk * OPB + l] = *reinterpret_cast<uint4*>(&a);
@@ -693,9 +699,10 @@ int phy_aa_batch_run(phy_ctx* ctx, phy_aa_batch* b, double* d_lnl) {
   if (!ctx || !b || !d_lnl) return PHY_EINVAL;
   if (b->ctx != ctx) return set_err(ctx, PHY_EINVAL, "aa_batch: built on another context");
   // 1. P operands for every (tree, branch, rate)
-  const int64_t nwg = b->n_branches * b->R;
-  hipLaunchKernelGGL(k_aa_pops, dim3((unsigned)nwg), dim3(64), 0, ctx->stream, b->eig, b->rates, b->blen,
-                     b->blen_off, b->br_tree, b->R, b->pop_off, b->pops);
+  const int64_t n_pairs = b->n_branches * b->R;
+  const int64_t nwg = (n_pairs + kPopsPerBlock - 1) / kPopsPerBlock;
+  hipLaunchKernelGGL(k_aa_pops, dim3((unsigned)nwg), dim3(64 * kPopsPerBlock), 0, ctx->stream, b->eig,
+                     b->rates, b->blen, b->blen_off, b->br_tree, b->R, n_pairs, b->pop_off, b->pops);
   PHY_LAUNCHED(ctx);
   // 2. the persistent pruning launch
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, ctx->stream, b->counter);
