@@ -288,3 +288,17 @@ def test_ptopology_store_is_functional():
     assert t3.get_comp_root(0).assignment == {} and 7 in t2.get_comp_root(0).assignment
     assert t3.node_data == {} and t3.root_data == {} and t2.get_node_data(3) == "a"
     assert t3.remove_comp_root(0).comp_root == {}
+
+
+def test_entry_points_refuse_a_null_context():
+    # argument checks that run before any device work: a null context is
+    # PHY_EINVAL for the per-node Fitch gather and the whole-tree passes
+    import ctypes as C
+
+    from phylocaml_amd._lib import lib, ptr
+
+    idx = np.zeros(4, np.int64)
+    assert lib.phy_fitch_select(None, None, 16, None, ptr(idx, C.c_int64), 4, None, None) == 1
+    assert lib.phy_fitch_downpass(None, None, None, 1, 16, None, None, None, None) == 1
+    assert lib.phy_fitch_uppass(None, None, None, None, 1, 16) == 1
+    assert lib.phy_sankoff_downpass(None, None, 4, None, None, 16, 16, None, None, None) == 1
