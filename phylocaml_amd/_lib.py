@@ -1,7 +1,7 @@
 """ctypes binding of libphylo_hip.so (the C ABI in include/phylo_hip.h and
 include/phylo_synth.h).
 
-The library is built in-tree by ``phylocaml_amd.build`` (or
+The library is built in-tree by ``phylocaml_amd/build.py`` (or
 ``__graft_entry__.build()``).  There is no fallback: if the shared library is
 missing, importing this module raises ImportError naming the build command.
 """
@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libphylo_hip.so")
 if not os.path.exists(LIB_PATH):
     raise ImportError(
         f"{LIB_PATH} not found: build the HIP library first "
-        "(python -m phylocaml_amd.build)"
+        "(python phylocaml_amd/build.py)"
     )
 
 # torch brings its own HIP runtime; load it first so that the library binds to

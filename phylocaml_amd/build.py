@@ -1,6 +1,7 @@
 """Build the gfx950 shared library libphylo_hip.so in-tree with hipcc.
 
-Usage: python -m phylocaml_amd.build   (or phylocaml_amd.build.build()).
+Usage: python phylocaml_amd/build.py  (run by path: importing the package
+needs the library this script builds).
 Every .hip and .cpp file under csrc/ goes into one shared library whose
 exported C ABI is declared in include/phylo_hip.h and include/phylo_synth.h.
 """
