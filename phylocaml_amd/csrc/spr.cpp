@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "../../include/phylo_hip.h"
@@ -29,19 +30,53 @@ struct Tree {
   int32_t root;
 };
 
+// A rooted binary tree of N leaves: every node but the root has exactly one
+// parent, and every node reaches the root (no node is its own child and no
+// cycle is detached from the root).
 bool load(int32_t N, const int32_t* children, int32_t root, Tree* t) {
   if (N < 3 || !children || root < N || root > 2 * N - 2) return false;
+  const int32_t M = 2 * N - 1;
   t->N = N;
   t->ch.assign(children, children + 2 * (N - 1));
-  t->parent.assign(2 * N - 1, -1);
+  t->parent.assign(M, -1);
   for (int32_t k = 0; k < N - 1; ++k)
     for (int c = 0; c < 2; ++c) {
       const int32_t x = t->ch[2 * k + c];
-      if (x < 0 || x > 2 * N - 2 || t->parent[x] >= 0) return false;
+      if (x < 0 || x >= M || x == N + k || t->parent[x] >= 0) return false;
       t->parent[x] = N + k;
     }
   t->root = root;
-  return t->parent[root] < 0;
+  if (t->parent[root] >= 0) return false;
+  // walk up from every node, marking nodes known to reach the root: each
+  // node is walked over at most once after it is marked, so O(M) overall
+  std::vector<char> ok(M, 0);
+  ok[root] = 1;
+  std::vector<int32_t> path;
+  for (int32_t v = 0; v < M; ++v) {
+    path.clear();
+    int32_t y = v;
+    while (!ok[y]) {
+      path.push_back(y);
+      if ((int32_t)path.size() > M) return false;  // a cycle
+      y = t->parent[y];
+      if (y < 0) return false;  // a second root
+    }
+    for (int32_t z : path) ok[z] = 1;
+  }
+  return true;
+}
+
+// C entry points return a status and never let an exception out: an
+// allocation failure is PHY_ENOMEM, anything else PHY_EINVAL.
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return PHY_ENOMEM;
+  } catch (...) {
+    return PHY_EINVAL;
+  }
 }
 
 // mark the subtree of s
@@ -163,19 +198,19 @@ void prune_distances(const Tree& t, int32_t s, std::vector<int32_t>* dist) {
     if (!in[y] && y != r) (*dist)[y] = ed[key(y)];
 }
 
-}  // namespace
+int phy_spr_apply_impl(int32_t N, const int32_t* children, int32_t root, const double* blen,
+                       int32_t s, int32_t v, int32_t* children_out, int32_t* root_out,
+                       double* blen_out);
 
-extern "C" {
-
-int phy_tbr_apply(int32_t N, const int32_t* children, int32_t root, const double* blen, int32_t s,
-                  int32_t w, int32_t v, int32_t* children_out, int32_t* root_out,
-                  double* blen_out) {
+int phy_tbr_apply_impl(int32_t N, const int32_t* children, int32_t root, const double* blen, int32_t s,
+                       int32_t w, int32_t v, int32_t* children_out, int32_t* root_out,
+                       double* blen_out) {
   Tree t;
   if (!children_out || !root_out || !blen || !blen_out || !load(N, children, root, &t))
     return PHY_EINVAL;
   const int32_t M = 2 * N - 1;
   if (s < 0 || s >= M || s == root || w < 0 || w >= M) return PHY_EINVAL;
-  if (w == s) return phy_spr_apply(N, children, root, blen, s, v, children_out, root_out, blen_out);
+  if (w == s) return phy_spr_apply_impl(N, children, root, blen, s, v, children_out, root_out, blen_out);
   std::vector<char> in;
   mark_subtree(t, s, &in);
   if (!in[w] || s < N) return PHY_EINVAL;
@@ -188,11 +223,11 @@ int phy_tbr_apply(int32_t N, const int32_t* children, int32_t root, const double
     *root_out = root;
     return PHY_OK;
   }
-  return phy_spr_apply(N, ch.data(), root, bl.data(), s, v, children_out, root_out, blen_out);
+  return phy_spr_apply_impl(N, ch.data(), root, bl.data(), s, v, children_out, root_out, blen_out);
 }
 
-int phy_tbr_moves(int32_t N, const int32_t* children, int32_t root, int32_t max_join,
-                  int32_t max_reroot, int64_t* n_moves, int32_t* moves, int64_t capacity) {
+int phy_tbr_moves_impl(int32_t N, const int32_t* children, int32_t root, int32_t max_join,
+                       int32_t max_reroot, int64_t* n_moves, int32_t* moves, int64_t capacity) {
   Tree t;
   if (!n_moves || !load(N, children, root, &t)) return PHY_EINVAL;
   const int32_t M = 2 * N - 1;
@@ -245,8 +280,8 @@ int phy_tbr_moves(int32_t N, const int32_t* children, int32_t root, int32_t max_
   return PHY_OK;
 }
 
-int phy_nni_moves(int32_t N, const int32_t* children, int32_t root, int64_t* n_moves,
-                  int32_t* moves, int64_t capacity) {
+int phy_nni_moves_impl(int32_t N, const int32_t* children, int32_t root, int64_t* n_moves,
+                       int32_t* moves, int64_t capacity) {
   Tree t;
   if (!n_moves || !load(N, children, root, &t)) return PHY_EINVAL;
   int64_t n = 0;
@@ -278,8 +313,8 @@ int phy_nni_moves(int32_t N, const int32_t* children, int32_t root, int64_t* n_m
   return PHY_OK;
 }
 
-int phy_spr_moves(int32_t N, const int32_t* children, int32_t root, int64_t* n_moves,
-                  int32_t* moves, int64_t capacity) {
+int phy_spr_moves_impl(int32_t N, const int32_t* children, int32_t root, int64_t* n_moves,
+                       int32_t* moves, int64_t capacity) {
   Tree t;
   if (!n_moves || !load(N, children, root, &t)) return PHY_EINVAL;
   int64_t n = 0;
@@ -303,8 +338,8 @@ int phy_spr_moves(int32_t N, const int32_t* children, int32_t root, int64_t* n_m
   return PHY_OK;
 }
 
-int phy_spr_apply(int32_t N, const int32_t* children, int32_t root, const double* blen, int32_t s,
-                  int32_t v, int32_t* children_out, int32_t* root_out, double* blen_out) {
+int phy_spr_apply_impl(int32_t N, const int32_t* children, int32_t root, const double* blen, int32_t s,
+                       int32_t v, int32_t* children_out, int32_t* root_out, double* blen_out) {
   Tree t;
   if (!children_out || !root_out || !blen || !blen_out || !load(N, children, root, &t))
     return PHY_EINVAL;
@@ -348,6 +383,42 @@ int phy_spr_apply(int32_t N, const int32_t* children, int32_t root, const double
   std::copy(bl.begin(), bl.end(), blen_out);
   *root_out = r;
   return PHY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phy_tbr_apply(int32_t N, const int32_t* children, int32_t root, const double* blen, int32_t s,
+                  int32_t w, int32_t v, int32_t* children_out, int32_t* root_out,
+                  double* blen_out) {
+  return guarded([&] {
+    return phy_tbr_apply_impl(N, children, root, blen, s, w, v, children_out, root_out, blen_out);
+  });
+}
+
+int phy_tbr_moves(int32_t N, const int32_t* children, int32_t root, int32_t max_join,
+                  int32_t max_reroot, int64_t* n_moves, int32_t* moves, int64_t capacity) {
+  return guarded([&] {
+    return phy_tbr_moves_impl(N, children, root, max_join, max_reroot, n_moves, moves, capacity);
+  });
+}
+
+int phy_nni_moves(int32_t N, const int32_t* children, int32_t root, int64_t* n_moves,
+                  int32_t* moves, int64_t capacity) {
+  return guarded([&] { return phy_nni_moves_impl(N, children, root, n_moves, moves, capacity); });
+}
+
+int phy_spr_moves(int32_t N, const int32_t* children, int32_t root, int64_t* n_moves,
+                  int32_t* moves, int64_t capacity) {
+  return guarded([&] { return phy_spr_moves_impl(N, children, root, n_moves, moves, capacity); });
+}
+
+int phy_spr_apply(int32_t N, const int32_t* children, int32_t root, const double* blen, int32_t s,
+                  int32_t v, int32_t* children_out, int32_t* root_out, double* blen_out) {
+  return guarded([&] {
+    return phy_spr_apply_impl(N, children, root, blen, s, v, children_out, root_out, blen_out);
+  });
 }
 
 }  // extern "C"

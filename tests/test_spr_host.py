@@ -91,3 +91,52 @@ def test_treeinfo_lca_and_ancestry(rng):
         assert info.is_anc(z, x) and info.is_anc(z, y)
     sets = _leafsets(N, ch, root)
     assert all(info.size[v] == len(sets[v]) for v in range(2 * N - 1))
+
+
+def _malformed_trees():
+    """(N, children, root) triples that are not rooted binary trees: a node
+    that is its own child, a detached two-node cycle, a node with two
+    parents, and a root that is someone's child."""
+    N = 4  # internal nodes 4, 5, 6; a valid tree: 4 = (0, 1), 5 = (4, 2), 6 = (5, 3)
+    return [
+        ("self child", N, np.array([0, 1, 5, 2, 6, 3], np.int32), 6),  # 5 is its own child
+        # 4 = (5, 0) and 5 = (4, 1): a cycle no path from root 6 = (2, 3) reaches
+        ("detached cycle", N, np.array([5, 0, 4, 1, 2, 3], np.int32), 6),
+        ("two parents", N, np.array([0, 1, 4, 1, 5, 3], np.int32), 6),
+        ("root is a child", N, np.array([0, 1, 6, 2, 4, 3], np.int32), 6),
+    ]
+
+
+def test_malformed_trees_are_refused_by_every_move_entry_point():
+    # phylo_hip.h promises PHY_EINVAL for a malformed tree: no entry point may
+    # loop, exhaust memory or abort on one
+    import ctypes as C
+
+    from phylocaml_amd._lib import lib
+
+    P_i32, P_i64, P_f64 = C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_double)
+    for what, N, ch, root in _malformed_trees():
+        M = 2 * N - 1
+        chp = ch.ctypes.data_as(P_i32)
+        n = C.c_int64(0)
+        out_ch = np.zeros(2 * (N - 1), np.int32)
+        out_bl = np.zeros(M)
+        bl = np.full(M, 0.1)
+        r = C.c_int32(0)
+        rcs = {
+            "spr_moves": lib.phy_spr_moves(N, chp, root, C.byref(n), None, 0),
+            "nni_moves": lib.phy_nni_moves(N, chp, root, C.byref(n), None, 0),
+            "tbr_moves": lib.phy_tbr_moves(N, chp, root, -1, -1, C.byref(n), None, 0),
+            "spr_apply": lib.phy_spr_apply(N, chp, root, bl.ctypes.data_as(P_f64), 0, 2,
+                                           out_ch.ctypes.data_as(P_i32), C.byref(r),
+                                           out_bl.ctypes.data_as(P_f64)),
+            "tbr_apply": lib.phy_tbr_apply(N, chp, root, bl.ctypes.data_as(P_f64), 0, 0, 2,
+                                           out_ch.ctypes.data_as(P_i32), C.byref(r),
+                                           out_bl.ctypes.data_as(P_f64)),
+        }
+        assert all(rc == 1 for rc in rcs.values()), (what, rcs)
+    # the valid tree of the same shape is accepted
+    ok = np.array([0, 1, 4, 2, 5, 3], np.int32)
+    n = C.c_int64(0)
+    assert lib.phy_spr_moves(4, ok.ctypes.data_as(P_i32), 6, C.byref(n), None, 0) == 0
+    assert n.value > 0
