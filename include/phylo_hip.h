@@ -345,13 +345,44 @@ int phy_lk_tree_partials(const phy_topo* topo, const phy_model* model,
                          void* d_work, int32_t v, void** d_partial,
                          int32_t** d_exp, int64_t* pitch);
 
-/* Per-node NodeData operations of Likelihood_c on stored partials
- * (L[r][i][p] with row pitch `pitch` >= n_patterns, exponents e[p]):
- *   leaf   (median_1 of a leaf): L_leaf from its tip nibble block; e = 0.
- *   median2 (median_2, lib/likelihood_c.ml:13):
- *     L_out[r] = (P(rate_r t_x) L_x[r]) (*) (P(rate_r t_y) L_y[r]),
- *     e_out = e_x + e_y + rescaling exponent.
- *   root   (root_cost, lib/likelihood_c.ml:18): d_lnl = sum_p w_p ln L_p. */
+/* Per-node NodeData operations of Likelihood_c (lib/likelihood_c.ml:11-24,
+ * all TODO in the reference) on values in the layout of the whole-tree
+ * stored partials: R*K rows of `pitch` values (row r*K + i holds L[r][i][p],
+ * FP32 or FP64 by form) and R rows of `pitch` int32 exponents (e[r][p]), the
+ * value being L[r][i][p] * 2^e[r][p].  Every (pattern, rate class) is
+ * rescaled on its own; the classes of a pattern are combined only in FP64.
+ * A value's branch length t is the length of the branch from its node to the
+ * point where the operation joins it.  d_weights (nullable: unit weights)
+ * are n_patterns uint32 pattern weights.  All device results are enqueued on
+ * the context's stream.
+ *   leaf    (median_1 of a leaf, lib/likelihood_c.ml:14): L = indicator of
+ *           the tip's state set in every class; e = 0.
+ *   median2 (median_2, lib/likelihood_c.ml:15; the step Node.Make1D.median_2
+ *           makes per block, lib/node.ml:183-198):
+ *           L_out[r] = (P(rate_r t_x) L_x[r]) (*) (P(rate_r t_y) L_y[r]),
+ *           e_out[r] = e_x[r] + e_y[r] + rescaling exponent.
+ *   median3 (median_3, lib/likelihood_c.ml:16; the unrooted root of
+ *           Node.Make1D.median_3, lib/node.ml:200-216): the same over three
+ *           neighbours x, y, z; d_out/d_eout (nullable together) receive
+ *           the joined value and d_lnl (nullable) its log-likelihood
+ *           sum_p w_p ln sum_r prob_r sum_i pi_i L_out[r][i][p] 2^e[r][p],
+ *           i.e. -cost of the tree seen from that node.
+ *   root    (root_cost, lib/likelihood_c.ml:21): d_lnl = that sum at a root
+ *           value (no further branch).
+ *   edge    lnL, dlnL/dt and d2lnL/dt2 of the tree made of the conditional
+ *           values x and y joined by one edge of length t (reversible model,
+ *           so pi sits on either side) into d_out[3]: distance_1
+ *           (lib/likelihood_c.ml:23) is -lnL at t = t_x + t_y; a Newton
+ *           step on the edge uses d_out[1..2].
+ *   adjust3 (adjust_3, lib/likelihood_c.ml:18, as Node.Make1D.readjust_3
+ *           calls it, lib/node.ml:239-256): the length of the edge between
+ *           a node's own value x (the median of its two children) and the
+ *           value y of the rest of the tree beyond its parent, optimised by
+ *           Newton-Raphson with step halving (at most max_iter steps, stop
+ *           when a step moves t by <= tol * t; t kept in [t_lo, t_hi]).
+ *           Synchronises the stream after each evaluation; *t_out receives
+ *           the length, *lnl_out (nullable) its lnL, *n_evals (nullable) the
+ *           number of edge evaluations. */
 int phy_lk_leaf(phy_ctx* ctx, const phy_model* model, int32_t form,
                 int64_t n_patterns, const uint64_t* d_tip, void* d_out,
                 int32_t* d_eout, int64_t pitch);
@@ -359,9 +390,25 @@ int phy_lk_median2(phy_ctx* ctx, const phy_model* model, int32_t form,
                    int64_t n_patterns, const void* d_x, const int32_t* d_ex,
                    double t_x, const void* d_y, const int32_t* d_ey, double t_y,
                    void* d_out, int32_t* d_eout, int64_t pitch);
+int phy_lk_median3(phy_ctx* ctx, const phy_model* model, int32_t form,
+                   int64_t n_patterns, const void* d_x, const int32_t* d_ex,
+                   double t_x, const void* d_y, const int32_t* d_ey, double t_y,
+                   const void* d_z, const int32_t* d_ez, double t_z, void* d_out,
+                   int32_t* d_eout, int64_t pitch, const uint32_t* d_weights,
+                   double* d_lnl);
 int phy_lk_root(phy_ctx* ctx, const phy_model* model, int32_t form,
                 int64_t n_patterns, const void* d_root, const int32_t* d_eroot,
                 int64_t pitch, const uint32_t* d_weights, double* d_lnl);
+int phy_lk_edge(phy_ctx* ctx, const phy_model* model, int32_t form,
+                int64_t n_patterns, const void* d_x, const int32_t* d_ex,
+                const void* d_y, const int32_t* d_ey, int64_t pitch, double t,
+                const uint32_t* d_weights, double* d_out);
+int phy_lk_adjust3(phy_ctx* ctx, const phy_model* model, int32_t form,
+                   int64_t n_patterns, const void* d_x, const int32_t* d_ex,
+                   const void* d_y, const int32_t* d_ey, int64_t pitch,
+                   const uint32_t* d_weights, double t0, double t_lo, double t_hi,
+                   int32_t max_iter, double tol, double* t_out, double* lnl_out,
+                   int32_t* n_evals);
 
 
 /* ------------------------------------------------------------------------- */

@@ -424,10 +424,12 @@ static inline void matvec(int32_t K, const double* P, const double* x, double* y
 double orc_lnl(int32_t N, const int32_t* ch, int32_t root, const double* blen, int32_t K,
                const double* pi, const double* Q, int32_t R, const double* rates,
                const double* probs, const uint32_t* tips, int64_t P, const uint32_t* weights,
-               double* site_lnl, double* d1, double* d2, int32_t n_threads) {
+               double* site_lnl, double* d1, double* d2, double* d1abs, double* d2abs,
+               int32_t n_threads) {
   const int32_t M = 2 * N - 1;
   const int32_t RK = R * K;
   const int want_d = d1 != NULL && d2 != NULL;
+  const int want_abs = want_d && d1abs != NULL && d2abs != NULL;
   int32_t* post = post_order(N, ch, root);
   int32_t* parent = (int32_t*)malloc(sizeof(int32_t) * M);
   for (int32_t v = 0; v < M; ++v) parent[v] = -1;
@@ -446,8 +448,9 @@ double orc_lnl(int32_t N, const int32_t* ch, int32_t root, const double* blen, i
 #else
   n_threads = 1;
 #endif
-  /* per-thread compensated derivative sums, merged in thread order */
-  neu_t* tdd = want_d ? (neu_t*)calloc((size_t)n_threads * 2 * M, sizeof(neu_t)) : NULL;
+  /* per-thread compensated derivative sums, merged in thread order: d1, d2
+   * and (want_abs) the sums of their terms' magnitudes */
+  neu_t* tdd = want_d ? (neu_t*)calloc((size_t)n_threads * 4 * M, sizeof(neu_t)) : NULL;
 #ifdef _OPENMP
 #pragma omp parallel num_threads(n_threads)
 #endif
@@ -457,8 +460,10 @@ double orc_lnl(int32_t N, const int32_t* ch, int32_t root, const double* blen, i
 #else
     const int tno = 0;
 #endif
-    neu_t* my1 = want_d ? tdd + (size_t)tno * 2 * M : NULL;
+    neu_t* my1 = want_d ? tdd + (size_t)tno * 4 * M : NULL;
     neu_t* my2 = want_d ? my1 + M : NULL;
+    neu_t* my3 = want_d ? my1 + 2 * M : NULL;
+    neu_t* my4 = want_d ? my1 + 3 * M : NULL;
     double* L = (double*)malloc(sizeof(double) * M * RK);
     int32_t* e = (int32_t*)malloc(sizeof(int32_t) * M);
     double* A = want_d ? (double*)malloc(sizeof(double) * M * RK) : NULL;
@@ -535,8 +540,11 @@ double orc_lnl(int32_t N, const int32_t* ch, int32_t root, const double* blen, i
         }
         const double g = l1 / l0;
         const double wp = weights ? (double)weights[p] : 1.0;
+        const double h = l2 / l0 - g * g;
         neu_add(&my1[c], wp * g);
-        neu_add(&my2[c], wp * (l2 / l0 - g * g));
+        neu_add(&my2[c], wp * h);
+        neu_add(&my3[c], wp * fabs(g));
+        neu_add(&my4[c], wp * fabs(h));
       }
     }
     free(L);
@@ -551,19 +559,27 @@ double orc_lnl(int32_t N, const int32_t* ch, int32_t root, const double* blen, i
   if (want_d) {
     const int32_t ra = ch[2 * (root - N)], rb = ch[2 * (root - N) + 1];
     for (int32_t c = 0; c < M; ++c) {
-      neu_t a1 = {0.0, 0.0}, a2 = {0.0, 0.0};
-      for (int t = 0; t < n_threads; ++t) {
-        neu_add(&a1, tdd[(size_t)t * 2 * M + c].s);
-        neu_add(&a1, tdd[(size_t)t * 2 * M + c].c);
-        neu_add(&a2, tdd[(size_t)t * 2 * M + M + c].s);
-        neu_add(&a2, tdd[(size_t)t * 2 * M + M + c].c);
+      neu_t a[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+      for (int t = 0; t < n_threads; ++t)
+        for (int q = 0; q < 4; ++q) {
+          neu_add(&a[q], tdd[(size_t)t * 4 * M + q * M + c].s);
+          neu_add(&a[q], tdd[(size_t)t * 4 * M + q * M + c].c);
+        }
+      d1[c] = a[0].s + a[0].c;
+      d2[c] = a[1].s + a[1].c;
+      if (want_abs) {
+        d1abs[c] = a[2].s + a[2].c;
+        d2abs[c] = a[3].s + a[3].c;
       }
-      d1[c] = a1.s + a1.c;
-      d2[c] = a2.s + a2.c;
     }
     d1[root] = d2[root] = 0.0;
     d1[rb] = d1[ra]; /* one unrooted edge across the root */
     d2[rb] = d2[ra];
+    if (want_abs) {
+      d1abs[root] = d2abs[root] = 0.0;
+      d1abs[rb] = d1abs[ra];
+      d2abs[rb] = d2abs[ra];
+    }
   }
   free(post);
   free(parent);

@@ -70,7 +70,8 @@ _lib.orc_compose.restype = None
 _lib.orc_compose.argtypes = [C.c_int32, _f64, C.c_double, _f64]
 _lib.orc_lnl.restype = C.c_double
 _lib.orc_lnl.argtypes = [C.c_int32, _i32, C.c_int32, _f64, C.c_int32, _f64, _f64, C.c_int32,
-                         _f64, _f64, _u32, C.c_int64, _u32, _f64, _f64, _f64, C.c_int32]
+                         _f64, _f64, _u32, C.c_int64, _u32, _f64, _f64, _f64, _f64, _f64,
+                         C.c_int32]
 _lib.orc_lnl_brute.restype = C.c_double
 _lib.orc_lnl_brute.argtypes = [C.c_int32, _i32, C.c_int32, _f64, C.c_int32, _f64, _f64,
                                C.c_int32, _f64, _f64, _u32, C.c_int64, _u32]
@@ -216,8 +217,10 @@ def compose(Q, t):
 
 
 def lnl(n_leaves, children, root, blen, pi, Q, rates, probs, tips, weights=None,
-        derivs=False, threads=1):
-    """tips: [N, P] uint32 state bitmasks.  Returns (lnl, site_lnl[, d1, d2])."""
+        derivs=False, threads=1, scales=False):
+    """tips: [N, P] uint32 state bitmasks.  Returns (lnl, site_lnl[, d1, d2
+    [, d1abs, d2abs]]): with ``scales``, also sum_p w_p |g_p| and
+    sum_p w_p |h_p| per branch (g = L'/L, h = L''/L - g^2)."""
     N = int(n_leaves)
     ch = _c(children, np.int32)
     bl = _c(blen, np.float64)
@@ -231,9 +234,14 @@ def lnl(n_leaves, children, root, blen, pi, Q, rates, probs, tips, weights=None,
     sl = np.zeros(P)
     d1 = np.zeros(2 * N - 1) if derivs else None
     d2 = np.zeros(2 * N - 1) if derivs else None
+    a1 = np.zeros(2 * N - 1) if derivs and scales else None
+    a2 = np.zeros(2 * N - 1) if derivs and scales else None
     v = _lib.orc_lnl(N, _p(ch, _i32), int(root), _p(bl, _f64), pi.size, _p(pi, _f64),
                      _p(Q, _f64), rates.size, _p(rates, _f64), _p(probs, _f64), _p(tips, _u32),
-                     P, _p(w, _u32), _p(sl, _f64), _p(d1, _f64), _p(d2, _f64), int(threads))
+                     P, _p(w, _u32), _p(sl, _f64), _p(d1, _f64), _p(d2, _f64), _p(a1, _f64),
+                     _p(a2, _f64), int(threads))
+    if derivs and scales:
+        return v, sl, d1, d2, a1, a2
     if derivs:
         return v, sl, d1, d2
     return v, sl

@@ -20,6 +20,7 @@
 // per-pattern int32 exponent carries the factor, so FP32 partials cover the
 // FP64 dynamic range without rounding from the scaling itself.
 #include <algorithm>
+#include <climits>
 #include <atomic>
 #include <cmath>
 #include <thread>
@@ -42,7 +43,6 @@ struct phy_model {
 
 namespace {
 
-constexpr int K4 = 4;
 
 __device__ __forceinline__ int cst_U(int K) { return 0; }
 __device__ __forceinline__ int cst_lam(int K) { return K * K; }
@@ -64,11 +64,16 @@ struct Thr<double> {
   static constexpr double lo = 0x1p-128;
 };
 
+// up to three branch lengths passed by value (the per-node operations)
+struct HostLens {
+  double t[3];
+};
+
 // P, P' and P'' of `nb` branches x R rates: one thread per (branch, rate, i, j).
-// t_b = d_t[b] if d_t, else t0 (b == 0) / t1 (b == 1).
+// t_b = d_t[b] if d_t, else hl.t[b] (b < 3).
 template <typename real>
 __global__ void k_pmats(const double* __restrict__ cst, int K, int R, const double* __restrict__ d_t,
-                        double t0, double t1, int nb, real* __restrict__ P, real* __restrict__ P1,
+                        HostLens hl, int nb, real* __restrict__ P, real* __restrict__ P1,
                         real* __restrict__ P2) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t per = (int64_t)R * K * K;
@@ -80,7 +85,7 @@ __global__ void k_pmats(const double* __restrict__ cst, int K, int R, const doub
   const double* lam = cst + cst_lam(K);
   const double* Ui = cst + cst_Ui(K);
   const double rate = cst[cst_rates(K) + r];
-  const double t = d_t ? d_t[b] : (b == 0 ? t0 : t1);
+  const double t = d_t ? d_t[b] : hl.t[b < 3 ? b : 2];
   const double rt = rate * t;
   double p = 0.0, p1 = 0.0, p2 = 0.0;
   for (int k = 0; k < K; ++k) {
@@ -167,6 +172,9 @@ __device__ __forceinline__ void rescale(real* v, int* e) {
 // Cross-lane data movement through DPP (no LDS round trip).  Controls:
 // quad_perm(1,0,3,2) = lane ^ 1, quad_perm(2,3,0,1) = lane ^ 2,
 // row_half_mirror = lane ^ 7 within 8, row_mirror = lane ^ 15 within 16.
+// exponent of an all-zero partial in a maximum over rate classes: below any
+// real exponent, and far enough from INT_MIN that e - kNoExp cannot overflow
+constexpr int kNoExp = INT_MIN / 2;
 constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140;
 
 template <int CTRL>
@@ -456,9 +464,14 @@ __device__ __forceinline__ void down_root(const DownState<real, R, G, FUSED, LV>
   }
   __syncthreads();
   if (st.r0 == 0) {
-    int m = rootE[lane];
+    // m over the classes with a non-zero sum only: a class whose partial is
+    // all zero (a zero rate, P = I, under a non-constant pattern) never
+    // rescales, so its exponent says nothing about the pattern's scale
+    int m = INT_MIN;
 #pragma unroll
-    for (int q = 1; q < R; ++q) m = max(m, rootE[q * 64 + lane]);
+    for (int q = 0; q < R; ++q)
+      if (rootS[q * 64 + lane] > 0.0) m = max(m, rootE[q * 64 + lane]);
+    if (m == INT_MIN) m = 0;
     double L = 0.0;
 #pragma unroll
     for (int q = 0; q < R; ++q) L += probs[q] * ldexp(rootS[q * 64 + lane], rootE[q * 64 + lane] - m);
@@ -565,35 +578,14 @@ __global__ __launch_bounds__(512) void k_prune_down_batch(
 }
 
 // Helpers of the derivative pass, whose threads are (pattern, rate) quads.
-// max / sum over the R lanes of one pattern (lanes p*R .. p*R+R-1); every lane
-// of the group gets the same bits (a + b == b + a exactly)
-template <typename real, int R>
-__device__ __forceinline__ real rate_max(real x) {
-  if (R >= 2) x = fmax(x, dpp<DPP_XOR1>(x));
-  if (R >= 4) x = fmax(x, dpp<DPP_XOR2>(x));
-  if (R >= 8) x = fmax(x, dpp<DPP_HALF_MIRROR>(x));
-  return x;
-}
+// sum over the R lanes of one pattern (lanes p*R .. p*R+R-1); every lane of
+// the group gets the same bits (a + b == b + a exactly)
 template <int R>
 __device__ __forceinline__ double rate_sum(double x) {
   if (R >= 2) x += dpp<DPP_XOR1>(x);
   if (R >= 4) x += dpp<DPP_XOR2>(x);
   if (R >= 8) x += dpp<DPP_HALF_MIRROR>(x);
   return x;
-}
-
-// exact power-of-two rescale of a pattern's R x 4 values held 4 per lane
-template <typename real, int R>
-__device__ __forceinline__ void rescale_pattern(real* v, int* e) {
-  real mx = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
-  mx = rate_max<real, R>(mx);
-  if (mx < Thr<real>::lo && mx > (real)0) {
-    int ex;
-    (void)frexp(mx, &ex);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = ldexp(v[k], -ex);
-    *e += ex;
-  }
 }
 
 // Sum over the lanes of a 16-lane row that are congruent mod S (S = 1, 2, 4,
@@ -663,11 +655,12 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
     int64_t n_pat, int64_t pitch, const real* __restrict__ tab, const real* __restrict__ E,
     const double* __restrict__ pi, const double* __restrict__ probs, const real* __restrict__ part,
     const int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
-    double* __restrict__ wsum, int64_t n_slots, int N) {
+    double* __restrict__ wsum, int64_t n_slots, int N, int up_depth) {
   constexpr int QPL = R >= 4 ? 1 : 4 / R;  // derivative quantities per lane
   extern __shared__ unsigned char lds_raw[];
   real* stk = reinterpret_cast<real*>(lds_raw);  // [depth][4][bd]
   const int tid = threadIdx.x, bd = blockDim.x;
+  int32_t* stke = reinterpret_cast<int32_t*>(stk + (size_t)up_depth * 4 * bd);  // [depth][bd]
   const int r = tid % R;
   const int64_t p = (int64_t)blockIdx.x * (bd / R) + tid / R;
   const bool act = p < n_pat;
@@ -705,18 +698,15 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
 #pragma unroll
     for (int k = 0; k < 4; ++k) ev[k] = es[k];
   };
-  // the R classes of a pattern were scaled separately in the post-order
-  // pass: bring them to the largest exponent (exact power-of-two scaling)
-  auto align_D = [&](real* Dv, int e) {
-    int m = e;
-    if (R >= 2) m = max(m, __builtin_amdgcn_update_dpp(0, m, DPP_XOR1, 0xF, 0xF, false));
-    if (R >= 4) m = max(m, __builtin_amdgcn_update_dpp(0, m, DPP_XOR2, 0xF, 0xF, false));
-    if (R >= 8) m = max(m, __builtin_amdgcn_update_dpp(0, m, DPP_HALF_MIRROR, 0xF, 0xF, false));
-    if (e != m) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) Dv[k] = ldexp(Dv[k], e - m);
-    }
-  };
+  // Every factor keeps the power-of-two exponent of its own rate class: the
+  // children partials the exponents of the post-order pass, the messages M
+  // their own (M_root = pi: 0).  A class's terms s = M . ((P_a D_a) (*)
+  // (P_b D_b)) are products of three rescaled factors, so they stay normal
+  // in the working precision; the classes of a pattern meet only in the FP64
+  // rate sum, aligned to the largest exponent among the classes whose term
+  // is non-zero.  (An all-zero class -- a zero rate under a non-constant
+  // pattern -- never rescales, and a class that dominates one child can be
+  // zero in the product, so no alignment of a single factor is safe.)
 
   real Dc[8], Ec[8];  // children partials and eigen factors of the current entry
   int Xc[2];
@@ -726,6 +716,7 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
     fetch(o.z, Dc + 4, &Xc[1], Ec + 4);
   }
   real accM[4] = {(real)0, (real)0, (real)0, (real)0};
+  int accE = 0;
   int sp = 0;
   for_each_entry_ahead(prog, n_ops, [&](int i, const int4 op, const int4 on) {
     real Dn[8], En[8];
@@ -738,23 +729,25 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
     const int fl = op.w, src = fl & 3;
     const int a = op.y, b = op.z;
     real Mv[4];
+    int eM;
     if (src == SRC_LEAF) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) Mv[k] = (real)pi[k];
+      eM = 0;
     } else if (src == SRC_ACC) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) Mv[k] = accM[k];
+      eM = accE;
     } else {
       --sp;
 #pragma unroll
       for (int k = 0; k < 4; ++k) Mv[k] = stk[((size_t)sp * 4 + k) * bd + tid];
+      eM = stke[(size_t)sp * bd + tid];
     }
-    real* Da = Dc;
-    real* Db = Dc + 4;
+    const real* Da = Dc;
+    const real* Db = Dc + 4;
     const real* ea = Ec;
     const real* eb = Ec + 4;
-    align_D(Da, Xc[0]);
-    align_D(Db, Xc[1]);
     real ua[4], ub[4], wa[4], wb[4], ya[4], yb[4], Aa[4], Ab[4], ga[4], gb[4];
     matvec_u(Ui, Da, ua);
     matvec_u(Ui, Db, ub);
@@ -783,9 +776,18 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
       s1b = fma(qb, l1[k], s1b);
       s2b = fma(qb, l2[k], s2b);
     }
-    const double L0 = rate_sum<R>(pr * (double)s0);
-    const double L1a = rate_sum<R>(pr * (double)s1a), L2a = rate_sum<R>(pr * (double)s2a);
-    const double L1b = rate_sum<R>(pr * (double)s1b), L2b = rate_sum<R>(pr * (double)s2b);
+    // class weight prob_r 2^(E_r - m), E_r = eM + e_a + e_b, m the largest E
+    // over the pattern's classes with a non-zero term (0 for the others)
+    const int E = eM + Xc[0] + Xc[1];
+    const bool nz = s0 != (real)0;
+    int m = nz ? E : kNoExp;
+    if (R >= 2) m = max(m, __builtin_amdgcn_update_dpp(0, m, DPP_XOR1, 0xF, 0xF, false));
+    if (R >= 4) m = max(m, __builtin_amdgcn_update_dpp(0, m, DPP_XOR2, 0xF, 0xF, false));
+    if (R >= 8) m = max(m, __builtin_amdgcn_update_dpp(0, m, DPP_HALF_MIRROR, 0xF, 0xF, false));
+    const double sc = nz ? ldexp(pr, E - m) : 0.0;
+    const double L0 = rate_sum<R>(sc * (double)s0);
+    const double L1a = rate_sum<R>(sc * (double)s1a), L2a = rate_sum<R>(sc * (double)s2a);
+    const double L1b = rate_sum<R>(sc * (double)s1b), L2b = rate_sum<R>(sc * (double)s2b);
     // The four per-pattern quantities q = (g_a, h_a, g_b, h_b) are spread
     // over the R lanes of the pattern (lane r holds q = r * QPL + j), summed
     // over the lanes of each 16-lane row that hold the same q (DPP row
@@ -817,33 +819,38 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
         }
       }
     }
-    // messages to internal children: M_c = Ui^T (e_c (*) g_c)
+    // messages to internal children: M_c = Ui^T (e_c (*) g_c), with the
+    // exponent of A_c (the parent's message times the sibling's partial)
     real Ma[4], Mb[4];
-    int dummy = 0;
+    int eMa = eM + Xc[1], eMb = eM + Xc[0];
     if (!(fl & UP_A_LEAF)) {
       real t[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) t[k] = ea[k] * ga[k];
       vecmat_u(Ui, t, Ma);
-      rescale_pattern<real, R>(Ma, &dummy);
+      rescale4(Ma, &eMa);
     }
     if (!(fl & UP_B_LEAF)) {
       real t[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) t[k] = eb[k] * gb[k];
       vecmat_u(Ui, t, Mb);
-      rescale_pattern<real, R>(Mb, &dummy);
+      rescale4(Mb, &eMb);
     }
     if (fl & UP_PUSH) {
-      const real* o = (fl & UP_CONT_B) ? Ma : Mb;
+      const bool cb = fl & UP_CONT_B;
+      const real* o = cb ? Ma : Mb;
 #pragma unroll
       for (int k = 0; k < 4; ++k) stk[((size_t)sp * 4 + k) * bd + tid] = o[k];
+      stke[(size_t)sp * bd + tid] = cb ? eMa : eMb;
       ++sp;
     }
     if (!(fl & UP_A_LEAF) || !(fl & UP_B_LEAF)) {
-      const real* c = (fl & UP_CONT_B) ? Mb : Ma;
+      const bool cb = fl & UP_CONT_B;
+      const real* c = cb ? Mb : Ma;
 #pragma unroll
       for (int k = 0; k < 4; ++k) accM[k] = c[k];
+      accE = cb ? eMb : eMa;
     }
     if (more) {
 #pragma unroll
@@ -940,7 +947,12 @@ __global__ void k_reduce_derivs(const double* __restrict__ ws, int64_t n_waves, 
 }
 
 // ---------------------------------------------------------------------------
-// Per-node API kernels
+// Per-node API kernels.  A value is R*K rows of `pitch` values, row r*K + i
+// holding L[r][i][p], and R rows of int32 exponents e[r][p] (the layout of
+// the whole-tree stored partials): every (pattern, rate class) is rescaled
+// on its own, and the classes of a pattern meet only in FP64 sums aligned to
+// the largest exponent among the classes with a non-zero term.
+// One thread per pattern.
 // ---------------------------------------------------------------------------
 template <typename real, int R>
 __global__ void k_lk_leaf(const uint64_t* __restrict__ tip, int64_t n_pat, int64_t pitch,
@@ -950,41 +962,97 @@ __global__ void k_lk_leaf(const uint64_t* __restrict__ tip, int64_t n_pat, int64
   real t[4];
   tipvec(tip_code(tip, 0, 0, p), t);
 #pragma unroll
-  for (int r = 0; r < R; ++r)
+  for (int r = 0; r < R; ++r) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) out[(size_t)(r * 4 + i) * pitch + p] = t[i];
-  eout[p] = 0;
+    eout[(size_t)r * pitch + p] = 0;
+  }
 }
 
+// y = P(rate_r t) x for class r of a stored value at column p
+template <typename real>
+__device__ __forceinline__ int class_product(const real* __restrict__ P, const real* __restrict__ x,
+                                             const int32_t* __restrict__ ex, int r, int64_t pitch,
+                                             int64_t p, real* y) {
+  real a[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) a[i] = x[(size_t)(r * 4 + i) * pitch + p];
+  matvec4(P, a, y);
+  return ex[(size_t)r * pitch + p];
+}
+
+// Per class, the FP64 weight prob_r 2^(E_r - m) of a term s_r with exponent
+// E_r, m the largest E_r over the classes whose term is non-zero; 0 for a
+// zero term.  Returns m (kNoExp when every term is zero).
+template <int R>
+__device__ __forceinline__ int class_weights(const double* s, const int* E,
+                                             const double* __restrict__ probs, double* wt) {
+  int m = kNoExp;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (s[r] != 0.0) m = max(m, E[r]);
+#pragma unroll
+  for (int r = 0; r < R; ++r) wt[r] = s[r] != 0.0 ? ldexp(probs[r], E[r] - m) : 0.0;
+  return m;
+}
+
+// median_2 (and median_3 when z != nullptr): per class
+//   L_out[r] = (P_x x[r]) (*) (P_y y[r]) [(*) (P_z z[r])],
+// e_out[r] = e_x[r] + e_y[r] [+ e_z[r]] + its own rescaling exponent.
+// With block_sums, also the root sum of the result: one weighted log-likelihood
+// partial sum per 64 patterns.
 template <typename real, int R>
-__global__ void k_lk_median2(const real* __restrict__ Pxy, const real* __restrict__ x,
-                             const int32_t* __restrict__ exx, const real* __restrict__ y,
-                             const int32_t* __restrict__ exy, int64_t n_pat, int64_t pitch,
-                             real* __restrict__ out, int32_t* __restrict__ eout) {
-  constexpr int RK = R * 4;
+__global__ void k_lk_median(const real* __restrict__ Pm, const real* __restrict__ x,
+                            const int32_t* __restrict__ ex, const real* __restrict__ y,
+                            const int32_t* __restrict__ ey, const real* __restrict__ z,
+                            const int32_t* __restrict__ ez, int64_t n_pat, int64_t pitch,
+                            real* __restrict__ out, int32_t* __restrict__ eout,
+                            const double* __restrict__ pi, const double* __restrict__ probs,
+                            const uint32_t* __restrict__ weights, double* __restrict__ block_sums) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n_pat) return;
-  real v[RK];
+  const bool act = p < n_pat;
+  const int64_t pc = act ? p : 0;
+  double S[R];
+  int E[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    real a[4], b[4], xa[4], yb[4];
+    real xa[4], yb[4];
+    int e = class_product(Pm + (size_t)r * 16, x, ex, r, pitch, pc, xa);
+    e += class_product(Pm + (size_t)(R + r) * 16, y, ey, r, pitch, pc, yb);
+    real v[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      a[i] = x[(size_t)(r * 4 + i) * pitch + p];
-      b[i] = y[(size_t)(r * 4 + i) * pitch + p];
+    for (int i = 0; i < 4; ++i) v[i] = xa[i] * yb[i];
+    if (z) {
+      real zc[4];
+      e += class_product(Pm + (size_t)(2 * R + r) * 16, z, ez, r, pitch, pc, zc);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] *= zc[i];
     }
-    matvec4(Pxy + (size_t)r * 16, a, xa);
-    matvec4(Pxy + (size_t)(R + r) * 16, b, yb);
+    rescale4(v, &e);
+    if (out && act) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) v[r * 4 + i] = xa[i] * yb[i];
+      for (int i = 0; i < 4; ++i) out[(size_t)(r * 4 + i) * pitch + p] = v[i];
+      eout[(size_t)r * pitch + p] = e;
+    }
+    S[r] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) S[r] += pi[i] * (double)v[i];
+    E[r] = e;
   }
-  int e = exx[p] + exy[p];
-  rescale<real, RK>(v, &e);
+  if (block_sums) {
+    double wt[R];
+    const int m = class_weights<R>(S, E, probs, wt);
+    double L = 0.0;
 #pragma unroll
-  for (int k = 0; k < RK; ++k) out[(size_t)k * pitch + p] = v[k];
-  eout[p] = e;
+    for (int r = 0; r < R; ++r) L += wt[r] * S[r];
+    const double contrib =
+        act ? (double)(weights ? weights[p] : 1u) * (log(L) + (double)m * 0.6931471805599453) : 0.0;
+    const double ws = wave_sum_d(contrib);
+    if ((threadIdx.x & 63) == 0) block_sums[p >> 6] = ws;
+  }
 }
 
+// root_cost: sum_p w_p ln(sum_r prob_r sum_i pi_i L[r][i][p] 2^e[r][p])
 template <typename real, int R>
 __global__ void k_lk_root(const real* __restrict__ L, const int32_t* __restrict__ e, int64_t n_pat,
                           int64_t pitch, const double* __restrict__ pi,
@@ -993,18 +1061,89 @@ __global__ void k_lk_root(const real* __restrict__ L, const int32_t* __restrict_
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double contrib = 0.0;
   if (p < n_pat) {
-    double lk = 0.0;
+    double S[R], wt[R];
+    int E[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      double s = 0.0;
+      S[r] = 0.0;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) s += pi[i] * (double)L[(size_t)(r * 4 + i) * pitch + p];
-      lk += probs[r] * s;
+      for (int i = 0; i < 4; ++i) S[r] += pi[i] * (double)L[(size_t)(r * 4 + i) * pitch + p];
+      E[r] = e[(size_t)r * pitch + p];
     }
-    contrib = (double)(weights ? weights[p] : 1u) * (log(lk) + (double)e[p] * 0.6931471805599453);
+    const int m = class_weights<R>(S, E, probs, wt);
+    double lk = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) lk += wt[r] * S[r];
+    contrib = (double)(weights ? weights[p] : 1u) * (log(lk) + (double)m * 0.6931471805599453);
   }
   const double ws = wave_sum_d(contrib);
   if ((threadIdx.x & 63) == 0) block_sums[p >> 6] = ws;
+}
+
+// One edge of length t between the conditional partials x and y (the two
+// sides of the edge; reversible model, so pi sits on either side):
+//   L_p = sum_r prob_r sum_ij pi_i x[r][i] P_ij(rate_r t) y[r][j],
+// and L', L'' with P' = rate_r Q P and P'' = (rate_r Q)^2 P.  Per 64
+// patterns, three partial sums: sum w ln L, sum w L'/L and
+// sum w (L''/L - (L'/L)^2), at block_sums[q * n_wave_slots + wave].
+template <typename real, int R>
+__global__ void k_lk_edge(const real* __restrict__ Pm, const real* __restrict__ P1m,
+                          const real* __restrict__ P2m, const real* __restrict__ x,
+                          const int32_t* __restrict__ ex, const real* __restrict__ y,
+                          const int32_t* __restrict__ ey, int64_t n_pat, int64_t pitch,
+                          const double* __restrict__ pi, const double* __restrict__ probs,
+                          const uint32_t* __restrict__ weights, double* __restrict__ block_sums,
+                          int64_t n_wave_slots) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+  if (p < n_pat) {
+    double S0[R], S1[R], S2[R], wt[R];
+    int E[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      real y0[4], y1[4], y2[4], xv[4];
+      int e = class_product(Pm + (size_t)r * 16, y, ey, r, pitch, p, y0);
+      real yv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) yv[i] = y[(size_t)(r * 4 + i) * pitch + p];
+      matvec4(P1m + (size_t)r * 16, yv, y1);
+      matvec4(P2m + (size_t)r * 16, yv, y2);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) xv[i] = x[(size_t)(r * 4 + i) * pitch + p];
+      e += ex[(size_t)r * pitch + p];
+      S0[r] = S1[r] = S2[r] = 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const double a = pi[i] * (double)xv[i];
+        S0[r] += a * (double)y0[i];
+        S1[r] += a * (double)y1[i];
+        S2[r] += a * (double)y2[i];
+      }
+      E[r] = e;
+    }
+    const int m = class_weights<R>(S0, E, probs, wt);
+    double L0 = 0.0, L1 = 0.0, L2 = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      L0 += wt[r] * S0[r];
+      L1 += wt[r] * S1[r];
+      L2 += wt[r] * S2[r];
+    }
+    const double w = (double)(weights ? weights[p] : 1u);
+    const double g = L1 / L0;
+    q0 = w * (log(L0) + (double)m * 0.6931471805599453);
+    q1 = w * g;
+    q2 = w * (L2 / L0 - g * g);
+  }
+  q0 = wave_sum_d(q0);
+  q1 = wave_sum_d(q1);
+  q2 = wave_sum_d(q2);
+  if ((threadIdx.x & 63) == 0) {
+    const int64_t wv = p >> 6;
+    block_sums[wv] = q0;
+    block_sums[n_wave_slots + wv] = q1;
+    block_sums[2 * n_wave_slots + wv] = q2;
+  }
 }
 
 int env_block(const char* name, int dflt) {
@@ -1053,8 +1192,10 @@ size_t down_lds(int R, size_t esz, bool fused, int depth, int reg_levels = 0) {
   const size_t rows = (size_t)std::max(depth - reg_levels, 0);
   return std::max(root, fused ? rows * bs * (4 * esz + 4) : 0);
 }
+// LDS of the derivative pass: a message stack of 4 values + an exponent per
+// thread and level
 size_t deriv_lds(int R, size_t esz, int bs, int depth) {
-  return (size_t)std::max(depth, 1) * bs * 4 * esz;
+  return (size_t)std::max(depth, 1) * bs * (4 * esz + 4);
 }
 
 // largest block size <= want whose LDS fits (0 if none)
@@ -1159,7 +1300,7 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
   {
     const int64_t n = (int64_t)M * R * K * K;
     hipLaunchKernelGGL((k_pmats<real>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                       cst, K, R, blen, 0.0, 0.0, M, P, (real*)nullptr, (real*)nullptr);
+                       cst, K, R, blen, HostLens{}, M, P, (real*)nullptr, (real*)nullptr);
     PHY_LAUNCHED(ctx);
   }
   real* part = fused ? nullptr : reinterpret_cast<real*>(work + pl.off_part);
@@ -1199,7 +1340,8 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
     PHY_LAUNCHED(ctx);
     hipLaunchKernelGGL((k_prune_deriv<real, R>), dim3((unsigned)pl.dblocks), dim3(pl.dbs), pl.dlds,
                        ctx->stream, t->d_up_prog, N - 1, tips, stride, n_pat, pl.pitch, tab, E,
-                       d_pi, d_probs, part, ex, weights, wsum, pl.n_dwaves, N);
+                       d_pi, d_probs, part, ex, weights, wsum, pl.n_dwaves, N,
+                       std::max(t->up_depth, 1));
     PHY_LAUNCHED(ctx);
     const int ra = t->children[2 * (t->root - N)], rb = t->children[2 * (t->root - N) + 1];
     hipLaunchKernelGGL(k_reduce_derivs, dim3((unsigned)M), dim3(256), 0, ctx->stream, wsum,
@@ -1291,7 +1433,7 @@ int phy_model_pmatrices(phy_ctx* ctx, const phy_model* m, const double* d_t, int
   if (nb == 0) return PHY_OK;
   const int64_t n = (int64_t)nb * m->R * m->K * m->K;
   hipLaunchKernelGGL((k_pmats<double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                     m->d_consts, m->K, m->R, d_t, 0.0, 0.0, nb, P, P1, P2);
+                     m->d_consts, m->K, m->R, d_t, HostLens{}, nb, P, P1, P2);
   PHY_LAUNCHED(ctx);
   return PHY_OK;
 }
@@ -1391,7 +1533,7 @@ int spr_launch(phy_ctx* ctx, const phy_model* m, int N, int64_t n_pat, const uin
   const double* bl = reinterpret_cast<const double*>(work + pl.off_bl);
   const int64_t n = (int64_t)C * M * R * 16;
   hipLaunchKernelGGL((k_pmats<real>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, cst,
-                     K, R, bl, 0.0, 0.0, C * M, P, (real*)nullptr, (real*)nullptr);
+                     K, R, bl, HostLens{}, C * M, P, (real*)nullptr, (real*)nullptr);
   PHY_LAUNCHED(ctx);
   const size_t lds = down_lds(R, sizeof(real), true, depth);
   if (lds > kLdsMax) return set_err(ctx, PHY_ERANGE, "spr scores: LDS stack too deep");
@@ -1496,13 +1638,17 @@ int score_candidates(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_p
 int check_scores_args(phy_ctx* ctx, const phy_model* m, int64_t n_pat, const uint64_t* tips,
                       int64_t stride, const uint32_t* weights, int32_t N, const int32_t* children,
                       const double* blen, int64_t n_moves, const int32_t* moves, int32_t chunk,
-                      double* d_lnl, void* d_work, const char* what) {
+                      double* d_lnl, void* d_work, int32_t root, const char* what) {
   if (!ctx || !m) return PHY_EINVAL;
   if (m->K != 4) return set_err(ctx, PHY_EINVAL, std::string(what) + ": 4-state models only");
   if (N < 3 || n_pat < 1 || chunk < 1 || n_moves < 0 || !children || !blen || !tips || !weights ||
       !d_lnl || !d_work || (n_moves > 0 && !moves))
     return set_err(ctx, PHY_EINVAL, std::string(what) + ": bad argument");
   if (stride < phy_words(n_pat)) return set_err(ctx, PHY_EINVAL, std::string(what) + ": tip stride < words");
+  // the base tree itself (one parent per node, no cycle), before any move
+  int64_t n_nni = 0;
+  if (phy_nni_moves(N, children, root, &n_nni, nullptr, 0) != PHY_OK)
+    return set_err(ctx, PHY_EINVAL, std::string(what) + ": not a rooted binary tree");
   return PHY_OK;
 }
 
@@ -1516,14 +1662,18 @@ int phy_lk_spr_scores(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_
                       const int32_t* moves, int32_t chunk, double* d_lnl, void* d_work,
                       size_t work_bytes) {
   int rc = check_scores_args(ctx, m, n_pat, tips, stride, weights, N, children, blen, n_moves,
-                             moves, chunk, d_lnl, d_work, "spr scores");
+                             moves, chunk, d_lnl, d_work, root, "spr scores");
   if (rc) return rc;
-  return score_candidates(ctx, m, form, n_pat, tips, stride, weights, N, n_moves, chunk, d_lnl,
-                          d_work, work_bytes, "spr scores",
-                          [&](int64_t i, int32_t* ch, int32_t* r, double* bl) {
-                            return phy_spr_apply(N, children, root, blen, moves[2 * i],
-                                                 moves[2 * i + 1], ch, r, bl);
-                          });
+  try {
+    return score_candidates(ctx, m, form, n_pat, tips, stride, weights, N, n_moves, chunk, d_lnl,
+                            d_work, work_bytes, "spr scores",
+                            [&](int64_t i, int32_t* ch, int32_t* r, double* bl) {
+                              return phy_spr_apply(N, children, root, blen, moves[2 * i],
+                                                   moves[2 * i + 1], ch, r, bl);
+                            });
+  } catch (...) {
+    return set_err(ctx, PHY_ENOMEM, "spr scores: host allocation or thread start failed");
+  }
 }
 
 int phy_lk_tbr_scores(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat,
@@ -1532,14 +1682,18 @@ int phy_lk_tbr_scores(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_
                       const int32_t* moves, int32_t chunk, double* d_lnl, void* d_work,
                       size_t work_bytes) {
   int rc = check_scores_args(ctx, m, n_pat, tips, stride, weights, N, children, blen, n_moves,
-                             moves, chunk, d_lnl, d_work, "tbr scores");
+                             moves, chunk, d_lnl, d_work, root, "tbr scores");
   if (rc) return rc;
-  return score_candidates(ctx, m, form, n_pat, tips, stride, weights, N, n_moves, chunk, d_lnl,
-                          d_work, work_bytes, "tbr scores",
-                          [&](int64_t i, int32_t* ch, int32_t* r, double* bl) {
-                            return phy_tbr_apply(N, children, root, blen, moves[3 * i],
-                                                 moves[3 * i + 1], moves[3 * i + 2], ch, r, bl);
-                          });
+  try {
+    return score_candidates(ctx, m, form, n_pat, tips, stride, weights, N, n_moves, chunk, d_lnl,
+                            d_work, work_bytes, "tbr scores",
+                            [&](int64_t i, int32_t* ch, int32_t* r, double* bl) {
+                              return phy_tbr_apply(N, children, root, blen, moves[3 * i],
+                                                   moves[3 * i + 1], moves[3 * i + 2], ch, r, bl);
+                            });
+  } catch (...) {
+    return set_err(ctx, PHY_ENOMEM, "tbr scores: host allocation or thread start failed");
+  }
 }
 
 }  // extern "C"
@@ -1556,23 +1710,49 @@ static int node_leaf(phy_ctx* ctx, int64_t n, const uint64_t* tip, void* out, in
   return PHY_OK;
 }
 
+// model constants: pi and the class probabilities
+inline const double* model_pi(const phy_model* m) { return m->d_consts + 2 * 4 * 4 + 4 + m->R; }
+
+// P (and P', P'' if with_derivs) of nb <= 3 branches into the model scratch,
+// followed by `extra` bytes for the caller; returns the scratch base.
 template <typename real, int R>
-static int node_median2(phy_ctx* ctx, phy_model* m, int64_t n, const void* x, const int32_t* ex,
-                        double tx, const void* y, const int32_t* ey, double ty, void* out,
-                        int32_t* eout, int64_t pitch) {
-  const size_t need = (size_t)2 * R * 16 * sizeof(real);
-  int rc = model_scratch(m, std::max<size_t>(need, 4096));
+static int node_pmats(phy_ctx* ctx, phy_model* m, HostLens hl, int nb, bool with_derivs,
+                      size_t extra, real** P, unsigned char** rest) {
+  const size_t mats = align256((size_t)(with_derivs ? 3 : 1) * nb * R * 16 * sizeof(real));
+  int rc = model_scratch(m, std::max<size_t>(mats + extra, 4096));
   if (rc) return rc;
-  real* Pxy = (real*)m->d_scratch;
-  const int64_t nm = (int64_t)2 * R * 16;
+  *P = (real*)m->d_scratch;
+  *rest = (unsigned char*)m->d_scratch + mats;
+  const int64_t nm = (int64_t)nb * R * 16;
+  real* P1 = with_derivs ? *P + nm : nullptr;
+  real* P2 = with_derivs ? *P + 2 * nm : nullptr;
   hipLaunchKernelGGL((k_pmats<real>), dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, ctx->stream,
-                     m->d_consts, 4, R, (const double*)nullptr, tx, ty, 2, Pxy, (real*)nullptr,
-                     (real*)nullptr);
+                     m->d_consts, 4, R, (const double*)nullptr, hl, nb, *P, P1, P2);
   PHY_LAUNCHED(ctx);
-  hipLaunchKernelGGL((k_lk_median2<real, R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     ctx->stream, Pxy, (const real*)x, ex, (const real*)y, ey, n, pitch, (real*)out,
-                     eout);
+  return PHY_OK;
+}
+
+template <typename real, int R>
+static int node_median(phy_ctx* ctx, phy_model* m, int64_t n, const void* x, const int32_t* ex,
+                       double tx, const void* y, const int32_t* ey, double ty, const void* z,
+                       const int32_t* ez, double tz, void* out, int32_t* eout, int64_t pitch,
+                       const uint32_t* w, double* lnl) {
+  const int64_t nb = (n + 255) / 256;
+  real* Pm;
+  unsigned char* rest;
+  int rc = node_pmats<real, R>(ctx, m, HostLens{{tx, ty, tz}}, z ? 3 : 2, false,
+                               lnl ? (size_t)nb * 4 * 8 : 0, &Pm, &rest);
+  if (rc) return rc;
+  double* bs = lnl ? (double*)rest : nullptr;
+  const double* d_pi = model_pi(m);
+  hipLaunchKernelGGL((k_lk_median<real, R>), dim3((unsigned)nb), dim3(256), 0, ctx->stream, Pm,
+                     (const real*)x, ex, (const real*)y, ey, (const real*)z, ez, n, pitch,
+                     (real*)out, eout, d_pi, d_pi + 4, w, bs);
   PHY_LAUNCHED(ctx);
+  if (lnl) {
+    hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, ctx->stream, bs, nb * 4, (int64_t)1, lnl);
+    PHY_LAUNCHED(ctx);
+  }
   return PHY_OK;
 }
 
@@ -1583,12 +1763,33 @@ static int node_root(phy_ctx* ctx, phy_model* m, int64_t n, const void* L, const
   int rc = model_scratch(m, std::max<size_t>((size_t)nb * 4 * 8, 4096));
   if (rc) return rc;
   double* bs = (double*)m->d_scratch;  // block sums
-  const int K = 4;
-  const double* d_pi = m->d_consts + 2 * K * K + K + R;
+  const double* d_pi = model_pi(m);
   hipLaunchKernelGGL((k_lk_root<real, R>), dim3((unsigned)nb), dim3(256), 0, ctx->stream,
-                     (const real*)L, e, n, pitch, d_pi, d_pi + K, w, bs);
+                     (const real*)L, e, n, pitch, d_pi, d_pi + 4, w, bs);
   PHY_LAUNCHED(ctx);
   hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, ctx->stream, bs, nb * 4, (int64_t)1, lnl);
+  PHY_LAUNCHED(ctx);
+  return PHY_OK;
+}
+
+// lnL, dlnL/dt, d2lnL/dt2 of the edge of length t between x and y, into the
+// device array out[3]
+template <typename real, int R>
+static int node_edge(phy_ctx* ctx, phy_model* m, int64_t n, const void* x, const int32_t* ex,
+                     const void* y, const int32_t* ey, int64_t pitch, double t, const uint32_t* w,
+                     double* out) {
+  const int64_t nb = (n + 255) / 256, slots = nb * 4;
+  real* Pm;
+  unsigned char* rest;
+  int rc = node_pmats<real, R>(ctx, m, HostLens{{t, t, t}}, 1, true, (size_t)3 * slots * 8, &Pm, &rest);
+  if (rc) return rc;
+  double* bs = (double*)rest;
+  const double* d_pi = model_pi(m);
+  hipLaunchKernelGGL((k_lk_edge<real, R>), dim3((unsigned)nb), dim3(256), 0, ctx->stream, Pm,
+                     Pm + R * 16, Pm + 2 * R * 16, (const real*)x, ex, (const real*)y, ey, n, pitch,
+                     d_pi, d_pi + 4, w, bs, slots);
+  PHY_LAUNCHED(ctx);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(3), dim3(256), 0, ctx->stream, bs, slots, out);
   PHY_LAUNCHED(ctx);
   return PHY_OK;
 }
@@ -1620,8 +1821,96 @@ int phy_lk_median2(phy_ctx* ctx, const phy_model* mc, int32_t form, int64_t n, c
   phy_model* m = const_cast<phy_model*>(mc);
   if (m->K != 4 || n < 1 || pitch < n || !x || !ex || !y || !ey || !out || !eout)
     return set_err(ctx, PHY_EINVAL, "lk_median2: bad argument");
-  if (form & PHY_F32) { PHY_DISPATCH_R(node_median2, float, ctx, m, n, x, ex, tx, y, ey, ty, out, eout, pitch); }
-  PHY_DISPATCH_R(node_median2, double, ctx, m, n, x, ex, tx, y, ey, ty, out, eout, pitch);
+  const void* z = nullptr;
+  const int32_t* ez = nullptr;
+  const uint32_t* w = nullptr;
+  double* lnl = nullptr;
+  if (form & PHY_F32) {
+    PHY_DISPATCH_R(node_median, float, ctx, m, n, x, ex, tx, y, ey, ty, z, ez, 0.0, out, eout, pitch, w, lnl);
+  }
+  PHY_DISPATCH_R(node_median, double, ctx, m, n, x, ex, tx, y, ey, ty, z, ez, 0.0, out, eout, pitch, w, lnl);
+}
+
+int phy_lk_median3(phy_ctx* ctx, const phy_model* mc, int32_t form, int64_t n, const void* x,
+                   const int32_t* ex, double tx, const void* y, const int32_t* ey, double ty,
+                   const void* z, const int32_t* ez, double tz, void* out, int32_t* eout,
+                   int64_t pitch, const uint32_t* w, double* lnl) {
+  if (!ctx || !mc) return PHY_EINVAL;
+  phy_model* m = const_cast<phy_model*>(mc);
+  if (m->K != 4 || n < 1 || pitch < n || !x || !ex || !y || !ey || !z || !ez ||
+      (out == nullptr) != (eout == nullptr) || (!out && !lnl))
+    return set_err(ctx, PHY_EINVAL, "lk_median3: bad argument");
+  if (form & PHY_F32) {
+    PHY_DISPATCH_R(node_median, float, ctx, m, n, x, ex, tx, y, ey, ty, z, ez, tz, out, eout, pitch, w, lnl);
+  }
+  PHY_DISPATCH_R(node_median, double, ctx, m, n, x, ex, tx, y, ey, ty, z, ez, tz, out, eout, pitch, w, lnl);
+}
+
+int phy_lk_edge(phy_ctx* ctx, const phy_model* mc, int32_t form, int64_t n, const void* x,
+                const int32_t* ex, const void* y, const int32_t* ey, int64_t pitch, double t,
+                const uint32_t* w, double* d_out) {
+  if (!ctx || !mc) return PHY_EINVAL;
+  phy_model* m = const_cast<phy_model*>(mc);
+  if (m->K != 4 || n < 1 || pitch < n || !x || !ex || !y || !ey || !d_out || !(t >= 0.0) ||
+      !std::isfinite(t))
+    return set_err(ctx, PHY_EINVAL, "lk_edge: bad argument");
+  if (form & PHY_F32) { PHY_DISPATCH_R(node_edge, float, ctx, m, n, x, ex, y, ey, pitch, t, w, d_out); }
+  PHY_DISPATCH_R(node_edge, double, ctx, m, n, x, ex, y, ey, pitch, t, w, d_out);
+}
+
+int phy_lk_adjust3(phy_ctx* ctx, const phy_model* mc, int32_t form, int64_t n, const void* x,
+                   const int32_t* ex, const void* y, const int32_t* ey, int64_t pitch,
+                   const uint32_t* w, double t0, double t_lo, double t_hi, int32_t max_iter,
+                   double tol, double* t_out, double* lnl_out, int32_t* n_evals) {
+  if (!ctx || !mc) return PHY_EINVAL;
+  if (!t_out || !(t_lo >= 0.0) || !(t_hi >= t_lo) || !std::isfinite(t_hi) || max_iter < 0 ||
+      !(tol >= 0.0) || !std::isfinite(t0))
+    return set_err(ctx, PHY_EINVAL, "lk_adjust3: bad argument");
+  // the edge's lnL and derivatives at t, back on the host
+  double* d_res = nullptr;
+  PHY_HIP(ctx, hipMallocAsync((void**)&d_res, 3 * sizeof(double), ctx->stream));
+  int evals = 0;
+  auto eval = [&](double t, double* f) -> int {
+    int rc = phy_lk_edge(ctx, mc, form, n, x, ex, y, ey, pitch, t, w, d_res);
+    if (rc) return rc;
+    PHY_HIP(ctx, hipMemcpyAsync(f, d_res, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    PHY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ++evals;
+    return PHY_OK;
+  };
+  auto clamp = [&](double t) { return std::min(std::max(t, t_lo), t_hi); };
+  double t = clamp(t0), f[3];
+  int rc = eval(t, f);
+  // Newton-Raphson on lnL(t) with step halving; where d2 >= 0 (not a
+  // local maximum) a gradient step of size t (or 0.1 at t = 0) is taken
+  for (int it = 0; rc == PHY_OK && it < max_iter; ++it) {
+    double step = f[2] < 0.0 ? -f[1] / f[2] : (f[1] > 0.0 ? std::max(t, 0.1) : -0.5 * t);
+    double tn = clamp(t + step), fn[3] = {0, 0, 0};
+    bool better = false;
+    for (int h = 0; h < 30 && rc == PHY_OK; ++h) {
+      if (tn == t) break;
+      rc = eval(tn, fn);
+      if (rc) break;
+      if (fn[0] >= f[0]) {
+        better = true;
+        break;
+      }
+      step *= 0.5;
+      tn = clamp(t + step);
+    }
+    if (!better) break;
+    const double dt = std::fabs(tn - t);
+    t = tn;
+    f[0] = fn[0], f[1] = fn[1], f[2] = fn[2];
+    if (dt <= tol * std::max(t, 1e-8)) break;
+  }
+  hipError_t fe = hipFreeAsync(d_res, ctx->stream);
+  if (rc) return rc;
+  if (fe != hipSuccess) return set_err(ctx, PHY_EDEVICE, "lk_adjust3: hipFreeAsync failed");
+  *t_out = t;
+  if (lnl_out) *lnl_out = f[0];
+  if (n_evals) *n_evals = evals;
+  return PHY_OK;
 }
 
 int phy_lk_root(phy_ctx* ctx, const phy_model* mc, int32_t form, int64_t n, const void* L,

@@ -16,9 +16,11 @@ Ptopology store, and tree / component costs.
   final sets top-down with readjust_3 and the leaf rule);
   ``diagnose_tree`` fills the same store from the whole-tree kernels (one
   post-order and one pre-order launch).  Both give the same sets and costs.
-* ``lnl_nodes`` / ``lnl_tree`` are the same pair for the likelihood
-  NodeData (``likelihood.leaf`` / ``median_2`` / ``root_lnl`` against
-  ``likelihood.Evaluator``).
+* ``diagnose_lnl_nodes`` runs the likelihood NodeData (``likelihood``)
+  through the same ``Make1D``: median_2 in post-order, the root cost -lnL,
+  and optionally a pre-order sweep of readjust_3 that optimises every branch
+  length by Newton-Raphson against the value of the rest of the tree (built
+  with median_2); ``lnl_tree`` is the whole-tree kernel's lnL.
 """
 from __future__ import annotations
 
@@ -46,7 +48,8 @@ def _sum(f, d: dict) -> float:
 
 
 class Make1D:
-    """lib/node.ml:59-279 with NodeData = ``nonadditive`` on device ``ctx``."""
+    """lib/node.ml:59-279 over a NodeData module ``nd`` (``nonadditive`` or
+    ``likelihood``) on device ``ctx``."""
 
     def __init__(self, ctx, nd=NA):
         self.ctx, self.nd = ctx, nd
@@ -85,12 +88,16 @@ class Make1D:
         return x.topot
 
     def compare(self, a: Node, b: Node) -> int:
-        """Order of the preliminary values (codes, then per-site sets)."""
-        def key(x: Node):
-            return [(k, self.nd.to_sets(x.prelim[k]).tobytes()) for k in sorted(x.prelim)]
-
-        ka, kb = key(a), key(b)
-        return (ka > kb) - (ka < kb)
+        """Order of the preliminary values: the codes, then NodeData.compare
+        of each code's values in code order."""
+        ka, kb = sorted(a.prelim), sorted(b.prelim)
+        if ka != kb:
+            return (ka > kb) - (ka < kb)
+        for k in ka:
+            c = self.nd.compare(a.prelim[k], b.prelim[k])
+            if c:
+                return c
+        return 0
 
     @staticmethod
     def filter_codes(cs, x: Node) -> Node:
@@ -138,16 +145,20 @@ class Make1D:
         return Node(prelim, None, min(a.topot, b.topot), max(a.height, b.height) + 1,
                     _sum(self.nd.cost, prelim), int(id))
 
-    def readjust_3(self, m, codes, n: Node, a: Node, b: Node, c: Node, prelim: bool = False):
-        """Final sets of n from its children a, b and its parent's final c
-        (lib/node.ml:238-256); returns (node, changed codes)."""
+    def readjust_3(self, m, codes, n: Node, a: Node | None, b: Node | None, c: Node,
+                   prelim: bool = False):
+        """NodeData.adjust_3 of n from its children a, b and c (lib/node.ml:
+        238-256): for Fitch, c is the parent's final sets; for likelihood, the
+        value of the rest of the tree at the parent, and a = b = None for a
+        leaf.  Returns (node, changed codes)."""
         def val(x: Node, k):
             return x.prelim[k] if prelim or x.final is None else x.final[k]
 
         out, changed = {}, set()
         for k in n.prelim:
-            v, s = self.nd.adjust_3(self.ctx, m, codes, val(n, k), a.prelim[k], b.prelim[k],
-                                    val(c, k))
+            v, s = self.nd.adjust_3(self.ctx, m, codes, val(n, k),
+                                    None if a is None else a.prelim[k],
+                                    None if b is None else b.prelim[k], val(c, k))
             out[k] = v
             changed |= s
         return replace(n, prelim=out, final=out), changed
@@ -164,6 +175,13 @@ class Make1D:
             t = self.nd.median_2(self.ctx, m, None, f, lv)
             fin[k] = replace(lv, states=self.nd.median_2(self.ctx, m, None, t, lv).states)
         return replace(leaf, final=fin)
+
+    def with_branch(self, x: Node, t: float) -> Node:
+        """x with every value's branch length set to t (NodeData.S carries no
+        branch lengths; the likelihood values hold their own)."""
+        return replace(x, prelim={k: self.nd.with_blen(v, t) for k, v in x.prelim.items()},
+                       final=None if x.final is None else
+                       {k: self.nd.with_blen(v, t) for k, v in x.final.items()})
 
     @staticmethod
     def root_cost(_, x: Node) -> float:
@@ -338,18 +356,72 @@ def diagnose_tree(ctx, topo, leaf_sets: np.ndarray | None, weights=None, code: i
     return _finish(pt, N, ch, topo.root, total, handle)
 
 
-def lnl_nodes(ctx, dm, N: int, children, root: int, blen, tip_sets: np.ndarray, weights,
-              form: int) -> float:
-    """lnL by likelihood.median_2 node by node (root_lnl at the root)."""
-    from . import likelihood as LK
-
+def diagnose_lnl_nodes(mk: Make1D, m, N: int, children, root: int, leaves: dict, blen,
+                       handle: int = 0, adjust: bool = False, codes=None):
+    """Likelihood diagnosis node by node through ``mk`` (NodeData =
+    ``likelihood``, m its DeviceModel): leaves[id] = {code: value}; every
+    value gets the tree's branch length blen[id].  median_2 in post-order;
+    the root's cost is -lnL.  With ``adjust``, one pre-order sweep of
+    readjust_3 over every non-root node v: its branch is optimised by Newton
+    on the edge between v's value and C_v, the value at v's parent of the
+    rest of the tree (at a root child: the sibling moved to the root, at any
+    other node: median_2 of the sibling and the parent's C), and each
+    internal value is recomputed once its children are done, so the next
+    edges see the new lengths.  Every step keeps or raises lnL.
+    Returns (store, lnL, branch lengths)."""
     children = np.asarray(children)
-    vals = {i: LK.leaf(ctx, dm, tip_sets[i], blen[i], weights, form) for i in range(N)}
-    for v in _post_order(N, children, root):
-        k = 2 * (v - N)
-        vals[v] = LK.median_2(ctx, dm, None, vals[int(children[k])], vals[int(children[k + 1])],
-                              blen=float(blen[v]))
-    return LK.root_lnl(ctx, dm, vals[root])
+    t = np.array(blen, np.float64)
+    pt = Ptopology()
+    for i in range(N):
+        pt = pt.add_node_data(i, mk.with_branch(mk.of_data(i, leaves[i]), t[i]))
+    order = _post_order(N, children, root)
+    kids = {v: (int(children[2 * (v - N)]), int(children[2 * (v - N) + 1])) for v in order}
+
+    def down(v):
+        a, b = kids[v]
+        n = mk.median_2(m, v, None, pt.get_node_data(a), pt.get_node_data(b))
+        return mk.with_branch(n, t[v])
+
+    for v in order:
+        pt = pt.add_node_data(v, down(v))
+    if adjust:
+        nd = mk.nd
+
+        def up_value(v, sib, c_parent):
+            """C_v as a Node: the sibling joined with the parent's C (or the
+            sibling alone, moved to the root), located at v's parent."""
+            s_node = pt.get_node_data(sib)
+            vals = {}
+            for k, sv in s_node.prelim.items():
+                other = (nd.missing(mk.ctx, m, sv) if c_parent is None
+                         else c_parent.prelim[k])
+                vals[k] = nd.median_2(mk.ctx, m, None, sv, other, blen=0.0)
+            return Node(vals, None, s_node.topot, s_node.height, 0.0, v)
+
+        def visit(v, c_v):
+            nonlocal pt
+            a, b = kids.get(v, (None, None))
+            n = pt.get_node_data(v)
+            an = None if a is None else pt.get_node_data(a)
+            bn = None if b is None else pt.get_node_data(b)
+            n2, _ = mk.readjust_3(m, codes, n, an, bn, c_v, prelim=True)
+            t[v] = next(iter(n2.prelim.values())).blen
+            pt = pt.add_node_data(v, replace(n2, final=None))
+            if a is None:
+                return
+            # the parent's C seen from below: c_v moved down v's branch
+            c_here = replace(c_v, prelim={k: nd.with_blen(x, t[v]) for k, x in c_v.prelim.items()})
+            visit(a, up_value(a, b, c_here))
+            visit(b, up_value(b, a, c_here))
+            pt = pt.add_node_data(v, down(v))
+
+        ra, rb = kids[root]
+        visit(ra, up_value(ra, rb, None))
+        visit(rb, up_value(rb, ra, None))
+        pt = pt.add_node_data(root, down(root))
+    rn = pt.get_node_data(root)
+    lnl = -mk.root_cost(m, rn)
+    return _finish(pt, N, children, root, -lnl, handle), lnl, t
 
 
 def lnl_tree(ctx, topo, dm, blen, tip_sets: np.ndarray, weights, form: int) -> float:
@@ -406,7 +478,7 @@ def main(argv=None):
 
 
 __all__ = ["Node", "Make1D", "RootData", "Ptopology", "diagnose_nodes", "diagnose_tree",
-           "lnl_nodes", "lnl_tree"]
+           "diagnose_lnl_nodes", "lnl_tree"]
 
 
 if __name__ == "__main__":

@@ -3,14 +3,20 @@ signature (lib/nodeData.ml:3-35; every function is TODO in the reference,
 lib/likelihood_c.ml:1-33).
 
 A value holds a node's conditional likelihoods on the device -- partials
-L[r][i][p] (FP32 or FP64, pattern-contiguous) with per-pattern int32
-power-of-two exponents -- plus the length of the branch to its parent.
-NodeData.S has no branch-length argument, so the value carries it (``blen``).
-NodeData costs are minimised, so ``cost`` / ``root_cost`` are -lnL.
+L[r][i][p] (FP32 or FP64, pattern-contiguous) with an int32 power-of-two
+exponent per (rate class, pattern) -- plus the length of the branch from its
+node to wherever the next operation joins it.  NodeData.S has no
+branch-length argument, so the value carries it (``blen``).  NodeData costs
+are minimised, so ``cost`` / ``root_cost`` are -lnL.
 
+The NodeData operations take the arguments ``Make1D`` passes to every
+NodeData module (``ctx``, the model ``m`` = a ``DeviceModel``, the previous
+value ``prev`` of the same node or None):
+  median_1 / median_2 / median_3, distance_1 / distance_2, adjust_3 (Newton
+  on the edge between a node and the rest of the tree), cost / root_cost /
+  leaf_cost, and the character bookkeeping.
 ``Evaluator`` runs a whole tree in one pass (lnL and optional per-branch
-first/second derivatives for Newton branch-length updates); ``newton_step``
-is the adjust_3 branch-length update built on them.
+first/second derivatives for simultaneous Newton branch-length updates).
 """
 from __future__ import annotations
 
@@ -57,18 +63,31 @@ class DeviceModel:
 @dataclass
 class Likelihood:
     partial: object      # device [R*K, pitch] float32/float64
-    exps: object         # device int32 [pitch]
-    blen: float          # branch to the parent
+    exps: object         # device int32 [R, pitch]: value = partial * 2**exps
+    blen: float          # branch from this node to the next join
     n_patterns: int
     weights: object = None  # device uint32 [n_patterns] (pattern weights)
     form: int = PHY_F64
-    lnl: float | None = None
+    lnl: float | None = None   # lnL with this value as the root (set by root_cost / median_3)
     codes: list = field(default_factory=list)  # character code of each pattern
+    dm: object = None    # the DeviceModel that made it (cost evaluates through it)
+
+    @property
+    def n_sites(self) -> int:
+        return self.n_patterns
+
+    @property
+    def pitch(self) -> int:
+        return int(self.partial.shape[1])
 
 
 def _dtype(form):
     torch = __import__("torch")
     return torch.float32 if form & PHY_F32 else torch.float64
+
+
+def _dev(ctx):
+    return f"cuda:{ctx.device}"
 
 
 def leaf(ctx, dm: DeviceModel, tip_sets: np.ndarray, blen: float, weights=None,
@@ -78,35 +97,138 @@ def leaf(ctx, dm: DeviceModel, tip_sets: np.ndarray, blen: float, weights=None,
     n = int(tip_sets.size)
     pitch = ((n + 63) // 64) * 64
     tip = D.to_device(D.pack_nibbles(np.asarray(tip_sets, np.uint8)), ctx.device)
-    part = torch.empty((dm.R * dm.K, pitch), dtype=_dtype(form), device=f"cuda:{ctx.device}")
-    ex = torch.zeros((pitch,), dtype=torch.int32, device=f"cuda:{ctx.device}")
+    part = torch.empty((dm.R * dm.K, pitch), dtype=_dtype(form), device=_dev(ctx))
+    ex = torch.zeros((dm.R, pitch), dtype=torch.int32, device=_dev(ctx))
     ctx.check(lib.phy_lk_leaf(ctx.handle, dm.handle, form, n, D.dptr(tip), D.dptr(part),
                               D.dptr(ex), pitch), "lk_leaf")
     w = weights if weights is None or not isinstance(weights, np.ndarray) else \
         D.to_device(np.asarray(weights, np.uint32), ctx.device)
-    return Likelihood(part, ex, float(blen), n, w, form)
+    return Likelihood(part, ex, float(blen), n, w, form, dm=dm)
 
 
-def median_2(ctx, dm: DeviceModel, prev, x: Likelihood, y: Likelihood,
-             blen: float = 0.0) -> Likelihood:
-    """One pruning step: L = (P(r t_x) L_x) (*) (P(r t_y) L_y) for every rate r."""
+def missing(ctx, dm: DeviceModel, like: Likelihood, blen: float = 0.0) -> Likelihood:
+    """A value that carries no information (every state possible at every
+    pattern, L = 1): joining it changes nothing, so median_2(x, missing)
+    moves x along its branch."""
     torch = __import__("torch")
-    pitch = x.partial.shape[1]
-    if prev is not None and prev.partial.shape == x.partial.shape:
-        out, ex = prev.partial, prev.exps
-    else:
-        out, ex = torch.empty_like(x.partial), torch.empty_like(x.exps)
-    ctx.check(lib.phy_lk_median2(ctx.handle, dm.handle, x.form, x.n_patterns, D.dptr(x.partial),
+    part = torch.ones_like(like.partial)
+    ex = torch.zeros_like(like.exps)
+    return replace(like, partial=part, exps=ex, blen=float(blen), lnl=None)
+
+
+def _out(prev, like: Likelihood):
+    torch = __import__("torch")
+    if prev is not None and prev.partial.shape == like.partial.shape and \
+            prev.partial.dtype == like.partial.dtype:
+        return prev.partial, prev.exps
+    return torch.empty_like(like.partial), torch.empty_like(like.exps)
+
+
+def _check_pair(x: Likelihood, y: Likelihood, what: str):
+    if x.n_patterns != y.n_patterns or x.pitch != y.pitch or x.form != y.form:
+        raise ValueError(f"{what}: values of different pattern sets or forms")
+
+
+def with_blen(x: Likelihood, t: float) -> Likelihood:
+    """x with branch length t (the same device buffers)."""
+    return replace(x, blen=float(t))
+
+
+def median_1(m, prev, x: Likelihood) -> Likelihood:
+    """median_1 (lib/likelihood_c.ml:14): a node with one child is its child."""
+    return x
+
+
+def median_2(ctx, m: DeviceModel, prev, x: Likelihood, y: Likelihood,
+             blen: float | None = None) -> Likelihood:
+    """median_2: L[r] = (P(r t_x) L_x[r]) (*) (P(r t_y) L_y[r]) for every
+    rate class r.  The result's branch length is ``blen``, else that of
+    ``prev`` (the node's previous value), else 0."""
+    _check_pair(x, y, "median_2")
+    out, ex = _out(prev, x)
+    ctx.check(lib.phy_lk_median2(ctx.handle, m.handle, x.form, x.n_patterns, D.dptr(x.partial),
                                  D.dptr(x.exps), x.blen, D.dptr(y.partial), D.dptr(y.exps),
-                                 y.blen, D.dptr(out), D.dptr(ex), pitch), "lk_median2")
-    return replace(x, partial=out, exps=ex, blen=float(blen), lnl=None)
+                                 y.blen, D.dptr(out), D.dptr(ex), x.pitch), "lk_median2")
+    t = blen if blen is not None else (prev.blen if prev is not None else 0.0)
+    return replace(x, partial=out, exps=ex, blen=float(t), lnl=None, dm=m)
+
+
+def _lnl_buf(ctx):
+    torch = __import__("torch")
+    return torch.zeros((3,), dtype=torch.float64, device=_dev(ctx))
+
+
+def median_3(ctx, m: DeviceModel, prev, x: Likelihood, y: Likelihood, z: Likelihood,
+             blen: float | None = None) -> Likelihood:
+    """median_3 (the unrooted root of Node.Make1D.median_3, lib/node.ml:200-216):
+    the join of three neighbours, with its lnL (so ``cost`` is -lnL of the
+    tree seen from this node)."""
+    _check_pair(x, y, "median_3")
+    _check_pair(x, z, "median_3")
+    out, ex = _out(prev, x)
+    buf = _lnl_buf(ctx)
+    w = x.weights
+    ctx.check(lib.phy_lk_median3(ctx.handle, m.handle, x.form, x.n_patterns, D.dptr(x.partial),
+                                 D.dptr(x.exps), x.blen, D.dptr(y.partial), D.dptr(y.exps),
+                                 y.blen, D.dptr(z.partial), D.dptr(z.exps), z.blen, D.dptr(out),
+                                 D.dptr(ex), x.pitch, D.dptr(w), D.dptr(buf)), "lk_median3")
+    t = blen if blen is not None else (prev.blen if prev is not None else 0.0)
+    return replace(x, partial=out, exps=ex, blen=float(t), lnl=float(buf.cpu()[0]), dm=m)
+
+
+def edge(ctx, m: DeviceModel, x: Likelihood, y: Likelihood, t: float) -> tuple:
+    """(lnL, dlnL/dt, d2lnL/dt2) of x and y joined by one edge of length t."""
+    _check_pair(x, y, "edge")
+    buf = _lnl_buf(ctx)
+    ctx.check(lib.phy_lk_edge(ctx.handle, m.handle, x.form, x.n_patterns, D.dptr(x.partial),
+                              D.dptr(x.exps), D.dptr(y.partial), D.dptr(y.exps), x.pitch,
+                              float(t), D.dptr(x.weights), D.dptr(buf)), "lk_edge")
+    v = buf.cpu().numpy()
+    return float(v[0]), float(v[1]), float(v[2])
+
+
+def distance_1(ctx, m: DeviceModel, a: Likelihood, b: Likelihood) -> float:
+    """distance_1 (lib/likelihood_c.ml:23): -lnL of a and b joined across
+    their two branches (one edge of length t_a + t_b)."""
+    return -edge(ctx, m, a, b, a.blen + b.blen)[0]
+
+
+def distance_2(ctx, m: DeviceModel, a: Likelihood, b: Likelihood, c: Likelihood) -> float:
+    """distance_2 (lib/likelihood_c.ml:24): -lnL of a, b and c joined at one
+    point (c inserted on the edge between a and b, each at its own branch
+    length from the insertion point)."""
+    return -median_3(ctx, m, None, a, b, c).lnl
+
+
+def adjust_3(ctx, m: DeviceModel, codes, n: Likelihood, a: Likelihood, b: Likelihood,
+             c: Likelihood, max_iter: int = 20, tol: float = 1e-8, t_lo: float = 1e-8,
+             t_hi: float = 10.0):
+    """adjust_3 (lib/likelihood_c.ml:18) as Node.Make1D.readjust_3 calls it
+    (lib/node.ml:239-256): n's own value is recomputed from its children a
+    and b, then the length of the edge between n and c -- the value of the
+    rest of the tree at n's parent -- is optimised by Newton-Raphson (a leaf
+    has a = b = None and keeps its value).  Returns (n', changed codes): n's codes when the length moved (limited to
+    ``codes`` when given; with ``codes`` sharing nothing with n, n is
+    returned unchanged)."""
+    if codes is not None and not (set(_codes(n)) & set(codes)):
+        return n, set()
+    d = n if a is None else median_2(ctx, m, None, a, b, blen=n.blen)
+    t, lnl, ev = C.c_double(), C.c_double(), C.c_int32()
+    ctx.check(lib.phy_lk_adjust3(ctx.handle, m.handle, d.form, d.n_patterns, D.dptr(d.partial),
+                                 D.dptr(d.exps), D.dptr(c.partial), D.dptr(c.exps), d.pitch,
+                                 D.dptr(d.weights), float(n.blen), t_lo, t_hi, max_iter, tol,
+                                 C.byref(t), C.byref(lnl), C.byref(ev)), "lk_adjust3")
+    out = replace(d, blen=float(t.value), lnl=None)
+    changed = set(_codes(n)) if t.value != n.blen else set()
+    if codes is not None:
+        changed &= set(codes)
+    return out, changed
 
 
 def root_lnl(ctx, dm: DeviceModel, x: Likelihood) -> float:
-    torch = __import__("torch")
-    out = torch.zeros((1,), dtype=torch.float64, device=f"cuda:{ctx.device}")
+    out = _lnl_buf(ctx)
     ctx.check(lib.phy_lk_root(ctx.handle, dm.handle, x.form, x.n_patterns, D.dptr(x.partial),
-                              D.dptr(x.exps), x.partial.shape[1], D.dptr(x.weights),
+                              D.dptr(x.exps), x.pitch, D.dptr(x.weights),
                               D.dptr(out)), "lk_root")
     x.lnl = float(out.cpu().numpy()[0])  # kept for cost
     return x.lnl
@@ -118,10 +240,13 @@ def root_cost(ctx, dm: DeviceModel, x: Likelihood) -> float:
 
 
 def cost(x: Likelihood) -> float:
-    """NodeData.cost: -lnL of the tree rooted at x, as the last root_lnl /
-    root_cost of x evaluated it (median_2 results start unevaluated)."""
+    """NodeData.cost: -lnL with x as the root (the patterns' likelihood
+    under x's subtree, or the whole tree for a median_3 value); evaluated
+    on first use through the model that made x."""
     if x.lnl is None:
-        raise ValueError("cost: evaluate root_cost(ctx, dm, x) first")
+        if x.dm is None:
+            raise ValueError("cost: the value has no model; evaluate root_cost(ctx, dm, x)")
+        return -root_lnl(x.dm.ctx, x.dm, x)
     return -x.lnl
 
 
@@ -163,8 +288,8 @@ def _select(x: Likelihood, keep: list):
     pitch = ((n + 63) // 64) * 64
     part = torch.ones((x.partial.shape[0], pitch), dtype=x.partial.dtype, device=dev)
     part[:, :n] = x.partial.index_select(1, idx)
-    ex = torch.zeros((pitch,), dtype=x.exps.dtype, device=dev)
-    ex[:n] = x.exps.index_select(0, idx)
+    ex = torch.zeros((x.exps.shape[0], pitch), dtype=x.exps.dtype, device=dev)
+    ex[:, :n] = x.exps.index_select(1, idx)
     w = None if x.weights is None else \
         x.weights.view(torch.int32).index_select(0, idx).view(torch.uint32)
     codes = _codes(x)
@@ -186,22 +311,23 @@ def filter_codes_comp(codes, x: Likelihood):
 
 def compare(a: Likelihood, b: Likelihood) -> int:
     """A total order: codes, then branch length, then the scaled partials
-    pattern by pattern (exponent first, then the values)."""
+    pattern by pattern (exponents first, then the values)."""
     ka, kb = _codes(a), _codes(b)
     if ka != kb:
         return -1 if ka < kb else 1
     if a.blen != b.blen:
         return -1 if a.blen < b.blen else 1
     n = a.n_patterns
-    ea, eb = a.exps[:n], b.exps[:n]
+    ea, eb = a.exps[:, :n], b.exps[:, :n]
     pa, pb = a.partial[:, :n], b.partial[:, :n]
-    diff = (ea != eb) | (pa != pb).any(0)
+    diff = (ea != eb).any(0) | (pa != pb).any(0)
     nz = diff.nonzero()
     if not nz.numel():
         return 0
     p = int(nz[0, 0])
-    if int(ea[p]) != int(eb[p]):
-        return -1 if int(ea[p]) < int(eb[p]) else 1
+    col_ea, col_eb = ea[:, p].cpu().tolist(), eb[:, p].cpu().tolist()
+    if col_ea != col_eb:
+        return -1 if col_ea < col_eb else 1
     col_a, col_b = pa[:, p].cpu().tolist(), pb[:, p].cpu().tolist()
     return -1 if col_a < col_b else 1
 
@@ -266,6 +392,7 @@ def newton_step(blen: np.ndarray, d1: np.ndarray, d2: np.ndarray, root: int,
     return t
 
 
-__all__ = ["DeviceModel", "Likelihood", "Evaluator", "leaf", "median_2", "root_lnl",
+__all__ = ["DeviceModel", "Likelihood", "Evaluator", "leaf", "missing", "with_blen", "median_1", "median_2",
+           "median_3", "edge", "distance_1", "distance_2", "adjust_3", "root_lnl",
            "root_cost", "cost", "leaf_cost", "cardinal", "get_codes", "mem", "recode", "filter_codes",
            "filter_codes_comp", "compare", "newton_step", "PHY_F32", "PHY_F64", "PHY_FUSED"]

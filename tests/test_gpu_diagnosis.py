@@ -113,7 +113,9 @@ def test_likelihood_node_by_node_equals_whole_tree(ctx, rng, form, tol):
     tips = (1 << rng.integers(0, 4, (N, P))).astype(np.uint8)
     w = rng.integers(1, 5, P).astype(np.uint32)
     dm = LK.DeviceModel(ctx, model)
-    a = G.lnl_nodes(ctx, dm, N, ch, root, bl, tips, w, form)
+    wd = D.to_device(w, ctx.device)
+    leaves = {i: {0: LK.leaf(ctx, dm, tips[i], bl[i], wd, form)} for i in range(N)}
+    _, a, _ = G.diagnose_lnl_nodes(G.Make1D(ctx, LK), dm, N, ch, root, leaves, bl)
     b = G.lnl_tree(ctx, D.Topology(N, ch, root, ctx), dm, bl, tips, w, form)
     ov = O.lnl(N, ch, root, bl, model.priors, model.q, model.rates, model.probs,
                tips.astype(np.uint32), w, threads=min(16, os.cpu_count() or 1))[0]

@@ -1,0 +1,256 @@
+"""GPU parity of the per-node Likelihood_c operations (lib/likelihood_c.ml:
+11-24, all TODO in the reference) as Node.Make1D calls them
+(lib/node.ml:154-256), against the FP64 oracle.
+
+Every per-node quantity is checked against a whole-tree oracle value that
+the mathematics fixes: for a reversible model the likelihood is the same
+seen from any node or edge (pulley principle), so
+  * edge(D_v, C_v, t_v) -- v's conditional value joined across its branch to
+    the rest of the tree -- gives the tree's lnL and the oracle's d1/d2 of
+    branch v;
+  * median_3 of a node's three neighbours gives the tree's lnL;
+  * distance_1 of the root's children is -lnL, distance_2 = -median_3 lnL;
+  * adjust_3 ends where the oracle's d1 of that branch vanishes, at the
+    maximiser a golden-section search on the oracle lnL finds.
+Tolerances as in test_gpu_prune.py: FP64 lnL 1e-10, FP32 1e-6 relative;
+derivatives against the branch's term magnitudes (FP64 1e-9, FP32 1e-3)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import random_tree
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+F64, F32 = 0, 1
+TOL = {F64: 1e-10, F32: 1e-6}
+DTOL = {F64: 1e-9, F32: 1e-3}
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from phylocaml_amd import device as D
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return D.Context(0)
+
+
+def _model(rng, mode):
+    from phylocaml_amd import mlmodel as MM
+
+    pi = rng.dirichlet(np.full(4, 5.0))
+    ex = rng.dirichlet(np.ones(6))
+    return MM.create(MM.Spec(4, ("GTR", (ex / ex[-1])[:-1]), ("DiscreteGamma", 4, 0.5),
+                             ("Empirical", pi), mode))
+
+
+def _case(rng, N, P, mode):
+    ch, root = random_tree(rng, N)
+    bl = rng.exponential(0.1, 2 * N - 1) + 1e-3
+    bl[root] = 0.0
+    model = _model(rng, mode)
+    tips = (1 << rng.integers(0, 4, (N, P))).astype(np.uint8)
+    amb = rng.random((N, P)) < 0.1
+    tips[amb] |= (1 << rng.integers(0, 4, amb.sum())).astype(np.uint8)
+    tips[rng.random((N, P)) < 0.03] = 15
+    w = rng.integers(1, 6, P).astype(np.uint32)
+    return ch, root, bl, model, tips, w
+
+
+def _oracle(N, ch, root, bl, model, tips, w, derivs=True):
+    return O.lnl(N, ch, root, bl, model.priors, model.q, model.rates, model.probs,
+                 tips.astype(np.uint32), w, derivs=derivs, scales=derivs,
+                 threads=min(16, os.cpu_count() or 1))
+
+
+def _values(ctx, dm, N, ch, root, bl, tips, w, form):
+    """D[v] (conditional value of v's subtree, branch t_v) for every node and
+    C[v] (the rest of the tree at v's parent) for every non-root node, from
+    median_2 alone."""
+    from phylocaml_amd import device as Dv, likelihood as LK
+
+    wd = Dv.to_device(w, ctx.device)
+    D, C = {}, {}
+    for i in range(N):
+        D[i] = LK.leaf(ctx, dm, tips[i], bl[i], wd, form)
+    kids = {N + k: (int(ch[2 * k]), int(ch[2 * k + 1])) for k in range(N - 1)}
+
+    def down(v):
+        if v < N:
+            return
+        a, b = kids[v]
+        down(a)
+        down(b)
+        D[v] = LK.median_2(ctx, dm, None, D[a], D[b], blen=bl[v])
+
+    down(root)
+
+    def up(v, cv):
+        C[v] = cv
+        if v < N:
+            return
+        a, b = kids[v]
+        c_here = LK.with_blen(cv, bl[v])
+        up(a, LK.median_2(ctx, dm, None, D[b], c_here, blen=0.0))
+        up(b, LK.median_2(ctx, dm, None, D[a], c_here, blen=0.0))
+
+    ra, rb = kids[root]
+    up(ra, LK.median_2(ctx, dm, None, D[rb], LK.missing(ctx, dm, D[rb]), blen=0.0))
+    up(rb, LK.median_2(ctx, dm, None, D[ra], LK.missing(ctx, dm, D[ra]), blen=0.0))
+    return D, C, kids
+
+
+@pytest.mark.parametrize("form", [F64, F32])
+@pytest.mark.parametrize("mode", [0, 1])  # PHY_GAMMA_REF (rate_0 = 0), PHY_GAMMA_MEAN
+def test_edge_median3_distances_match_oracle(ctx, rng, form, mode):
+    from phylocaml_amd import likelihood as LK
+
+    N, P = 40, 900
+    ch, root, bl, model, tips, w = _case(rng, N, P, mode)
+    _say(f"per-node likelihood N={N} P={P} form={form} gamma mode={mode}")
+    dm = LK.DeviceModel(ctx, model)
+    D, C, kids = _values(ctx, dm, N, ch, root, bl, tips, w, form)
+    ov, _, od1, od2, a1, a2 = _oracle(N, ch, root, bl, model, tips, w)
+    tol, dtol = TOL[form], DTOL[form]
+    assert abs(LK.root_lnl(ctx, dm, D[root]) - ov) <= tol * abs(ov)
+    worst = 0.0
+    for v in range(2 * N - 1):
+        if v == root:
+            continue
+        f, g, h = LK.edge(ctx, dm, D[v], C[v], bl[v])
+        assert abs(f - ov) <= tol * abs(ov), (v, f, ov)
+        e = max(abs(g - od1[v]) / a1[v], abs(h - od2[v]) / a2[v])
+        worst = max(worst, e)
+        assert e <= dtol, (v, g, od1[v], h, od2[v])
+        if v >= N:
+            a, b = kids[v]
+            m3 = LK.median_3(ctx, dm, None, D[a], D[b], LK.with_blen(C[v], bl[v]))
+            assert abs(m3.lnl - ov) <= tol * abs(ov), (v, m3.lnl, ov)
+            assert LK.cost(m3) == -m3.lnl
+            d2 = LK.distance_2(ctx, dm, D[a], D[b], LK.with_blen(C[v], bl[v]))
+            assert abs(d2 + ov) <= tol * abs(ov)
+    _say(f"edge derivatives: worst error / term magnitude {worst:.3e}")
+    ra, rb = kids[root]
+    assert abs(LK.distance_1(ctx, dm, D[ra], D[rb]) + ov) <= tol * abs(ov)
+
+
+def _golden_max(f, lo, hi, iters=80):
+    g = (np.sqrt(5) - 1) / 2
+    a, b = lo, hi
+    c, d = b - g * (b - a), a + g * (b - a)
+    fc, fd = f(c), f(d)
+    for _ in range(iters):
+        if fc > fd:
+            b, d, fd = d, c, fc
+            c = b - g * (b - a)
+            fc = f(c)
+        else:
+            a, c, fc = c, d, fd
+            d = a + g * (b - a)
+            fd = f(d)
+    return 0.5 * (a + b)
+
+
+@pytest.mark.parametrize("form", [F64, F32])
+def test_adjust3_reaches_the_oracle_maximum(ctx, rng, form):
+    from phylocaml_amd import likelihood as LK
+
+    N, P = 24, 1500
+    ch, root, bl, model, tips, w = _case(rng, N, P, 1)
+    dm = LK.DeviceModel(ctx, model)
+    D, C, kids = _values(ctx, dm, N, ch, root, bl, tips, w, form)
+    ov0 = _oracle(N, ch, root, bl, model, tips, w, False)[0]
+    picks = [v for v in range(2 * N - 1) if v != root][:: max(1, (2 * N - 2) // 8)]
+    for v in picks:
+        _say(f"adjust_3 branch {v} form={form}")
+        a, b = kids.get(v, (None, None))
+        n = D[v]
+        out, changed = LK.adjust_3(ctx, dm, None, n,
+                                   None if a is None else D[a], None if b is None else D[b],
+                                   C[v])
+        t = out.blen
+        bl2 = bl.copy()
+        bl2[v] = t
+        o = _oracle(N, ch, root, bl2, model, tips, w)
+        assert o[0] >= ov0 - 1e-10 * abs(ov0)
+        # at the maximum the branch's d1 vanishes and d2 < 0 (unless the
+        # length sits at its lower bound)
+        if t > 1e-7:
+            dtol = 1e-6 if form == F64 else 1e-3
+            assert abs(o[2][v]) <= dtol * o[4][v], (v, o[2][v], o[4][v])
+            assert o[3][v] < 0
+
+            def f(x):
+                b3 = bl.copy()
+                b3[v] = x
+                return _oracle(N, ch, root, b3, model, tips, w, False)[0]
+
+            tg = _golden_max(f, t * 0.5, t * 1.5 + 1e-6)
+            assert abs(tg - t) <= (1e-4 if form == F64 else 3e-3) * t, (v, t, tg)
+        assert changed == (set(range(P)) if t != bl[v] else set())
+
+
+@pytest.mark.parametrize("form", [F64, F32])
+def test_make1d_likelihood_diagnosis(ctx, rng, form):
+    """The likelihood NodeData through diagnosis.Make1D: node by node equals
+    the whole-tree kernel and the oracle; the readjust_3 sweep never lowers
+    lnL and its result is the oracle's lnL at the returned lengths."""
+    from phylocaml_amd import device as Dv, diagnosis as G, likelihood as LK
+
+    N, P = 30, 700
+    ch, root, bl, model, tips, w = _case(rng, N, P, 1)
+    dm = LK.DeviceModel(ctx, model)
+    mk = G.Make1D(ctx, LK)
+    wd = Dv.to_device(w, ctx.device)
+    leaves = {i: {0: LK.leaf(ctx, dm, tips[i], bl[i], wd, form)} for i in range(N)}
+    pt, lnl, t = G.diagnose_lnl_nodes(mk, dm, N, ch, root, leaves, bl)
+    ov = _oracle(N, ch, root, bl, model, tips, w, False)[0]
+    whole = G.lnl_tree(ctx, Dv.Topology(N, ch, root, ctx), dm, bl, tips, w, form)
+    tol = TOL[form]
+    assert abs(lnl - ov) <= tol * abs(ov) and abs(whole - ov) <= tol * abs(ov)
+    assert abs(lnl - whole) <= tol * abs(ov)
+    assert pt.get_comp_root(0).component_cost == -lnl
+    assert np.array_equal(t, bl)
+    pt2, lnl2, t2 = G.diagnose_lnl_nodes(mk, dm, N, ch, root, leaves, bl, adjust=True)
+    ov2 = _oracle(N, ch, root, t2, model, tips, w, False)[0]
+    _say(f"readjust sweep: lnL {lnl:.6f} -> {lnl2:.6f} (oracle at new lengths {ov2:.6f})")
+    assert lnl2 >= lnl - tol * abs(lnl)
+    assert ov2 > ov
+    assert abs(lnl2 - ov2) <= tol * abs(ov2)
+    r = pt2.get_node_data(root)
+    assert mk.compare(r, r) == 0 and mk.cardinal(r) == P
+
+
+def test_argument_errors(ctx, rng):
+    from phylocaml_amd import _lib, likelihood as LK
+
+    N, P = 4, 50
+    ch, root, bl, model, tips, w = _case(rng, N, P, 1)
+    dm = LK.DeviceModel(ctx, model)
+    x = LK.leaf(ctx, dm, tips[0], 0.1, w)
+    y = LK.leaf(ctx, dm, tips[1], 0.1, w)
+    with pytest.raises(_lib.PhyError):
+        LK.edge(ctx, dm, x, y, -1.0)
+    with pytest.raises(_lib.PhyError):
+        LK.edge(ctx, dm, x, y, float("nan"))
+    with pytest.raises(_lib.PhyError):
+        LK.adjust_3(ctx, dm, None, x, None, None, y, t_lo=1.0, t_hi=0.5)
+    # median_3 needs an output or an lnL
+    rc = _lib.lib.phy_lk_median3(ctx.handle, dm.handle, 0, P, x.partial.data_ptr(),
+                                 x.exps.data_ptr(), 0.1, y.partial.data_ptr(), y.exps.data_ptr(),
+                                 0.1, y.partial.data_ptr(), y.exps.data_ptr(), 0.1, None, None,
+                                 x.pitch, None, None)
+    assert rc == 1
+    # codes that n does not hold leave it unchanged
+    out, ch2 = LK.adjust_3(ctx, dm, [10_000], x, None, None, y)
+    assert out is x and ch2 == set()
