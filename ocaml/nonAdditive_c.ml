@@ -41,7 +41,8 @@ external adjust_3_ : t -> t -> t -> t -> dbuf * bool = "nonadditive_CAML_adjust3
 external select_ : t -> int array -> dbuf * dbuf = "nonadditive_CAML_select"
 external topo_create : int -> int array -> int -> topo = "phylo_hip_CAML_topo_create"
 external tree_rows_ : int array array -> int -> dbuf * int = "nonadditive_CAML_tree_rows"
-external downpass_ : topo -> dbuf -> int -> dbuf -> int -> float = "nonadditive_CAML_downpass"
+external downpass_ : topo -> dbuf -> int -> dbuf -> int -> float * int array
+  = "nonadditive_CAML_downpass"
 external uppass_ : topo -> dbuf -> dbuf -> int -> int -> unit = "nonadditive_CAML_uppass"
 external row_ : dbuf -> int -> int -> int -> int array = "nonadditive_CAML_row"
 
@@ -129,8 +130,8 @@ let to_string t =
 (** Node.Make1D.median_2 at every node of a tree in one launch: [leaves] are
     the N leaves' state sets (one int per site), [children] the rooted
     layout (internal node N + k has children.(2k), children.(2k+1)).
-    Returns the weighted total cost and the final sets of every node
-    (row v of the returned arrays is node v). *)
+    Returns the weighted total cost, the number of union events per site,
+    and the preliminary and final sets of every node (row v is node v). *)
 let diagnose ~children ~root (leaves : int array array) (weights : float array) =
   let n = Array.length leaves in
   let s = if n = 0 then 0 else Array.length leaves.(0) in
@@ -138,8 +139,8 @@ let diagnose ~children ~root (leaves : int array array) (weights : float array) 
   let rows, stride = tree_rows_ leaves (2 * n - 1) in
   let final, _ = tree_rows_ leaves (2 * n - 1) in
   let w = weights_ weights in
-  let total = downpass_ tp rows stride w s in
+  let total, site_costs = downpass_ tp rows stride w s in
   uppass_ tp rows final stride s;
   let prelim = Array.init (2 * n - 1) (fun v -> row_ rows stride v s) in
   let fin = Array.init (2 * n - 1) (fun v -> row_ final stride v s) in
-  total, prelim, fin
+  total, site_costs, prelim, fin

@@ -399,16 +399,37 @@ value nonadditive_CAML_tree_rows(value leaves, value n_nodes) {
 }
 
 /* Node.Make1D.median_2 over the whole post-order in one launch: fills the
- * internal rows, returns the weighted total cost (lib/node.ml:183-198,272) */
+ * internal rows; returns the weighted total cost (lib/node.ml:183-198,272)
+ * and the number of union events at each site */
 value nonadditive_CAML_downpass(value topo, value rows, value stride, value weights,
                                 value n_sites) {
   CAMLparam5(topo, rows, stride, weights, n_sites);
-  CAMLlocal1(tot);
+  CAMLlocal3(tot, sc, arr);
+  const int64_t S = Long_val(n_sites);
+  uint32_t* h = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(S > 0 ? S : 1));
+  int64_t s;
+  int rc;
+  double total;
+  if (!h) caml_failwith("downpass: out of memory");
   tot = na_cost_buf();
-  check(phy_fitch_downpass(ctx, Topo_val(topo), (uint64_t*)Dptr(rows), Long_val(stride),
-                           Long_val(n_sites), (const uint32_t*)Dptr(weights), NULL, NULL,
-                           (uint64_t*)Dptr(tot)));
-  CAMLreturn(caml_copy_double(read_cost((uint64_t*)Dptr(tot))));
+  sc = dbuf_alloc(sizeof(uint32_t) * (size_t)S);
+  rc = phy_fitch_downpass(ctx, Topo_val(topo), (uint64_t*)Dptr(rows), Long_val(stride), S,
+                          (const uint32_t*)Dptr(weights), (uint32_t*)Dptr(sc), NULL,
+                          (uint64_t*)Dptr(tot));
+  if (rc == PHY_OK) {
+    caml_enter_blocking_section();
+    rc = phy_copy_d2h(ctx, h, Dptr(sc), sizeof(uint32_t) * (size_t)S);
+    caml_leave_blocking_section();
+  }
+  if (rc != PHY_OK) {
+    free(h);
+    check(rc);
+  }
+  total = read_cost((uint64_t*)Dptr(tot));
+  arr = caml_alloc((mlsize_t)S, 0);
+  for (s = 0; s < S; ++s) Store_field(arr, s, Val_long(h[s]));
+  free(h);
+  CAMLreturn(pair(caml_copy_double(total), arr));
 }
 
 /* final sets of every node in one pre-order launch */

@@ -544,7 +544,7 @@ double orc_lnl(int32_t N, const int32_t* ch, int32_t root, const double* blen, i
         neu_add(&my1[c], wp * g);
         neu_add(&my2[c], wp * h);
         neu_add(&my3[c], wp * fabs(g));
-        neu_add(&my4[c], wp * fabs(h));
+        neu_add(&my4[c], wp * (fabs(l2 / l0) + g * g));
       }
     }
     free(L);

@@ -72,8 +72,9 @@ void orc_compose(int32_t n, const double* Q, double t, double* P);
  * of lnL in the branch above each non-root node (root branches: the shared
  * unrooted edge's value in both), d1 = sum_p w_p g_p, d2 = sum_p w_p h_p
  * with g = L'/L, h = L''/L - g^2.  d1abs/d2abs (nullable, with d1/d2) [2N-1]:
- * sum_p w_p |g_p| and sum_p w_p |h_p|, the scale a rounding error of the
- * per-pattern terms is measured against.  Returns lnL. */
+ * sum_p w_p |g_p| and sum_p w_p (|L''/L| + g^2), the magnitudes of the
+ * per-pattern terms (for h, of the two parts of the difference): the scale a
+ * rounding error of those terms is measured against.  Returns lnL. */
 double orc_lnl(int32_t N, const int32_t* children, int32_t root, const double* blen, int32_t K,
                const double* pi, const double* Q, int32_t R, const double* rates,
                const double* probs, const uint32_t* tips, int64_t P, const uint32_t* weights,

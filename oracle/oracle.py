@@ -220,7 +220,8 @@ def lnl(n_leaves, children, root, blen, pi, Q, rates, probs, tips, weights=None,
         derivs=False, threads=1, scales=False):
     """tips: [N, P] uint32 state bitmasks.  Returns (lnl, site_lnl[, d1, d2
     [, d1abs, d2abs]]): with ``scales``, also sum_p w_p |g_p| and
-    sum_p w_p |h_p| per branch (g = L'/L, h = L''/L - g^2)."""
+    sum_p w_p (|L''/L| + g^2) per branch (g = L'/L, h = L''/L - g^2: the
+    magnitudes of the terms, for h of its two parts)."""
     N = int(n_leaves)
     ch = _c(children, np.int32)
     bl = _c(blen, np.float64)

@@ -606,17 +606,25 @@ __device__ __forceinline__ double row_sum_stride(double v) {
 // per-pattern g = L'/L and h = L''/L - g^2, summed per branch over the wave.
 //
 // Everything runs in the eigenbasis of Q = U diag(lam) Ui: with
-// e_k = exp(lam_k rate_r t_c), u = Ui D_c, w = e (*) u and g = U^T A_c,
-//   P_c D_c = U w,  A_c . (P^(n) D_c) = sum_k g_k w_k (lam_k rate_r)^n,
-//   M_c = Ui^T (e (*) g),
+// e_k = exp(lam_k rate_r t_c) = 1 + m_k, u = Ui D_c, w = e (*) u and
+// g = U^T A_c,
+//   P_c D_c = U w = D_c + U (m (*) u),
+//   A_c . (P^(n) D_c) = sum_k g_k w_k (lam_k rate_r)^n,
+//   M_c = Ui^T (e (*) g) = A_c + Ui^T (m (*) g)
+// (U Ui = I; the forms with m keep full relative precision on short
+// branches),
 // so a lane needs only the 4 factors e of each child branch per node; U and
 // Ui are wave-uniform.  Threads are (pattern, rate) quads as in the node
 // kernels: lane p*R + r, rate sums by DPP, one exponent per pattern.
 // ---------------------------------------------------------------------------
 
-// exp(lam_k rate_r t_b) of nb branches (1 where rate_r t_b < 1e-10, the
+// exp(lam_k rate_r t_b) - 1 of nb branches (0 where rate_r t_b < 1e-10, the
 // identity rule of lib/mlmodel.c:332-341), plus real copies of U, Ui and
-// lam_k rate_r: tab = [U 16 | Ui 16 | lr R*4]
+// lam_k rate_r: tab = [U 16 | Ui 16 | lr R*4].  The factors are kept as
+// e - 1 (expm1, full relative precision) so that P D = D + U ((e - 1) (*)
+// Ui D) keeps the relative precision of P's off-diagonal entries (~ Q t) on
+// short branches, which U (e (*) Ui D) with e ~ 1 would lose to
+// cancellation (a relative error of eps / (|lambda| t)).
 template <typename real>
 __global__ void k_eigen_exp(const double* __restrict__ cst, int R, const double* __restrict__ d_t,
                             int nb, real* __restrict__ E, real* __restrict__ tab) {
@@ -633,7 +641,7 @@ __global__ void k_eigen_exp(const double* __restrict__ cst, int R, const double*
   if (idx >= (int64_t)nb * R * K) return;
   const int b = (int)(idx / (R * K)), r = (int)(idx / K) % R, k = (int)(idx % K);
   const double rt = cst[cst_rates(K) + r] * d_t[b];
-  E[idx] = (real)((rt >= 1e-10) ? exp(lam[k] * rt) : 1.0);
+  E[idx] = (real)((rt >= 1e-10) ? expm1(lam[k] * rt) : 0.0);
 }
 
 template <typename real>
@@ -748,16 +756,27 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
     const real* Db = Dc + 4;
     const real* ea = Ec;
     const real* eb = Ec + 4;
+    // ea / eb hold e - 1 (see k_eigen_exp)
     real ua[4], ub[4], wa[4], wb[4], ya[4], yb[4], Aa[4], Ab[4], ga[4], gb[4];
     matvec_u(Ui, Da, ua);
     matvec_u(Ui, Db, ub);
+    {
+      real ma[4], mb[4], ta[4], tb[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      wa[k] = ea[k] * ua[k];
-      wb[k] = eb[k] * ub[k];
+      for (int k = 0; k < 4; ++k) {
+        ma[k] = ea[k] * ua[k];
+        mb[k] = eb[k] * ub[k];
+        wa[k] = ua[k] + ma[k];
+        wb[k] = ub[k] + mb[k];
+      }
+      matvec_u(U, ma, ta);
+      matvec_u(U, mb, tb);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        ya[k] = Da[k] + ta[k];
+        yb[k] = Db[k] + tb[k];
+      }
     }
-    matvec_u(U, wa, ya);
-    matvec_u(U, wb, yb);
     real s0 = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -828,6 +847,8 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
 #pragma unroll
       for (int k = 0; k < 4; ++k) t[k] = ea[k] * ga[k];
       vecmat_u(Ui, t, Ma);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) Ma[k] += Aa[k];
       rescale4(Ma, &eMa);
     }
     if (!(fl & UP_B_LEAF)) {
@@ -835,6 +856,8 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
 #pragma unroll
       for (int k = 0; k < 4; ++k) t[k] = eb[k] * gb[k];
       vecmat_u(Ui, t, Mb);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) Mb[k] += Ab[k];
       rescale4(Mb, &eMb);
     }
     if (fl & UP_PUSH) {

@@ -171,7 +171,8 @@ static void fitch(int32_t N, int64_t S, uint64_t seed) {
   }
   tot = nonadditive_CAML_downpass(topo, rows, Field(rows_pair, 1), wv, Val_long(S));
   nonadditive_CAML_uppass(topo, rows, final, Field(rows_pair, 1), Val_long(S));
-  printf("fitch_tree_total %.1f\n", Double_val(tot));
+  printf("fitch_tree_total %.1f\n", Double_val(Field(tot, 0)));
+  printf("fitch_site_cost_hash %llu\n", (unsigned long long)fnv(Field(tot, 1)));
   {
     uint64_t hp = 0, hf = 0;
     for (i = N; i < 2 * N - 1; ++i) {

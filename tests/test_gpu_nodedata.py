@@ -103,10 +103,10 @@ def test_likelihood_values_filter_patterns(ctx, rng):
         root = LK.median_2(ctx, dm, None, fx, fy)
         parts.append(LK.root_lnl(ctx, dm, root))
     top = LK.median_2(ctx, dm, None, x, y)
-    with pytest.raises(ValueError):
-        LK.cost(top)  # not evaluated yet
+    assert top.lnl is None
+    c = LK.cost(top)  # evaluated on first use, through the model that made top
     full = LK.root_lnl(ctx, dm, top)
-    assert LK.cost(top) == -full == LK.root_cost(ctx, dm, top)
+    assert c == -full == LK.cost(top) == LK.root_cost(ctx, dm, top)
     assert abs(parts[0] + parts[1] - full) <= 1e-11 * abs(full)
     # the same as the oracle's two-taxon tree on the kept patterns
     sel = sorted(keep)

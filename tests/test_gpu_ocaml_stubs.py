@@ -55,10 +55,11 @@ def test_fitch_stubs_match_the_oracle(out):
     tree = synth.yule(NF, SEED, 0.1)
     leaves = D.unpack_nibbles(synth.fitch_leaves(NF, SF, SEED), SF)
     w = (1 + np.arange(SF) % 3).astype(np.uint32)
-    sets, _, node_cost, total = O.fitch_downpass(NF, tree.children, tree.root, leaves, w)
+    sets, site_cost, node_cost, total = O.fitch_downpass(NF, tree.children, tree.root, leaves, w)
     fin = O.fitch_uppass(NF, tree.children, tree.root, sets)
     assert float(out["fitch_nodes_total"][0]) == total
     assert float(out["fitch_tree_total"][0]) == total
+    assert int(out["fitch_site_cost_hash"][0]) == _fnv(site_cost)
     assert int(out["fitch_root_hash"][0]) == _fnv(sets[tree.root])
     assert float(out["fitch_prev_cost"][0]) == node_cost[tree.root - NF]
     hp = hf = 0
