@@ -334,6 +334,23 @@ int phy_lk_tree(phy_ctx* ctx, const phy_topo* topo, const phy_model* model,
                 double* d_lnl, double* d_site_lnl, double* d_d1, double* d_d2,
                 void* d_work, size_t work_bytes);
 
+/* No counterpart in the reference.  The whole-tree lnL of phy_lk_tree in
+ * its fused form (form = PHY_FUSED, optionally | PHY_F32; no derivatives)
+ * with no workspace argument and no allocation: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The workspace (P matrices of
+ * every branch and rate class, one partial sum per 64 patterns) lives in a
+ * fixed 8 MiB device array per GPU, resolved when a context is created and
+ * held until the process ends; a tree and pattern count whose workspace
+ * (phy_lk_tree_work_size) exceeds it, or whose stack does not fit the fused
+ * form, gives PHY_ERANGE.  Results are those of phy_lk_tree with the same
+ * arguments.  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (other streams, other contexts) race. */
+int phy_lk_tree_noalloc(phy_ctx* ctx, const phy_topo* topo, const phy_model* model,
+                        int32_t form, int64_t n_patterns, const uint64_t* d_tips,
+                        int64_t stride, const uint32_t* d_weights, const double* d_blen,
+                        double* d_lnl, double* d_site_lnl);
+
 /* Pointers into d_work of the stored partials and exponents of internal node
  * v (N <= v <= 2N-2) after phy_lk_tree without PHY_FUSED: the partial is
  * R*K rows of pitch values (FP32 or FP64 by form), row r*K + i holding

@@ -13,7 +13,14 @@ struct phy_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  // this device's copy of the fixed scratch array of phy_lk_tree_noalloc
+  // (resolved at creation; null if that lookup failed)
+  void* noalloc_scratch = nullptr;
 };
+
+// Resolve (once per device, with that device current) the device address of
+// the fixed scratch array behind phy_lk_tree_noalloc; *out receives it.
+int phy_noalloc_scratch_init(int device, void** out);
 
 // Operand sources of a program entry (SRC_EXT: pre-order entry of a part,
 // whose parent's final set is read from memory).

@@ -49,6 +49,9 @@ int phy_ctx_create(int device, void* stream, phy_ctx** out) {
   phy_ctx* c = new phy_ctx();
   c->device = device;
   c->stream = (hipStream_t)stream;  // null = the default stream
+  // a failed lookup leaves the allocation-free entry point unusable (it then
+  // reports PHY_EDEVICE) and every other entry point as it is
+  if (phy_noalloc_scratch_init(device, &c->noalloc_scratch) != PHY_OK) c->noalloc_scratch = nullptr;
   *out = c;
   return PHY_OK;
 }

@@ -23,12 +23,43 @@
 #include <climits>
 #include <atomic>
 #include <cmath>
+#include <mutex>
 #include <thread>
 #include <type_traits>
 
 #include "common.hpp"
 #include "stream.hpp"
 #include "model_host.hpp"
+
+// Workspace of phy_lk_tree_noalloc (per-branch P matrices and per-64-pattern
+// partial sums of the fused lnL pass): a fixed device array, one copy per
+// GPU, held until the process ends.  Host code never passes `scratch` itself
+// (in host code the name is a stand-in, not a device address): it uses the
+// address phy_noalloc_scratch_init resolves for each device.  Calls on one
+// GPU share the array, so calls that overlap on one GPU race.
+constexpr size_t kNoAllocScratchBytes = (size_t)8 << 20;  // 8 MiB
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+int phy_noalloc_scratch_init(int device, void** out) {
+  static std::mutex mu;
+  static void* addr[64] = {};
+  static bool done[64] = {};
+  if (!out || device < 0 || device >= 64) return PHY_EINVAL;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!done[device]) {
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != device) return PHY_EDEVICE;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return PHY_EDEVICE;
+    }
+    addr[device] = p;
+    done[device] = true;
+  }
+  *out = addr[device];
+  return PHY_OK;
+}
 
 struct phy_model {
   phy_ctx* ctx = nullptr;
@@ -1481,6 +1512,35 @@ int phy_lk_tree_partials(const phy_topo* t, const phy_model* m, int64_t n_pat, i
   if (d_exp) *d_exp = reinterpret_cast<int32_t*>(w + pl.off_ex) + (size_t)(v - N) * m->R * pl.pitch;
   if (pitch) *pitch = pl.pitch;
   return PHY_OK;
+}
+
+int phy_lk_tree_noalloc(phy_ctx* ctx, const phy_topo* t, const phy_model* m, int32_t form,
+                        int64_t n_pat, const uint64_t* tips, int64_t stride,
+                        const uint32_t* weights, const double* blen, double* lnl,
+                        double* site_lnl) {
+  if (!ctx) return PHY_EINVAL;
+  if (!t || !t->d_down_prog || !m) return set_err(ctx, PHY_EINVAL, "lk_tree_noalloc: topology/model missing");
+  if (m->K != 4) return set_err(ctx, PHY_EINVAL, "lk_tree_noalloc: this entry point is for 4-state models");
+  if (form & ~(PHY_F32 | PHY_FUSED)) return set_err(ctx, PHY_EINVAL, "lk_tree_noalloc: unknown form bits");
+  if (!(form & PHY_FUSED)) return set_err(ctx, PHY_EINVAL, "lk_tree_noalloc: the fused form only (PHY_FUSED)");
+  if (n_pat < 1) return set_err(ctx, PHY_EINVAL, "lk_tree_noalloc: need at least one pattern");
+  if (stride < phy_words(n_pat)) return set_err(ctx, PHY_EINVAL, "lk_tree_noalloc: tip stride < words");
+  if (!tips || !weights || !blen || !lnl) return set_err(ctx, PHY_EINVAL, "lk_tree_noalloc: null argument");
+  const Plan pl = make_plan(t, m, n_pat, form, 0);
+  if (!pl.fused)
+    return set_err(ctx, PHY_ERANGE, "lk_tree_noalloc: the tree's stack does not fit the fused form");
+  if (pl.total > kNoAllocScratchBytes)
+    return set_err(ctx, PHY_ERANGE, "lk_tree_noalloc: workspace of " + std::to_string(pl.total) +
+                                        " bytes exceeds the fixed scratch of " +
+                                        std::to_string(kNoAllocScratchBytes));
+  if (!ctx->noalloc_scratch)
+    return set_err(ctx, PHY_EDEVICE, "lk_tree_noalloc: the scratch array of this device is unresolved");
+  char* w = static_cast<char*>(ctx->noalloc_scratch);
+  if (form & PHY_F32)
+    return dispatch_tree<float>(ctx, t, m, true, false, n_pat, tips, stride, weights, blen, lnl,
+                                site_lnl, nullptr, nullptr, w, pl);
+  return dispatch_tree<double>(ctx, t, m, true, false, n_pat, tips, stride, weights, blen, lnl,
+                               site_lnl, nullptr, nullptr, w, pl);
 }
 
 int phy_lk_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, int32_t form, int64_t n_pat,
