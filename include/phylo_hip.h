@@ -89,6 +89,23 @@ int phy_ctx_sync(phy_ctx* ctx);
 int phy_ctx_device(const phy_ctx* ctx);
 void* phy_ctx_stream(const phy_ctx* ctx);
 
+/* Measurement hooks (no counterpart in the reference).  With the timer on,
+ * each entry point records a HIP event pair on the context's stream around
+ * its dominant kernel launches, labelled "prune_down", "prune_deriv",
+ * "prune_down_batch", "fitch_down", "fitch_up", "aa_prune", "sankoff_down",
+ * "mlp_forward", "mlp_backward" or "copy_stream".  phy_ctx_kernel_timer
+ * switches it on or off and clears what was recorded;
+ * phy_ctx_kernel_time waits for the recorded spans and returns the total
+ * milliseconds and the number of spans of one label since the timer was
+ * switched on. */
+int phy_ctx_kernel_timer(phy_ctx* ctx, int32_t enable);
+int phy_ctx_kernel_time(phy_ctx* ctx, const char* label, double* total_ms, int64_t* count);
+
+/* STREAM-style device copy (16-byte non-temporal loads and stores; both
+ * pointers 16-byte aligned): the bandwidth reference the HBM-bound kernels
+ * are compared with.  No counterpart in the reference. */
+int phy_copy_stream(phy_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
+
 /* Device memory for callers without their own allocator (the OCaml custom
  * blocks; lib/bitvector/bv.c:163-189 mallocs/frees host vectors the same way). */
 int phy_dev_alloc(phy_ctx* ctx, size_t bytes, void** d_out);

@@ -82,6 +82,9 @@ SIGNATURES = {
     "phy_lk_tree": (C.c_int, [vp, vp, vp, i32, i64, vp, i64, vp, vp, vp, vp, vp, vp, vp, sz]),
     "phy_lk_tree_partials": (C.c_int, [vp, vp, i64, i32, i32, vp, i32, PP, PP, P_i64]),
     "phy_lk_tree_noalloc": (C.c_int, [vp, vp, vp, i32, i64, vp, i64, vp, vp, vp, vp]),
+    "phy_ctx_kernel_timer": (C.c_int, [vp, i32]),
+    "phy_ctx_kernel_time": (C.c_int, [vp, C.c_char_p, P_f64, P_i64]),
+    "phy_copy_stream": (C.c_int, [vp, vp, vp, sz]),
     "phy_lk_leaf": (C.c_int, [vp, vp, i32, i64, vp, vp, vp, i64]),
     "phy_lk_median2": (C.c_int, [vp, vp, i32, i64, vp, vp, f64, vp, vp, f64, vp, vp, i64]),
     "phy_lk_root": (C.c_int, [vp, vp, i32, i64, vp, vp, i64, vp, vp]),

@@ -183,24 +183,37 @@ def autotune(ctx: D.Context, trees: list[AATree], sample: int = 48, reps: int = 
 
 
 class TunedBatch:
-    """The trees split by size bucket, one persistent launch per bucket with
-    that bucket's autotuned waves; lnl in the original tree order."""
+    """The whole batch in one persistent launch when every size bucket's
+    autotuned waves agree; otherwise the trees split by bucket, one
+    persistent launch per bucket with its own waves.  lnl in the original
+    tree order."""
 
     def __init__(self, ctx: D.Context, trees: list[AATree], choice: dict):
         torch = __import__("torch")
         self.ctx = ctx
         self.parts = []
-        for k in range(len(BUCKETS)):
-            idx = [i for i, t in enumerate(trees) if bucket_of(t.n_leaves) == k]
-            if not idx:
-                continue
-            w = choice.get(k, {"waves": 4})["waves"]
-            self.parts.append((np.array(idx), AABatch(ctx, [trees[i] for i in idx], waves=w)))
+        waves = {k: choice.get(k, {"waves": 4})["waves"] for k in range(len(BUCKETS))}
+        present = {bucket_of(t.n_leaves) for t in trees}
+        if len({waves[k] for k in present}) == 1:
+            w = waves[next(iter(present))]
+            self.parts.append((np.arange(len(trees)), AABatch(ctx, trees, waves=w)))
+        else:
+            for k in range(len(BUCKETS)):
+                idx = [i for i, t in enumerate(trees) if bucket_of(t.n_leaves) == k]
+                if not idx:
+                    continue
+                self.parts.append((np.array(idx),
+                                   AABatch(ctx, [trees[i] for i in idx], waves=waves[k])))
+        self.launches = len(self.parts)
         self.lnl = torch.zeros((len(trees),), dtype=torch.float64, device=f"cuda:{ctx.device}")
         self.pattern_nodes = sum(b.pattern_nodes for _, b in self.parts)
         self._idx = [torch.from_numpy(i).to(self.lnl.device) for i, _ in self.parts]
 
     def run(self) -> None:
+        if self.launches == 1:
+            self.parts[0][1].run()
+            self.lnl.copy_(self.parts[0][1].lnl)
+            return
         for (idx, b), di in zip(self.parts, self._idx):
             b.run()
             self.lnl.index_copy_(0, di, b.lnl)

@@ -513,6 +513,7 @@ int launch(phy_ctx* ctx, phy_aa_batch* b) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_aa_prune<WAVES, R, LV>, 64 * WAVES, lds) != hipSuccess || per_cu < 1)
     per_cu = 1;
   const int64_t grid = std::min<int64_t>((int64_t)n_cu * per_cu, b->n_items);
+  KernelSpan span(ctx, "aa_prune");
   hipLaunchKernelGGL((k_aa_prune<WAVES, R, LV>), dim3((unsigned)std::max<int64_t>(grid, 1)), dim3(64 * WAVES), lds,
                      ctx->stream, a);
   PHY_LAUNCHED(ctx);

@@ -13,9 +13,47 @@ struct phy_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  // per-kernel timing (phy_ctx_kernel_timer): HIP event pairs recorded on the
+  // stream around the dominant launches of each entry point, resolved into
+  // per-label totals by phy_ctx_kernel_time
+  bool timing = false;
+  struct Span {
+    const char* label;
+    hipEvent_t start, stop;
+  };
+  std::vector<Span> spans;
+  std::vector<std::pair<std::string, std::pair<double, int64_t>>> totals;
   // this device's copy of the fixed scratch array of phy_lk_tree_noalloc
   // (resolved at creation; null if that lookup failed)
   void* noalloc_scratch = nullptr;
+};
+
+// Records a HIP event pair around the launches in its scope when the context's
+// timing is on (one span per scope, labelled by the kernel it covers).
+struct KernelSpan {
+  phy_ctx* c;
+  const char* label;
+  hipEvent_t a = nullptr, b = nullptr;
+  KernelSpan(phy_ctx* ctx, const char* lbl) : c(ctx), label(lbl) {
+    if (!c || !c->timing) return;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess ||
+        hipEventRecord(a, c->stream) != hipSuccess) {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+      a = b = nullptr;
+      (void)hipGetLastError();
+    }
+  }
+  ~KernelSpan() {
+    if (!a) return;
+    if (hipEventRecord(b, c->stream) == hipSuccess) {
+      c->spans.push_back({label, a, b});
+    } else {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+      (void)hipGetLastError();
+    }
+  }
 };
 
 // Resolve (once per device, with that device current) the device address of

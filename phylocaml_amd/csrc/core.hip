@@ -56,7 +56,90 @@ int phy_ctx_create(int device, void* stream, phy_ctx** out) {
   return PHY_OK;
 }
 
-void phy_ctx_destroy(phy_ctx* ctx) { delete ctx; }
+static void drop_spans(phy_ctx* ctx) {
+  for (auto& s : ctx->spans) {
+    (void)hipEventDestroy(s.start);
+    (void)hipEventDestroy(s.stop);
+  }
+  ctx->spans.clear();
+}
+
+void phy_ctx_destroy(phy_ctx* ctx) {
+  if (ctx) drop_spans(ctx);
+  delete ctx;
+}
+
+int phy_ctx_kernel_timer(phy_ctx* ctx, int32_t enable) {
+  if (!ctx) return PHY_EINVAL;
+  drop_spans(ctx);
+  ctx->totals.clear();
+  ctx->timing = enable != 0;
+  return PHY_OK;
+}
+
+int phy_ctx_kernel_time(phy_ctx* ctx, const char* label, double* total_ms, int64_t* count) {
+  if (!ctx || !label || !total_ms || !count) return PHY_EINVAL;
+  for (auto& s : ctx->spans) {
+    float ms = 0.f;
+    PHY_HIP(ctx, hipEventSynchronize(s.stop));
+    PHY_HIP(ctx, hipEventElapsedTime(&ms, s.start, s.stop));
+    auto it = std::find_if(ctx->totals.begin(), ctx->totals.end(),
+                           [&](const auto& t) { return t.first == s.label; });
+    if (it == ctx->totals.end()) {
+      ctx->totals.push_back({s.label, {0.0, 0}});
+      it = ctx->totals.end() - 1;
+    }
+    it->second.first += ms;
+    it->second.second += 1;
+  }
+  drop_spans(ctx);
+  *total_ms = 0.0;
+  *count = 0;
+  for (const auto& t : ctx->totals)
+    if (t.first == label) {
+      *total_ms = t.second.first;
+      *count = t.second.second;
+    }
+  return PHY_OK;
+}
+
+// STREAM-style copy: each lane moves 16-byte vectors with non-temporal loads
+// and stores (grid-stride), the tail byte by byte.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ void k_copy_stream(const u32x4* __restrict__ src, u32x4* __restrict__ dst, int64_t n16,
+                              const unsigned char* __restrict__ src_tail,
+                              unsigned char* __restrict__ dst_tail, int64_t n_tail) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) {
+    const u32x4 v = __builtin_nontemporal_load(src + i);
+    __builtin_nontemporal_store(v, dst + i);
+  }
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n_tail) dst_tail[t] = src_tail[t];
+}
+
+int phy_copy_stream(phy_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
+  if (!ctx || (bytes && (!d_dst || !d_src))) return PHY_EINVAL;
+  if (((uintptr_t)d_dst | (uintptr_t)d_src) & 15)
+    return set_err(ctx, PHY_EINVAL, "copy_stream: pointers must be 16-byte aligned");
+  if (bytes == 0) return PHY_OK;
+  const int64_t n16 = (int64_t)(bytes / 16), n_tail = (int64_t)(bytes % 16);
+  int n_cu = 256;
+  (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+  const int bs = 256;
+  // workgroups of 4 waves per CU (PHY_COPY_WG_PER_CU, default 8)
+  int per_cu = 8;
+  if (const char* e = getenv("PHY_COPY_WG_PER_CU")) per_cu = std::max(1, std::min(64, atoi(e)));
+  const int64_t grid =
+      std::max<int64_t>(1, std::min<int64_t>((n16 + bs - 1) / bs, (int64_t)n_cu * per_cu));
+  KernelSpan span(ctx, "copy_stream");
+  hipLaunchKernelGGL(k_copy_stream, dim3((unsigned)grid), dim3(bs), 0, ctx->stream,
+                     (const u32x4*)d_src, (u32x4*)d_dst, n16,
+                     (const unsigned char*)d_src + n16 * 16, (unsigned char*)d_dst + n16 * 16,
+                     n_tail);
+  PHY_LAUNCHED(ctx);
+  return PHY_OK;
+}
 
 const char* phy_last_error(const phy_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 

@@ -674,6 +674,7 @@ int downpass_launch(phy_ctx* ctx, const phy_topo* topo, uint32_t* s32, int64_t s
   if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "downpass: LDS stack too deep for this tree");
   const unsigned grid = (unsigned)((W32 / V + bs - 1) / bs);
   if (!split) {
+    KernelSpan span(ctx, "fitch_down");
     if (node_cost)
       hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_WHOLE, V, NT>), dim3(grid), dim3(bs), lds, ctx->stream,
                          topo->d_down_prog, N - 1, topo->d_down_fifo, nullptr, 0, nullptr, 0, s32, stride32, W32,
@@ -704,6 +705,7 @@ int downpass_launch(phy_ctx* ctx, const phy_topo* topo, uint32_t* s32, int64_t s
   // PHY_FITCH_NARROW=0 keeps 32-bit counts
   const char* ne = getenv("PHY_FITCH_NARROW");
   const int narrow = max_ops <= 255 && !(ne && ne[0] == '0');
+  KernelSpan span(ctx, "fitch_down");
   if (node_cost) {
     hipLaunchKernelGGL((k_fitch_down<BATCH, true, MODE_PART, V, NT>), dim3(grid, (unsigned)P), dim3(bs), lds,
                        ctx->stream, sp.d_prog, 0, sp.d_fifo, sp.d_meta, P, scr, narrow, s32, stride32, W32, n_sites,
@@ -736,6 +738,7 @@ int uppass_launch(phy_ctx* ctx, const phy_topo* topo, const uint32_t* s32, uint3
   const size_t lds = lds_of(bs);
   if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "uppass: LDS stack too deep for this tree");
   const unsigned grid = (unsigned)((W32 / V + bs - 1) / bs);
+  KernelSpan span(ctx, "fitch_up");
   if (!split) {
     hipLaunchKernelGGL((k_fitch_up<BATCH, MODE_WHOLE, V, NT>), dim3(grid), dim3(bs), lds, ctx->stream,
                        topo->d_up_prog, N - 1, topo->d_up_fifo, nullptr, 0, nullptr, s32, f32, stride32,

@@ -178,6 +178,7 @@ int forward(phy_ctx* ctx, phy_mlp* m, const float* X, int64_t B, float* y) {
   float* Z2 = H1 + m->cap * H;
   float* H2 = Z2 + m->cap * H;
   const int Mi = (int)B;
+  KernelSpan span(ctx, "mlp_forward");
   hipLaunchKernelGGL(k_gemm, tiles(Mi, H), dim3(64), 0, ctx->stream, X, (int64_t)n_in, (int64_t)1, p.W1,
                      (int64_t)H, (int64_t)1, Z1, (int64_t)H, Mi, H, n_in, 0, p.b1);
   const int64_t nh = B * H;
@@ -205,6 +206,7 @@ int backward(phy_ctx* ctx, phy_mlp* m, const float* X, const float* t, int64_t B
   const int Mi = (int)B;
   int rc = forward(ctx, m, X, B, y);
   if (rc) return rc;
+  KernelSpan span(ctx, "mlp_backward");
   hipLaunchKernelGGL(k_loss, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, ctx->stream, y, t, dy, loss, B);
   // output layer: dw3 = H2^T dy (GEMM with N = 1), db3 = sum dy
   hipLaunchKernelGGL(k_gemm, tiles(H, 1), dim3(64), 0, ctx->stream, H2, (int64_t)1, (int64_t)H, dy,

@@ -1366,20 +1366,23 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
                        d_probs, part, ex, weights, site_lnl, bsum, N, t->down_depth);
   };
   constexpr int G2 = (R % 2 == 0 && sizeof(real) == 4) ? 2 : 1;
-  if (!fused) {
-    pl.rpw == 2 ? down(k_prune_down<real, R, G2, false, 0>, pl.lds)
-                : down(k_prune_down<real, R, 1, false, 0>, pl.lds);
-  } else if (pl.rpw == 2) {
-    down(k_prune_down<real, R, G2, true, 0>, pl.lds);
-  } else {
-    const int lv = register_levels<real, R>(t->down_depth, pl.bs);
-    const size_t lds = down_lds(R, sizeof(real), true, t->down_depth, lv);
-    if (lv == 2)
-      down(k_prune_down<real, R, 1, true, 2>, lds);
-    else if (lv == 1)
-      down(k_prune_down<real, R, 1, true, 1>, lds);
-    else
-      down(k_prune_down<real, R, 1, true, 0>, lds);
+  const int lv = (fused && pl.rpw != 2) ? register_levels<real, R>(t->down_depth, pl.bs) : 0;
+  {
+    KernelSpan span(ctx, "prune_down");
+    if (!fused) {
+      pl.rpw == 2 ? down(k_prune_down<real, R, G2, false, 0>, pl.lds)
+                  : down(k_prune_down<real, R, 1, false, 0>, pl.lds);
+    } else if (pl.rpw == 2) {
+      down(k_prune_down<real, R, G2, true, 0>, pl.lds);
+    } else {
+      const size_t lds = down_lds(R, sizeof(real), true, t->down_depth, lv);
+      if (lv == 2)
+        down(k_prune_down<real, R, 1, true, 2>, lds);
+      else if (lv == 1)
+        down(k_prune_down<real, R, 1, true, 1>, lds);
+      else
+        down(k_prune_down<real, R, 1, true, 0>, lds);
+    }
   }
   PHY_LAUNCHED(ctx);
   hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, ctx->stream, bsum, pl.n_waves, (int64_t)1, lnl);
@@ -1392,10 +1395,13 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
     hipLaunchKernelGGL((k_eigen_exp<real>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0,
                        ctx->stream, cst, R, blen, M, E, tab);
     PHY_LAUNCHED(ctx);
-    hipLaunchKernelGGL((k_prune_deriv<real, R>), dim3((unsigned)pl.dblocks), dim3(pl.dbs), pl.dlds,
-                       ctx->stream, t->d_up_prog, N - 1, tips, stride, n_pat, pl.pitch, tab, E,
-                       d_pi, d_probs, part, ex, weights, wsum, pl.n_dwaves, N,
-                       std::max(t->up_depth, 1));
+    {
+      KernelSpan span(ctx, "prune_deriv");
+      hipLaunchKernelGGL((k_prune_deriv<real, R>), dim3((unsigned)pl.dblocks), dim3(pl.dbs), pl.dlds,
+                         ctx->stream, t->d_up_prog, N - 1, tips, stride, n_pat, pl.pitch, tab, E,
+                         d_pi, d_probs, part, ex, weights, wsum, pl.n_dwaves, N,
+                         std::max(t->up_depth, 1));
+    }
     PHY_LAUNCHED(ctx);
     const int ra = t->children[2 * (t->root - N)], rb = t->children[2 * (t->root - N) + 1];
     hipLaunchKernelGGL(k_reduce_derivs, dim3((unsigned)M), dim3(256), 0, ctx->stream, wsum,
@@ -1628,10 +1634,13 @@ int spr_launch(phy_ctx* ctx, const phy_model* m, int N, int64_t n_pat, const uin
                        pl.pitch, P, d_pi, d_pi + K, w, ws, pl.n_waves, depth);
   };
   constexpr int G2 = (R % 2 == 0 && sizeof(real) == 4) ? 2 : 1;
-  if (pl.rpw == 2 && G2 == 2)
-    down(k_prune_down_batch<real, R, G2>);
-  else
-    down(k_prune_down_batch<real, R, 1>);
+  {
+    KernelSpan span(ctx, "prune_down_batch");
+    if (pl.rpw == 2 && G2 == 2)
+      down(k_prune_down_batch<real, R, G2>);
+    else
+      down(k_prune_down_batch<real, R, 1>);
+  }
   PHY_LAUNCHED(ctx);
   hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)C), dim3(256), 0, ctx->stream, ws, pl.n_waves, out);
   PHY_LAUNCHED(ctx);

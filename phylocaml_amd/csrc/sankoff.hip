@@ -186,6 +186,7 @@ int sankoff_launch(phy_ctx* ctx, const phy_topo* topo, const int32_t* tcm, const
   const size_t lds = sankoff_lds(K, topo->down_depth, bs);
   if (lds > 64 * 1024) return set_err(ctx, PHY_ERANGE, "sankoff: LDS stack too deep");
   const int64_t nb = (n_sites + bs - 1) / bs;
+  KernelSpan span(ctx, "sankoff_down");
   hipLaunchKernelGGL((k_sankoff_down<K>), dim3((unsigned)nb), dim3(bs), lds, ctx->stream,
                      topo->d_down_prog, topo->n_leaves - 1, topo->d_down_fifo, tcm, tips, stride,
                      n_sites, wt, site_cost, total);

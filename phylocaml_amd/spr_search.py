@@ -199,6 +199,39 @@ def search(c: Context, net, norm, guided: bool, k: int = 64, eps: float = 1e-6,
             "moves_per_round": int(len(spr.moves(inst.N, ch, r)))}
 
 
+def compare(ctx: D.Context, N: int, P: int, pert: int, train_instances: int = 4,
+            top_k: int = 64, fp64: bool = True) -> dict:
+    """Train the MLP on `train_instances` instances' exact move scores, then
+    run the exhaustive and the guided search from the same held-out start
+    (seed 7) to convergence; returns their summaries and ratios."""
+    log(f"training data: {train_instances} instances, N={N}, P={P}")
+    t0 = time.time()
+    X, T = training_pairs(ctx, range(100, 100 + train_instances), N, P, pert)
+    t_data = time.time() - t0
+    t0 = time.time()
+    net, norm, losses = train(ctx, X, T)
+    t_train = time.time() - t0
+    inst = make_instance(7, N, P, pert)
+    c = Context(ctx, inst)
+    log("exhaustive search")
+    ex = search(c, net, norm, guided=False, k=top_k, fp64=fp64)
+    log(json.dumps(ex))
+    log("guided search")
+    gd = search(c, net, norm, guided=True, k=top_k, fp64=fp64)
+    log(json.dumps(gd))
+    out = {
+        "workload": f"{N}-taxon Yule tree, {P} unique GTR+G4 patterns, start = true tree "
+                    f"after {pert} random SPR moves; every rooted SPR move each round",
+        "training_pairs": int(len(X)), "training_data_s": t_data, "training_s": t_train,
+        "train_loss_per_epoch": losses, "top_k": top_k, "exact_scores": "FP64" if fp64 else "FP32",
+        "exhaustive": ex, "guided": gd,
+        "exact_evaluation_ratio": ex["exact_evaluations"] / max(gd["exact_evaluations"], 1),
+        "wall_time_ratio": ex["wall_s"] / max(gd["wall_s"], 1e-9),
+        "final_lnl_gap": gd["final_lnl"] - ex["final_lnl"],
+    }
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true", help="small instances (tests)")
@@ -208,32 +241,8 @@ def main(argv=None):
                     help="score the exhaustive baseline in FP32 (faster)")
     args = ap.parse_args(argv)
     N, P, pert = (64, 5_000, 6) if args.quick else (256, 50_000, 10)
-    ctx = D.Context(0)
-    log(f"training data: {args.train_instances} instances, N={N}, P={P}")
-    t0 = time.time()
-    X, T = training_pairs(ctx, range(100, 100 + args.train_instances), N, P, pert)
-    t_data = time.time() - t0
-    t0 = time.time()
-    net, norm, losses = train(ctx, X, T)
-    t_train = time.time() - t0
-    inst = make_instance(7, N, P, pert)
-    c = Context(ctx, inst)
-    log("exhaustive search")
-    ex = search(c, net, norm, guided=False, k=args.top_k, fp64=not args.fp32_exhaustive)
-    log(json.dumps(ex))
-    log("guided search")
-    gd = search(c, net, norm, guided=True, k=args.top_k, fp64=not args.fp32_exhaustive)
-    log(json.dumps(gd))
-    out = {
-        "workload": f"{N}-taxon Yule tree, {P} unique GTR+G4 patterns, start = true tree "
-                    f"after {pert} random SPR moves; every rooted SPR move each round",
-        "training_pairs": int(len(X)), "training_data_s": t_data, "training_s": t_train,
-        "train_loss_per_epoch": losses, "top_k": args.top_k,
-        "exhaustive": ex, "guided": gd,
-        "exact_evaluation_ratio": ex["exact_evaluations"] / max(gd["exact_evaluations"], 1),
-        "wall_time_ratio": ex["wall_s"] / max(gd["wall_s"], 1e-9),
-        "final_lnl_gap": gd["final_lnl"] - ex["final_lnl"],
-    }
+    out = compare(D.Context(0), N, P, pert, args.train_instances, args.top_k,
+                  fp64=not args.fp32_exhaustive)
     print(json.dumps(out), flush=True)
     return out
 
