@@ -142,7 +142,7 @@ def stream_copy(torch, ctx, clock, gib=2):
     return {"bytes": nbytes, "kernel_ms": k_ms, "read_plus_write_GBs": gbs,
             "frac_of_peak": gbs / HBM_PEAK_GBS,
             "kernel": "k_copy_stream: 16-byte non-temporal loads and stores, grid-stride, "
-                      "8 workgroups of 256 threads per CU"}
+                      "4 workgroups of 256 threads per CU"}
 
 
 def bench_prune(torch, ctx, clock, steps, warm, cpu_baseline):
@@ -530,7 +530,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["all", "prune", "fitch", "aa", "spr"], default="all")
+    ap.add_argument("--only", choices=["all", "prune", "fitch", "aa", "spr", "search", "mlp",
+                                       "copy"], default="all",
+                    help="one part only (the pruning line is always measured unless the part "
+                         "is copy or mlp)")
     ap.add_argument("--aa-trees", type=int, default=2048)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -550,6 +553,11 @@ def main():
     everything = args.only == "all"
 
     copy = stream_copy(torch, ctx, clock)
+    if args.only in ("copy", "mlp"):
+        print(json.dumps({"copy_stream": copy,
+                          "mlp": bench_mlp(torch, ctx, clock) if args.only == "mlp" else None}),
+              flush=True)
+        return
     pr = bench_prune(torch, ctx, clock, steps, warm, cpu_on)
     units = pr["units"]
     fitch = bench_fitch(torch, ctx, clock, steps, warm, copy["read_plus_write_GBs"], cpu_on) \
@@ -557,7 +565,7 @@ def main():
     aa_res = bench_aa(torch, ctx, clock, args.aa_trees, max(2, steps // 5), cpu_on) \
         if args.only in ("all", "aa") else None
     spr_res = bench_spr(torch, ctx, cpu_on) if args.only in ("all", "spr") else None
-    search = bench_search(ctx) if everything else None
+    search = bench_search(ctx) if args.only in ("all", "search") else None
     mlp = bench_mlp(torch, ctx, clock) if everything else None
     config0 = bench_config0(ctx) if everything and cpu_on else None
 

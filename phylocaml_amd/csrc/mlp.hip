@@ -6,6 +6,7 @@
 //
 // Parameters (flat FP32): W1[n_in][H] b1[H] W2[H][H] b2[H] w3[H] b3[1].
 // Loss: 0.5 * mean((y - t)^2).
+#include <algorithm>
 #include <cmath>
 
 #include "common.hpp"
@@ -23,6 +24,9 @@ struct phy_mlp {
   // activations for up to cap rows: Z1 H1 Z2 H2 [cap][H], y [cap], dy, dH [cap][H] x2
   float* act = nullptr;
   int64_t cap = 0;
+  // partial results of the split-K weight-gradient GEMMs and column sums
+  float* ws = nullptr;
+  int64_t ws_floats = 0;
 };
 
 namespace {
@@ -36,21 +40,26 @@ __device__ __forceinline__ float gelu_grad(float x) {
 
 // C[M][N] (+)= op(A)[M][K] op(B)[K][N] (+ bias[n]), A at A[m*sam + k*sak],
 // B at B[k*sbk + n*sbn], C row-major with ldc.  One wave per 16x16 tile;
-// operands are rounded to BF16, accumulation is FP32.
+// operands are rounded to BF16, accumulation is FP32.  Split K: workgroup z
+// takes k in [z * kps, (z + 1) * kps) and writes its partial tile at
+// Cm + z * c_split (the caller sums the splits).
 __global__ __launch_bounds__(64) void k_gemm(const float* __restrict__ A, int64_t sam, int64_t sak,
                                              const float* __restrict__ B, int64_t sbk, int64_t sbn,
                                              float* __restrict__ Cm, int64_t ldc, int M, int N, int K,
-                                             int accumulate, const float* __restrict__ bias) {
+                                             int accumulate, const float* __restrict__ bias,
+                                             int kps = 1 << 30, int64_t c_split = 0) {
   const int l = threadIdx.x, q = l >> 4, r16 = l & 15;
   const int m0 = blockIdx.y * 16, n0 = blockIdx.x * 16;
   const int am = m0 + r16, bn = n0 + r16;
+  const int kb = (int)blockIdx.z * kps, ke = min(K, kb + kps);
+  Cm += (int64_t)blockIdx.z * c_split;
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 32) {
+  for (int k0 = kb; k0 < ke; k0 += 32) {
     bf16x8 a, b;
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
       const int k = k0 + 8 * q + v;
-      const bool ka = k < K;
+      const bool ka = k < ke;
       a[v] = (__bf16)((ka && am < M) ? A[(int64_t)am * sam + (int64_t)k * sak] : 0.0f);
       b[v] = (__bf16)((ka && bn < N) ? B[(int64_t)k * sbk + (int64_t)bn * sbn] : 0.0f);
     }
@@ -113,20 +122,27 @@ __global__ void k_gelu_bwd(const float* __restrict__ dH, const float* __restrict
   dZ[i] = g * gelu_grad(Z[i]);
 }
 
-// column sums: out[j] = sum_m X[m][j] (FP32 partials per block, then atomics)
-__global__ void k_colsum(const float* __restrict__ X, int64_t M, int H, float* __restrict__ out) {
-  const int j = blockIdx.x;
+// out[i] = sum over s < splits of part[s * n + i] (fixed order), written or
+// added to out
+__global__ void k_sum_splits(const float* __restrict__ part, int splits, int64_t n,
+                             float* __restrict__ out, int accumulate) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
   float s = 0.f;
-  for (int64_t m = threadIdx.x; m < M; m += blockDim.x) s += X[m * H + j];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  __shared__ float sh[4];
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float t = 0.f;
-    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += sh[k];
-    out[j] = t;
+  for (int z = 0; z < splits; ++z) s += part[(int64_t)z * n + i];
+  out[i] = accumulate ? out[i] + s : s;
+}
+
+// column sums by row chunks: part[c][j] = sum of X[m][j] over the rows of
+// chunk c (threads over columns: coalesced rows)
+__global__ void k_colsum_chunks(const float* __restrict__ X, int64_t M, int H, int64_t rows,
+                                float* __restrict__ part) {
+  const int64_t c = blockIdx.x;
+  const int64_t m0 = c * rows, m1 = min(M, m0 + rows);
+  for (int j = threadIdx.x; j < H; j += blockDim.x) {
+    float s = 0.f;
+    for (int64_t m = m0; m < m1; ++m) s += X[m * H + j];
+    part[c * H + j] = s;
   }
 }
 
@@ -157,6 +173,46 @@ Views views(float* base, int n_in, int H) {
   return v;
 }
 
+// split-K plan of a weight gradient (reduction over K = the batch): enough
+// workgroups to fill the chip, each a multiple of 32 rows long
+struct SplitK {
+  int splits, kps;
+};
+SplitK split_k(int64_t K, int tiles) {
+  SplitK s;
+  const int64_t want = std::max<int64_t>(1, std::min<int64_t>((4096 + tiles - 1) / tiles, (K + 255) / 256));
+  s.kps = (int)((((K + want - 1) / want) + 31) / 32 * 32);
+  s.splits = (int)((K + s.kps - 1) / s.kps);
+  return s;
+}
+
+// grad (M x N, ldc = N) = A^T B over K rows, split over K into m->ws, then summed
+int grad_gemm(phy_ctx* ctx, phy_mlp* m, const float* A, int64_t lda, const float* Bm, int64_t ldb,
+              float* grad, int M, int N, int64_t K) {
+  const dim3 t((unsigned)((N + 15) / 16), (unsigned)((M + 15) / 16));
+  const SplitK s = split_k(K, (int)(t.x * t.y));
+  const int64_t n = (int64_t)M * N;
+  hipLaunchKernelGGL(k_gemm, dim3(t.x, t.y, (unsigned)s.splits), dim3(64), 0, ctx->stream, A, (int64_t)1,
+                     lda, Bm, ldb, (int64_t)1, m->ws, (int64_t)N, M, N, (int)K, 0, (const float*)nullptr,
+                     s.kps, n);
+  hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, m->ws,
+                     s.splits, n, grad, 0);
+  PHY_LAUNCHED(ctx);
+  return PHY_OK;
+}
+
+// out[j] = sum_m X[m][j], by row chunks into m->ws, then summed
+int colsum(phy_ctx* ctx, phy_mlp* m, const float* X, int64_t M, int H, float* out) {
+  const int64_t rows = std::max<int64_t>(64, (M + 1023) / 1024);
+  const int64_t chunks = (M + rows - 1) / rows;
+  hipLaunchKernelGGL(k_colsum_chunks, dim3((unsigned)chunks), dim3(std::min(H, 256)), 0, ctx->stream, X, M,
+                     H, rows, m->ws);
+  hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((H + 255) / 256)), dim3(256), 0, ctx->stream, m->ws,
+                     (int)chunks, (int64_t)H, out, 0);
+  PHY_LAUNCHED(ctx);
+  return PHY_OK;
+}
+
 int ensure_cap(phy_mlp* m, int64_t B) {
   if (B <= m->cap) return PHY_OK;
   if (m->act) PHY_HIP(m->ctx, hipFree(m->act));
@@ -180,11 +236,11 @@ int forward(phy_ctx* ctx, phy_mlp* m, const float* X, int64_t B, float* y) {
   const int Mi = (int)B;
   KernelSpan span(ctx, "mlp_forward");
   hipLaunchKernelGGL(k_gemm, tiles(Mi, H), dim3(64), 0, ctx->stream, X, (int64_t)n_in, (int64_t)1, p.W1,
-                     (int64_t)H, (int64_t)1, Z1, (int64_t)H, Mi, H, n_in, 0, p.b1);
+                     (int64_t)H, (int64_t)1, Z1, (int64_t)H, Mi, H, n_in, 0, p.b1, 1 << 30, (int64_t)0);
   const int64_t nh = B * H;
   hipLaunchKernelGGL(k_gelu_fwd, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, ctx->stream, Z1, H1, nh);
   hipLaunchKernelGGL(k_gemm, tiles(Mi, H), dim3(64), 0, ctx->stream, H1, (int64_t)H, (int64_t)1, p.W2,
-                     (int64_t)H, (int64_t)1, Z2, (int64_t)H, Mi, H, H, 0, p.b2);
+                     (int64_t)H, (int64_t)1, Z2, (int64_t)H, Mi, H, H, 0, p.b2, 1 << 30, (int64_t)0);
   hipLaunchKernelGGL(k_gelu_fwd, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, ctx->stream, Z2, H2, nh);
   hipLaunchKernelGGL(k_out, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, ctx->stream, H2, p.w3, p.b3, y, B, H);
   PHY_LAUNCHED(ctx);
@@ -209,24 +265,22 @@ int backward(phy_ctx* ctx, phy_mlp* m, const float* X, const float* t, int64_t B
   KernelSpan span(ctx, "mlp_backward");
   hipLaunchKernelGGL(k_loss, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, ctx->stream, y, t, dy, loss, B);
   // output layer: dw3 = H2^T dy (GEMM with N = 1), db3 = sum dy
-  hipLaunchKernelGGL(k_gemm, tiles(H, 1), dim3(64), 0, ctx->stream, H2, (int64_t)1, (int64_t)H, dy,
-                     (int64_t)1, (int64_t)1, g.w3, (int64_t)1, H, 1, Mi, 0, (const float*)nullptr);
-  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, ctx->stream, dy, B, 1, g.b3);
+  if ((rc = grad_gemm(ctx, m, H2, H, dy, 1, g.w3, H, 1, B))) return rc;
+  if ((rc = colsum(ctx, m, dy, B, 1, g.b3))) return rc;
   const int64_t nh = B * H;
   hipLaunchKernelGGL(k_gelu_bwd, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, ctx->stream,
                      (const float*)nullptr, dy, p.w3, Z2, dZ2, B, H);
   // dW2 = H1^T dZ2 ; db2 = colsum dZ2 ; dH1 = dZ2 W2^T
-  hipLaunchKernelGGL(k_gemm, tiles(H, H), dim3(64), 0, ctx->stream, H1, (int64_t)1, (int64_t)H, dZ2,
-                     (int64_t)H, (int64_t)1, g.W2, (int64_t)H, H, H, Mi, 0, (const float*)nullptr);
-  hipLaunchKernelGGL(k_colsum, dim3(H), dim3(256), 0, ctx->stream, dZ2, B, H, g.b2);
+  if ((rc = grad_gemm(ctx, m, H1, H, dZ2, H, g.W2, H, H, B))) return rc;
+  if ((rc = colsum(ctx, m, dZ2, B, H, g.b2))) return rc;
   hipLaunchKernelGGL(k_gemm, tiles(Mi, H), dim3(64), 0, ctx->stream, dZ2, (int64_t)H, (int64_t)1, p.W2,
-                     (int64_t)1, (int64_t)H, dH1, (int64_t)H, Mi, H, H, 0, (const float*)nullptr);
+                     (int64_t)1, (int64_t)H, dH1, (int64_t)H, Mi, H, H, 0, (const float*)nullptr,
+                     1 << 30, (int64_t)0);
   // dZ1 = dH1 * gelu'(Z1) (in place) ; dW1 = X^T dZ1 ; db1 = colsum dZ1
   hipLaunchKernelGGL(k_gelu_bwd, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, ctx->stream, dH1,
                      (const float*)nullptr, (const float*)nullptr, Z1, dH1, B, H);
-  hipLaunchKernelGGL(k_gemm, tiles(n_in, H), dim3(64), 0, ctx->stream, X, (int64_t)1, (int64_t)n_in, dH1,
-                     (int64_t)H, (int64_t)1, g.W1, (int64_t)H, n_in, H, Mi, 0, (const float*)nullptr);
-  hipLaunchKernelGGL(k_colsum, dim3(H), dim3(256), 0, ctx->stream, dH1, B, H, g.b1);
+  if ((rc = grad_gemm(ctx, m, X, n_in, dH1, H, g.W1, n_in, H, B))) return rc;
+  if ((rc = colsum(ctx, m, dH1, B, H, g.b1))) return rc;
   PHY_LAUNCHED(ctx);
   return PHY_OK;
 }
@@ -277,6 +331,7 @@ void phy_mlp_destroy(phy_mlp* m) {
   if (m->m1) (void)hipFree(m->m1);
   if (m->m2) (void)hipFree(m->m2);
   if (m->act) (void)hipFree(m->act);
+  if (m->ws) (void)hipFree(m->ws);
   delete m;
 }
 
@@ -311,6 +366,24 @@ int phy_mlp_backward(phy_ctx* ctx, phy_mlp* m, const float* d_x, const float* d_
   if (B > (1 << 30)) return set_err(ctx, PHY_ERANGE, "mlp: batch too large");
   int rc = ensure_cap(m, B);
   if (rc) return rc;
+  {
+    // split-K partials of the three weight gradients, and the column-sum chunks
+    const int H = m->H;
+    auto part = [&](int M, int N) {
+      const SplitK s = split_k(B, ((N + 15) / 16) * ((M + 15) / 16));
+      return (int64_t)s.splits * M * N;
+    };
+    const int64_t rows = std::max<int64_t>(64, (B + 1023) / 1024);
+    const int64_t need = std::max({part(H, 1), part(H, H), part(m->n_in, H),
+                                   ((B + rows - 1) / rows) * (int64_t)H});
+    if (m->ws_floats < need) {
+      if (m->ws) PHY_HIP(ctx, hipFree(m->ws));
+      m->ws = nullptr;
+      m->ws_floats = 0;
+      PHY_HIP(ctx, hipMalloc((void**)&m->ws, sizeof(float) * (size_t)need));
+      m->ws_floats = need;
+    }
+  }
   if (d_loss && (rc = phy_zero_u64(ctx, reinterpret_cast<uint64_t*>(d_loss), 1))) return rc;
   rc = backward(ctx, m, d_x, d_t, B, d_loss);
   if (rc) return rc;
