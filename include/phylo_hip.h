@@ -503,6 +503,28 @@ int phy_lk_spr_scores(phy_ctx* ctx, const phy_model* model, int32_t form,
                       int32_t root, const double* blen, int64_t n_moves,
                       const int32_t* moves, int32_t chunk, double* d_lnl, void* d_work,
                       size_t work_bytes);
+/* The same scores without rebuilding a tree per move (form PHY_F32 or
+ * PHY_F64, n_rates 1, 2, 4 or 8): per pruned subtree s, the base tree's
+ * stored partials are repaired along the ancestors of s's parent and one
+ * pre-order pass over the pruned tree scores the insertion at the middle of
+ * every branch (the lnL of every regraft of s is a sum over patterns of
+ * log sum_r prob_r [P(t_v/2)^T A_v] . ([P(t_v/2) D_v] (*) P(t_s) D_s)).
+ * About 2N node operations per pruned subtree instead of N per move; the
+ * values equal phy_lk_spr_scores' up to rounding.  s_batch pruned subtrees
+ * run per launch; the workspace depends on the tree (its longest repair
+ * path) and on s_batch.  d_base_lnl (nullable, device) receives the base
+ * tree's lnL.  Replaces the per-move evaluation of the candidates of
+ * TreeNeighborhoods' SPR generator (lib/treeNeighborhoods.ml:27-75), whose
+ * trees the reference scores one Tree.join at a time (lib/tree.ml:400-494). */
+int phy_lk_spr_incr_work_size(int32_t n_leaves, const int32_t* children, int32_t root,
+                              const phy_model* model, int64_t n_patterns, int32_t form,
+                              int32_t s_batch, size_t* bytes);
+int phy_lk_spr_scores_incr(phy_ctx* ctx, const phy_model* model, int32_t form,
+                           int64_t n_patterns, const uint64_t* d_tips, int64_t stride,
+                           const uint32_t* d_weights, int32_t n_leaves, const int32_t* children,
+                           int32_t root, const double* blen, int64_t n_moves,
+                           const int32_t* moves, int32_t s_batch, double* d_lnl,
+                           double* d_base_lnl, void* d_work, size_t work_bytes);
 /* The same for TBR moves (s, w, v), 3 ints each (workspace as for SPR). */
 int phy_lk_tbr_scores(phy_ctx* ctx, const phy_model* model, int32_t form,
                       int64_t n_patterns, const uint64_t* d_tips, int64_t stride,

@@ -138,6 +138,15 @@ inline int set_err(phy_ctx* ctx, int code, const std::string& msg) {
 // set); returns 0 on success.  Host only.
 int phy_compile_topo(phy_topo* t);
 
+// Pre-order program ({v, a, b, flags} per internal node, root first with
+// source SRC_LEAF) of the tree under `top` in the children layout ch of a
+// tree with N leaves; nodes not below `top` are ignored, so a tree with
+// nodes removed (a pruned subtree) compiles on the original node ids.  size
+// orders the walk (the smaller subtree continues in the accumulator); depth
+// receives the stack depth.  Host only.
+void phy_compile_up_program(int32_t N, const std::vector<int32_t>& ch, const std::vector<int32_t>& size,
+                            int32_t top, std::vector<int32_t>* prog, int32_t* depth);
+
 // Device copies of leaf / node id streams are padded past their end with a
 // valid id (the stream's first), so kernels prefetch whole batches of ids and
 // rows with no bounds check: two batches of at most 64 ids may be read past

@@ -364,6 +364,13 @@ std::vector<int32_t> choose_cuts(int32_t N, const std::vector<int32_t>& ch,
 
 }  // namespace
 
+void phy_compile_up_program(int32_t N, const std::vector<int32_t>& ch, const std::vector<int32_t>& size,
+                            int32_t top, std::vector<int32_t>* prog, int32_t* depth) {
+  std::vector<int32_t> fifo;
+  const Stop is_leaf = [N](int32_t v) { return v < N; };
+  compile_up(N, ch, size, top, is_leaf, SRC_LEAF, prog, &fifo, depth);
+}
+
 int phy_compile_topo(phy_topo* t) {
   const int32_t N = t->n_leaves, M = 2 * N - 1;
   const std::vector<int32_t>& ch = t->children;

@@ -1001,6 +1001,368 @@ __global__ void k_reduce_derivs(const double* __restrict__ ws, int64_t n_waves, 
 }
 
 // ---------------------------------------------------------------------------
+// Incremental SPR scoring.
+//
+// The move (s, v) prunes the subtree of s -- its parent p is removed and its
+// sibling x hangs from g = parent(p) with t_x + t_p (x becomes the root when
+// p is) -- and regrafts it at the middle of the branch above v.  With T' the
+// tree the prune leaves, D'_v the partial of v in T' and A'_v the message of
+// the pre-order pass of T' (L(T') = sum_r prob_r A'_v . (P(t_v) D'_v)), the
+// candidate's likelihood per pattern and rate class is, exactly,
+//   L_r = sum_j [P(t_v / 2)^T A'_v]_j [P(t_v / 2) D'_v]_j X_j,  X = P(t_s) D_s.
+// So a repair of the base tree's partials (only the ancestors g .. root of p
+// change) and one pre-order pass over T' score the insertion at every branch
+// of T': about 2N node operations per pruned subtree instead of N per move.
+// The half-branch products take the forms of the derivative pass,
+//   P(h)^T A = A + Ui^T (m (*) U^T A),  P(h) D = D + U (m (*) Ui D),
+// m = exp(lam rate h) - 1, which keep short branches precise.
+//
+// Partials live in one pool of rows in the stored layout of the post-order
+// pass (row i: R*4 value rows and R exponent rows of `pitch` patterns): rows
+// 0 .. N-2 hold the base tree's internal nodes (row v - N), the rows after
+// them the repaired ancestors.  ref[k][c] is the row of internal node c in
+// the pruned tree of subtree k.  meta[k] = {path offset, path length, x (-1
+// when p is the root), s, program entries, stack depth, 0, 0}.  Threads are
+// (pattern, rate) pairs, lane p*R + r, as in the derivative pass.
+// ---------------------------------------------------------------------------
+constexpr int kIncrMeta = 8;
+
+// y = P D = D + U (f (*) Ui D) for the factors f = exp(lam rate t) - 1; u
+// receives Ui D
+template <typename real>
+__device__ __forceinline__ void eig_apply(const real* U, const real* Ui, const real* f, const real* D,
+                                          real* u, real* y) {
+  real m[4], t[4];
+  matvec_u(Ui, D, u);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) m[k] = f[k] * u[k];
+  matvec_u(U, m, t);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) y[k] = D[k] + t[k];
+}
+
+template <typename real, int R>
+__device__ __forceinline__ void pool_read(const real* __restrict__ part, const int32_t* __restrict__ ex,
+                                          int row, int r, int64_t pitch, int64_t pc, real* D, int* e) {
+  const real* sv = part + (((size_t)row * R + r) * 4) * pitch + pc;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) D[k] = sv[(size_t)k * pitch];
+  *e = ex[((size_t)row * R + r) * pitch + pc];
+}
+
+// Repair along the path g .. root of each pruned subtree (grid.y): ops
+// {y, a, b, flags} bottom-up, flags bit 0 (bit 1): a (b) is the previous
+// op's node, held in registers.  The branch of x takes the merged factors
+// Ex[k]; the other branches keep the base tree's E.
+template <typename real, int R>
+__global__ __launch_bounds__(256) void k_spr_repair(
+    const int4* __restrict__ ops, const int32_t* __restrict__ meta, const int32_t* __restrict__ ref,
+    const real* __restrict__ tab, const real* __restrict__ E, const real* __restrict__ Ex,
+    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
+    real* __restrict__ part, int32_t* __restrict__ ex, int N, int M) {
+  const int k = blockIdx.y;
+  const int32_t* mk = meta + kIncrMeta * k;
+  const int n = mk[1];
+  const int tid = threadIdx.x;
+  const int r = tid % R;
+  const int64_t p = (int64_t)blockIdx.x * (blockDim.x / R) + tid / R;
+  if (n == 0 || p >= n_pat) return;  // no barriers below
+  const int32_t* rf = ref + (size_t)k * M;
+  const int x = mk[2];
+  const int4* op = ops + mk[0];
+  real U[16], Ui[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    U[i] = tab[i];
+    Ui[i] = tab[16 + i];
+  }
+  real acc[4] = {(real)0, (real)0, (real)0, (real)0};
+  int acce = 0;
+  for (int i = 0; i < n; ++i) {
+    const int4 o = op[i];
+    real y[2][4];
+    int ye[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int node = c ? o.z : o.y;
+      real D[4], f[4], u[4];
+      int e;
+      if ((o.w >> c) & 1) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) D[j] = acc[j];
+        e = acce;
+      } else if (node < N) {
+        tipvec(tip_code(tips, stride, node, p), D);
+        e = 0;
+      } else {
+        pool_read<real, R>(part, ex, rf[node], r, pitch, p, D, &e);
+      }
+      const real* fs = (node == x) ? Ex + ((size_t)k * R + r) * 4 : E + ((size_t)node * R + r) * 4;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) f[j] = fs[j];
+      eig_apply(U, Ui, f, D, u, y[c]);
+      ye[c] = e;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = y[0][j] * y[1][j];
+    acce = ye[0] + ye[1];
+    rescale4(acc, &acce);
+    const int row = rf[o.x];
+    real* d = part + (((size_t)row * R + r) * 4) * pitch + p;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) d[(size_t)j * pitch] = acc[j];
+    ex[((size_t)row * R + r) * pitch + p] = acce;
+  }
+}
+
+// Pre-order pass over each pruned tree (grid.y = subtree k) scoring the
+// insertion of X at the middle of every branch: per wave and target c, the
+// weighted sum of the per-pattern lnL goes to wsum[(k * M + c) * n_waves +
+// wave].  The message flow is the derivative pass's; each entry adds the
+// insertion at both children.
+template <typename real, int R>
+__global__ __launch_bounds__(256) void k_spr_insert(
+    const int4* __restrict__ prog, int64_t prog_stride, const int32_t* __restrict__ meta,
+    const int32_t* __restrict__ ref, const real* __restrict__ tab, const real* __restrict__ E,
+    const real* __restrict__ Eh, const real* __restrict__ Ex, const double* __restrict__ pi,
+    const double* __restrict__ probs, const uint64_t* __restrict__ tips, int64_t stride,
+    int64_t n_pat, int64_t pitch, const real* __restrict__ part, const int32_t* __restrict__ ex,
+    const uint32_t* __restrict__ weights, double* __restrict__ wsum, int64_t n_waves, int N, int M,
+    int up_depth) {
+  constexpr int QPL = R >= 2 ? 1 : 2;  // insertion targets per lane
+  extern __shared__ unsigned char lds_raw[];
+  const int k = blockIdx.y;
+  const int32_t* mk = meta + kIncrMeta * k;
+  const int n_ops = mk[4];
+  if (n_ops == 0) return;  // the whole block: no barriers below
+  const int tid = threadIdx.x, bd = blockDim.x;
+  real* stk = reinterpret_cast<real*>(lds_raw);                                  // [depth][4][bd]
+  int32_t* stke = reinterpret_cast<int32_t*>(stk + (size_t)up_depth * 4 * bd);  // [depth][bd]
+  const int r = tid % R;
+  const int64_t p = (int64_t)blockIdx.x * (bd / R) + tid / R;
+  const bool act = p < n_pat;
+  const int64_t pc = act ? p : 0;
+  const double wp = act ? (double)weights[p] : 0.0;
+  const int64_t wave = ((int64_t)blockIdx.x * bd + tid) >> 6;
+  const int lane = tid & 63;
+  const int32_t* rf = ref + (size_t)k * M;
+  const int x = mk[2], s = mk[3];
+  const int4* pg = prog + (size_t)k * prog_stride;
+  double* ws = wsum + (size_t)k * M * n_waves;
+
+  real U[16], Ui[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    U[i] = tab[i];
+    Ui[i] = tab[16 + i];
+  }
+  const double pr = probs[r];
+
+  // X = P(t_s) D_s: the subtree of s and its branch are untouched
+  real X[4];
+  int eX;
+  {
+    real D[4], f[4], u[4];
+    if (s < N) {
+      tipvec(tip_code(tips, stride, s, pc), D);
+      eX = 0;
+    } else {
+      pool_read<real, R>(part, ex, s - N, r, pitch, pc, D, &eX);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) f[j] = E[((size_t)s * R + r) * 4 + j];
+    eig_apply(U, Ui, f, D, u, X);
+  }
+
+  // partial, exponent, full and half factors of a child, fetched one node
+  // ahead
+  auto fetch = [&](int c, real* Dv, int* e, real* ev, real* hv) {
+    if (c < N) {
+      tipvec(tip_code(tips, stride, c, pc), Dv);
+      *e = 0;
+    } else {
+      pool_read<real, R>(part, ex, rf[c], r, pitch, pc, Dv, e);
+    }
+    const real* es = (c == x) ? Ex + ((size_t)k * R + r) * 4 : E + ((size_t)c * R + r) * 4;
+    const real* hs = Eh + ((size_t)c * R + r) * 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      ev[j] = es[j];
+      hv[j] = hs[j];
+    }
+  };
+  // insertion at the middle of the branch above a child with partial D
+  // (exponent eD, u = Ui D, half factors h) under the arriving message A
+  // (exponent eA, gA = U^T A): the class's term and, after the FP64 rate sum
+  // aligned to the largest exponent among the non-zero terms, L and m with
+  // the pattern's likelihood L 2^m
+  auto insert = [&](const real* A, const real* gA, int eA, const real* D, const real* u, int eD,
+                    const real* h, double* L, int* mo) {
+    real t[4], left[4], q[4], right[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) t[j] = h[j] * gA[j];
+    vecmat_u(Ui, t, left);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) left[j] += A[j];
+    int eL = eA;
+    rescale4(left, &eL);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) q[j] = h[j] * u[j];
+    matvec_u(U, q, right);
+    real sv = (real)0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sv = fma(left[j] * (D[j] + right[j]), X[j], sv);
+    const int Ev = eL + eD + eX;
+    const bool nz = sv != (real)0;
+    int m = nz ? Ev : kNoExp;
+    if (R >= 2) m = max(m, __builtin_amdgcn_update_dpp(0, m, DPP_XOR1, 0xF, 0xF, false));
+    if (R >= 4) m = max(m, __builtin_amdgcn_update_dpp(0, m, DPP_XOR2, 0xF, 0xF, false));
+    if (R >= 8) m = max(m, __builtin_amdgcn_update_dpp(0, m, DPP_HALF_MIRROR, 0xF, 0xF, false));
+    const double sc = nz ? ldexp(pr, Ev - m) : 0.0;
+    *L = rate_sum<R>(sc * (double)sv);
+    *mo = m;
+  };
+
+  real Dc[8], Ec[8], Hc[8];
+  int Xc[2];
+  {
+    const int4 o = pg[0];
+    fetch(o.y, Dc, &Xc[0], Ec, Hc);
+    fetch(o.z, Dc + 4, &Xc[1], Ec + 4, Hc + 4);
+  }
+  real accM[4] = {(real)0, (real)0, (real)0, (real)0};
+  int accE = 0;
+  int sp = 0;
+  for_each_entry_ahead(pg, n_ops, [&](int i, const int4 op, const int4 on) {
+    real Dn[8], En[8], Hn[8];
+    int Xn[2] = {0, 0};
+    const bool more = i + 1 < n_ops;
+    if (more) {
+      fetch(on.y, Dn, &Xn[0], En, Hn);
+      fetch(on.z, Dn + 4, &Xn[1], En + 4, Hn + 4);
+    }
+    const int fl = op.w, src = fl & 3;
+    const int a = op.y, b = op.z;
+    real Mv[4];
+    int eM;
+    if (src == SRC_LEAF) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Mv[j] = (real)pi[j];
+      eM = 0;
+    } else if (src == SRC_ACC) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Mv[j] = accM[j];
+      eM = accE;
+    } else {
+      --sp;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Mv[j] = stk[((size_t)sp * 4 + j) * bd + tid];
+      eM = stke[(size_t)sp * bd + tid];
+    }
+    real ua[4], ub[4], ya[4], yb[4], Aa[4], Ab[4], ga[4], gb[4];
+    eig_apply(U, Ui, Ec, Dc, ua, ya);
+    eig_apply(U, Ui, Ec + 4, Dc + 4, ub, yb);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      Aa[j] = Mv[j] * yb[j];
+      Ab[j] = Mv[j] * ya[j];
+    }
+    vecmat_u(U, Aa, ga);
+    vecmat_u(U, Ab, gb);
+    double Lq[2];
+    int mq[2];
+    insert(Aa, ga, eM + Xc[1], Dc, ua, Xc[0], Hc, &Lq[0], &mq[0]);
+    insert(Ab, gb, eM + Xc[0], Dc + 4, ub, Xc[1], Hc + 4, &Lq[1], &mq[1]);
+    // weighted per-pattern lnL of the two targets, spread over the R lanes
+    // of the pattern (lane r takes target r * QPL + j), summed over each
+    // 16-lane row's lanes of one target, then over the wave's four rows
+    {
+      double qv[QPL];
+#pragma unroll
+      for (int j = 0; j < QPL; ++j) {
+        const int q = r * QPL + j;
+        double v = 0.0;
+        if (act && q < 2) {
+          const double L = q ? Lq[1] : Lq[0];
+          const int m = q ? mq[1] : mq[0];
+          v = wp * (log(L) + (double)m * 0.6931471805599453);
+        }
+        qv[j] = row_sum_stride<R>(v);
+      }
+      double S[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const double vq = (QPL == 1) ? qv[0] : qv[q];
+        const int l0 = (QPL == 1) ? q : 0;
+        S[q] = (readlane(vq, l0) + readlane(vq, 16 + l0)) + (readlane(vq, 32 + l0) + readlane(vq, 48 + l0));
+      }
+      if (lane == 0) {
+        ws[(size_t)a * n_waves + wave] = S[0];
+        ws[(size_t)b * n_waves + wave] = S[1];
+      }
+    }
+    // messages to internal children, as in the derivative pass
+    real Ma[4], Mb[4];
+    int eMa = eM + Xc[1], eMb = eM + Xc[0];
+    if (!(fl & UP_A_LEAF)) {
+      real t[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) t[j] = Ec[j] * ga[j];
+      vecmat_u(Ui, t, Ma);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Ma[j] += Aa[j];
+      rescale4(Ma, &eMa);
+    }
+    if (!(fl & UP_B_LEAF)) {
+      real t[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) t[j] = Ec[4 + j] * gb[j];
+      vecmat_u(Ui, t, Mb);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Mb[j] += Ab[j];
+      rescale4(Mb, &eMb);
+    }
+    if (fl & UP_PUSH) {
+      const bool cb = fl & UP_CONT_B;
+      const real* o = cb ? Ma : Mb;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) stk[((size_t)sp * 4 + j) * bd + tid] = o[j];
+      stke[(size_t)sp * bd + tid] = cb ? eMa : eMb;
+      ++sp;
+    }
+    if (!(fl & UP_A_LEAF) || !(fl & UP_B_LEAF)) {
+      const bool cb = fl & UP_CONT_B;
+      const real* c = cb ? Mb : Ma;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) accM[j] = c[j];
+      accE = cb ? eMb : eMa;
+    }
+    if (more) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        Dc[j] = Dn[j];
+        Ec[j] = En[j];
+        Hc[j] = Hn[j];
+      }
+      Xc[0] = Xn[0];
+      Xc[1] = Xn[1];
+    }
+  });
+}
+
+// out[idx[j]] = compensated sum of row rows[j] of x (n values per row); one
+// block per j
+__global__ void k_reduce_gather(const double* __restrict__ x, int64_t n, const int64_t* __restrict__ rows,
+                                const int64_t* __restrict__ idx, double* __restrict__ out) {
+  const int64_t j = blockIdx.x;
+  const double* r = x + rows[j] * n;
+  CSum a;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) a.add(r[i]);
+  const double t = block_csum(a);
+  if (threadIdx.x == 0) out[idx[j]] = t;
+}
+
+// ---------------------------------------------------------------------------
 // Per-node API kernels.  A value is R*K rows of `pitch` values, row r*K + i
 // holding L[r][i][p], and R rows of int32 exponents e[r][p] (the layout of
 // the whole-tree stored partials): every (pattern, rate class) is rescaled
@@ -1785,6 +2147,410 @@ int phy_lk_tbr_scores(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_
                             });
   } catch (...) {
     return set_err(ctx, PHY_ENOMEM, "tbr scores: host allocation or thread start failed");
+  }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Incremental SPR scoring (host): the base tree's stored post-order pass
+// once, then per batch of pruned subtrees the repair of the path g .. root,
+// the insertion pre-order pass and the per-move sums.
+// ---------------------------------------------------------------------------
+namespace {
+
+// rooted tree with parents, leaf counts, depths and Euler intervals
+struct IncrTree {
+  int32_t N = 0, M = 0, root = -1;
+  std::vector<int32_t> ch, parent, size, depth, tin, tout;
+  int32_t lmax = 0;  // longest repair path: most ancestors of a non-root internal node
+
+  bool load(int32_t n, const int32_t* children, int32_t r) {
+    int64_t n_nni = 0;
+    if (phy_nni_moves(n, children, r, &n_nni, nullptr, 0) != PHY_OK) return false;
+    N = n;
+    M = 2 * n - 1;
+    root = r;
+    ch.assign(children, children + 2 * (n - 1));
+    parent.assign(M, -1);
+    for (int32_t k = 0; k < n - 1; ++k) parent[ch[2 * k]] = parent[ch[2 * k + 1]] = n + k;
+    size.assign(M, 1);
+    depth.assign(M, 0);
+    tin.assign(M, 0);
+    tout.assign(M, 0);
+    std::vector<std::pair<int32_t, int>> st = {{root, 0}};
+    int32_t t = 0;
+    while (!st.empty()) {
+      auto& f = st.back();
+      const int32_t v = f.first;
+      if (f.second == 0) {
+        tin[v] = t++;
+        f.second = 1;
+        if (v >= N) {
+          const int32_t a = ch[2 * (v - N)], b = ch[2 * (v - N) + 1];
+          depth[a] = depth[b] = depth[v] + 1;
+          st.push_back({b, 0});
+          st.push_back({a, 0});
+        }
+      } else {
+        tout[v] = t;
+        if (v >= N) size[v] = size[ch[2 * (v - N)]] + size[ch[2 * (v - N) + 1]];
+        st.pop_back();
+      }
+    }
+    lmax = 0;
+    for (int32_t v = N; v < M; ++v)
+      if (v != root) lmax = std::max(lmax, depth[v]);
+    return true;
+  }
+  bool below(int32_t a, int32_t b) const { return tin[a] <= tin[b] && tout[b] <= tout[a]; }  // b under a
+  int32_t sibling(int32_t s) const {
+    const int32_t p = parent[s];
+    return ch[2 * (p - N)] == s ? ch[2 * (p - N) + 1] : ch[2 * (p - N)];
+  }
+};
+
+struct IncrPlan {
+  Plan base;  // stored post-order pass of the base tree
+  size_t off_bl = 0, off_E = 0, off_Eh = 0, off_Ex = 0, off_tab = 0, off_dprog = 0, off_dfifo = 0;
+  size_t off_P = 0, off_part = 0, off_ex = 0, off_bsum = 0, off_lnl = 0;
+  size_t off_prog = 0, off_meta = 0, off_ref = 0, off_ops = 0, off_wsum = 0, off_rows = 0, total = 0;
+  int64_t pitch = 0, n_waves = 0, pool_rows = 0, max_moves = 0;
+  int S = 0, lmax = 0;
+  size_t esz = 8;
+};
+
+IncrPlan incr_plan(const IncrTree& t, const phy_model* m, int64_t n_pat, int32_t form, int32_t S) {
+  IncrPlan pl;
+  const int N = t.N, M = t.M, R = m->R, K = 4;
+  pl.esz = (form & PHY_F32) ? 4 : 8;
+  pl.S = S;
+  pl.lmax = std::max(t.lmax, 1);
+  pl.pitch = ((n_pat + 255) / 256) * 256;
+  pl.n_waves = pl.pitch * R / 64;
+  pl.pool_rows = (int64_t)(N - 1) + (int64_t)S * pl.lmax;
+  // every move of a batch has a distinct (subtree, target) pair
+  pl.max_moves = (int64_t)S * M;
+  const size_t esz = pl.esz;
+  size_t o = 0;
+  pl.off_bl = o; o = align256(o + (size_t)(2 * M + S) * 8);
+  pl.off_E = o; o = align256(o + (size_t)M * R * K * esz);
+  pl.off_Eh = o; o = align256(o + (size_t)M * R * K * esz);
+  pl.off_Ex = o; o = align256(o + (size_t)S * R * K * esz);
+  pl.off_tab = o; o = align256(o + (size_t)(2 * K * K + R * K) * esz);
+  pl.off_dprog = o; o = align256(o + (size_t)(N - 1) * 16);
+  pl.off_dfifo = o; o = align256(o + phy_fifo_padded(N) * 4);
+  pl.off_P = o; o = align256(o + (size_t)M * R * K * K * esz);
+  pl.off_part = o; o = align256(o + (size_t)pl.pool_rows * R * K * pl.pitch * esz);
+  pl.off_ex = o; o = align256(o + (size_t)pl.pool_rows * R * pl.pitch * 4);
+  pl.off_bsum = o; o = align256(o + (size_t)(pl.pitch / 64) * 8);
+  pl.off_lnl = o; o = align256(o + 8);
+  pl.off_prog = o; o = align256(o + (size_t)S * (N - 1) * 16);
+  pl.off_meta = o; o = align256(o + (size_t)S * kIncrMeta * 4);
+  pl.off_ref = o; o = align256(o + (size_t)S * M * 4);
+  pl.off_ops = o; o = align256(o + (size_t)S * pl.lmax * 16);
+  pl.off_wsum = o; o = align256(o + (size_t)S * M * pl.n_waves * 8);
+  pl.off_rows = o; o = align256(o + (size_t)pl.max_moves * 16);
+  pl.total = o;
+  return pl;
+}
+
+// Per pruned subtree: the pre-order program of the pruned tree on the
+// original node ids, pool rows, repair ops and the merged branch of x.
+struct IncrSub {
+  int32_t s = -1, x = -1, n_ops = 0, depth = 0;
+  double merged = 0.0;
+  std::vector<int32_t> prog, ref, ops;  // ops: {y, a, b, flags} bottom-up
+};
+
+void incr_prepare(const IncrTree& t, const double* blen, int32_t s, int slot, int lmax, IncrSub* o) {
+  const int32_t N = t.N, M = t.M;
+  const int32_t p = t.parent[s], x = t.sibling(s), g = t.parent[p];
+  std::vector<int32_t> ch = t.ch, size = t.size;
+  int32_t top = x;
+  o->s = s;
+  o->x = -1;
+  o->merged = 0.0;
+  o->ops.clear();
+  o->ref.resize(M);
+  for (int32_t c = 0; c < M; ++c) o->ref[c] = c >= N ? c - N : 0;
+  if (g >= 0) {
+    (ch[2 * (g - N)] == p ? ch[2 * (g - N)] : ch[2 * (g - N) + 1]) = x;
+    top = t.root;
+    o->x = x;
+    o->merged = blen[x] + blen[p];
+    int32_t prev = -1, j = 0;
+    for (int32_t y = g; y >= 0; y = t.parent[y], ++j) {
+      size[y] -= t.size[s];
+      o->ref[y] = (N - 1) + slot * lmax + j;
+      const int32_t a = ch[2 * (y - N)], b = ch[2 * (y - N) + 1];
+      o->ops.insert(o->ops.end(), {y, a, b, (a == prev ? 1 : 0) | (b == prev ? 2 : 0)});
+      prev = y;
+    }
+  }
+  o->prog.clear();
+  o->depth = 0;
+  if (top >= N) phy_compile_up_program(N, ch, size, top, &o->prog, &o->depth);
+  o->n_ops = (int32_t)o->prog.size() / 4;
+}
+
+template <typename real, int R>
+int incr_launch(phy_ctx* ctx, const phy_model* m, const IncrTree& t, int64_t n_pat, const uint64_t* tips,
+                int64_t stride, const uint32_t* weights, const double* blen, const IncrPlan& pl,
+                char* work, const std::vector<int32_t>& subs, const std::vector<int64_t>& mv_start,
+                const std::vector<int64_t>& mv_rows, const std::vector<int64_t>& mv_idx, double* d_lnl,
+                double* d_base) {
+  const int N = t.N, M = t.M, K = 4;
+  const double* cst = m->d_consts;
+  const double* d_pi = cst + 2 * K * K + K + R;
+  const double* d_probs = d_pi + K;
+  double* bl = reinterpret_cast<double*>(work + pl.off_bl);
+  real* E = reinterpret_cast<real*>(work + pl.off_E);
+  real* Eh = reinterpret_cast<real*>(work + pl.off_Eh);
+  real* Ex = reinterpret_cast<real*>(work + pl.off_Ex);
+  real* tab = reinterpret_cast<real*>(work + pl.off_tab);
+  real* part = reinterpret_cast<real*>(work + pl.off_part);
+  int32_t* ex = reinterpret_cast<int32_t*>(work + pl.off_ex);
+  double* wsum = reinterpret_cast<double*>(work + pl.off_wsum);
+  int64_t* rows = reinterpret_cast<int64_t*>(work + pl.off_rows);
+  int64_t* idx = rows + pl.max_moves;
+  std::vector<double> hb(2 * (size_t)M);
+  for (int c = 0; c < M; ++c) {
+    hb[c] = blen[c];
+    hb[M + c] = 0.5 * blen[c];
+  }
+  PHY_HIP(ctx, hipMemcpyAsync(bl, hb.data(), hb.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  const int64_t ne = std::max<int64_t>((int64_t)M * R * K, 32 + 4 * R);
+  hipLaunchKernelGGL((k_eigen_exp<real>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, ctx->stream, cst,
+                     R, bl, M, E, tab);
+  PHY_LAUNCHED(ctx);
+  hipLaunchKernelGGL((k_eigen_exp<real>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, ctx->stream, cst,
+                     R, bl + M, M, Eh, tab);
+  PHY_LAUNCHED(ctx);
+
+  // the base tree's partials: stored post-order pass into pool rows 0 .. N-2
+  {
+    phy_topo* bt = nullptr;
+    int rc = phy_topo_create(nullptr, N, t.ch.data(), t.root, &bt);
+    if (rc) return set_err(ctx, rc, "spr incremental: base topology");
+    std::vector<int32_t> fifo(phy_fifo_padded(N), bt->down_fifo[0]);
+    std::copy(bt->down_fifo.begin(), bt->down_fifo.end(), fifo.begin());
+    hipError_t e = hipMemcpyAsync(work + pl.off_dprog, bt->down_prog.data(), bt->down_prog.size() * 4,
+                                  hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(work + pl.off_dfifo, fifo.data(), fifo.size() * 4, hipMemcpyHostToDevice, ctx->stream);
+    // the host sources (and hb) go out of scope: wait for the copies
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      phy_topo_destroy(bt);
+      return set_err(ctx, PHY_EDEVICE, std::string("spr incremental: ") + hipGetErrorString(e));
+    }
+    bt->d_down_prog = reinterpret_cast<int4*>(work + pl.off_dprog);
+    bt->d_down_fifo = reinterpret_cast<int32_t*>(work + pl.off_dfifo);
+    Plan bp = pl.base;
+    bp.off_P = pl.off_P;
+    bp.off_part = pl.off_part;
+    bp.off_ex = pl.off_ex;
+    bp.off_bsum = pl.off_bsum;
+    double* base_lnl = d_base ? d_base : reinterpret_cast<double*>(work + pl.off_lnl);
+    rc = launch_tree<real, R>(ctx, bt, m, false, false, n_pat, tips, stride, weights, bl, base_lnl, nullptr,
+                              nullptr, nullptr, work, bp);
+    bt->d_down_prog = nullptr;
+    bt->d_down_fifo = nullptr;
+    phy_topo_destroy(bt);
+    if (rc) return rc;
+  }
+
+  // per batch of pruned subtrees
+  const int n_sub = (int)subs.size();
+  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<IncrSub> sub(pl.S);
+  std::vector<int32_t> hprog, hmeta, href, hops;
+  std::vector<double> hmerged;
+  const int dbs_want = env_block("PHY_SPR_INCR_BLOCK", 256);
+  for (int b0 = 0; b0 < n_sub; b0 += pl.S) {
+    const int S = std::min(pl.S, n_sub - b0);
+    {
+      std::vector<std::thread> th;
+      for (unsigned w = 0; w < std::min<unsigned>(nt, S); ++w)
+        th.emplace_back([&, w] {
+          for (int k = (int)w; k < S; k += (int)nt) incr_prepare(t, blen, subs[b0 + k], k, pl.lmax, &sub[k]);
+        });
+      for (auto& x : th) x.join();
+    }
+    hprog.assign((size_t)S * (N - 1) * 4, 0);
+    hmeta.assign((size_t)S * kIncrMeta, 0);
+    href.assign((size_t)S * M, 0);
+    hops.assign((size_t)S * pl.lmax * 4, 0);
+    hmerged.assign(S, 0.0);
+    int depth = 1;
+    for (int k = 0; k < S; ++k) {
+      const IncrSub& q = sub[k];
+      std::copy(q.prog.begin(), q.prog.end(), hprog.begin() + (size_t)k * (N - 1) * 4);
+      std::copy(q.ref.begin(), q.ref.end(), href.begin() + (size_t)k * M);
+      std::copy(q.ops.begin(), q.ops.end(), hops.begin() + (size_t)k * pl.lmax * 4);
+      int32_t* mk = &hmeta[(size_t)k * kIncrMeta];
+      mk[0] = k * pl.lmax;
+      mk[1] = (int32_t)q.ops.size() / 4;
+      mk[2] = q.x;
+      mk[3] = q.s;
+      mk[4] = q.n_ops;
+      mk[5] = q.depth;
+      hmerged[k] = q.merged;
+      depth = std::max(depth, q.depth);
+    }
+    const int64_t m0 = mv_start[b0], m1 = mv_start[b0 + S];
+    PHY_HIP(ctx, hipMemcpyAsync(work + pl.off_prog, hprog.data(), hprog.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    PHY_HIP(ctx, hipMemcpyAsync(work + pl.off_meta, hmeta.data(), hmeta.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    PHY_HIP(ctx, hipMemcpyAsync(work + pl.off_ref, href.data(), href.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    PHY_HIP(ctx, hipMemcpyAsync(work + pl.off_ops, hops.data(), hops.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    PHY_HIP(ctx, hipMemcpyAsync(bl + 2 * M, hmerged.data(), (size_t)S * 8, hipMemcpyHostToDevice, ctx->stream));
+    const int64_t nx = std::max<int64_t>((int64_t)S * R * K, 32 + 4 * R);
+    hipLaunchKernelGGL((k_eigen_exp<real>), dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, ctx->stream,
+                       cst, R, bl + 2 * M, S, Ex, tab);
+    PHY_LAUNCHED(ctx);
+    const auto* d_meta = reinterpret_cast<const int32_t*>(work + pl.off_meta);
+    const auto* d_ref = reinterpret_cast<const int32_t*>(work + pl.off_ref);
+    {
+      KernelSpan span(ctx, "spr_repair");
+      hipLaunchKernelGGL((k_spr_repair<real, R>), dim3((unsigned)(pl.pitch * R / 256), (unsigned)S), dim3(256), 0,
+                         ctx->stream, reinterpret_cast<const int4*>(work + pl.off_ops), d_meta, d_ref, tab, E,
+                         Ex, tips, stride, n_pat, pl.pitch, part, ex, N, M);
+    }
+    PHY_LAUNCHED(ctx);
+    const int dbs = fit_block(dbs_want, [&](int bs) { return deriv_lds(R, sizeof(real), bs, depth); });
+    if (!dbs) return set_err(ctx, PHY_ERANGE, "spr incremental: LDS stack too deep");
+    {
+      KernelSpan span(ctx, "spr_insert");
+      hipLaunchKernelGGL((k_spr_insert<real, R>), dim3((unsigned)(pl.pitch * R / dbs), (unsigned)S), dim3(dbs),
+                         deriv_lds(R, sizeof(real), dbs, depth), ctx->stream,
+                         reinterpret_cast<const int4*>(work + pl.off_prog), (int64_t)(N - 1), d_meta, d_ref,
+                         tab, E, Eh, Ex, d_pi, d_probs, tips, stride, n_pat, pl.pitch, part, ex, weights, wsum,
+                         pl.n_waves, N, M, depth);
+    }
+    PHY_LAUNCHED(ctx);
+    // per-move sums, max_moves at a time (a list may repeat a move); the
+    // sources stay alive for the whole call and stream order keeps each
+    // upload behind the previous gather
+    for (int64_t c0 = m0; c0 < m1; c0 += pl.max_moves) {
+      const int64_t c = std::min<int64_t>(pl.max_moves, m1 - c0);
+      PHY_HIP(ctx, hipMemcpyAsync(rows, mv_rows.data() + c0, (size_t)c * 8, hipMemcpyHostToDevice, ctx->stream));
+      PHY_HIP(ctx, hipMemcpyAsync(idx, mv_idx.data() + c0, (size_t)c * 8, hipMemcpyHostToDevice, ctx->stream));
+      hipLaunchKernelGGL(k_reduce_gather, dim3((unsigned)c), dim3(256), 0, ctx->stream, wsum, pl.n_waves, rows,
+                         idx, d_lnl);
+      PHY_LAUNCHED(ctx);
+    }
+    // the host staging vectors are refilled by the next batch
+    PHY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return PHY_OK;
+}
+
+// Moves grouped by pruned subtree (ascending s): subs, the first move of
+// each subtree in the grouped order (n_sub + 1 entries), and per grouped
+// move its wsum row (position of s in its batch * M + v) and its index in
+// the caller's list.  Invalid moves give their index in *bad.
+bool incr_group(const IncrTree& t, int64_t n_moves, const int32_t* moves, int S, std::vector<int32_t>* subs,
+                std::vector<int64_t>* start, std::vector<int64_t>* rows, std::vector<int64_t>* idx,
+                int64_t* bad) {
+  const int32_t M = t.M;
+  std::vector<int64_t> count(M, 0);
+  for (int64_t i = 0; i < n_moves; ++i) {
+    const int32_t s = moves[2 * i], v = moves[2 * i + 1];
+    bool ok = s >= 0 && s < M && s != t.root && v >= 0 && v < M;
+    if (ok) {
+      const int32_t p = t.parent[s], x = t.sibling(s);
+      const int32_t new_root = (p == t.root) ? x : t.root;
+      ok = !t.below(s, v) && v != p && v != x && v != new_root;
+    }
+    if (!ok) {
+      *bad = i;
+      return false;
+    }
+    ++count[s];
+  }
+  std::vector<int32_t> pos(M, -1);
+  subs->clear();
+  for (int32_t s = 0; s < M; ++s)
+    if (count[s]) {
+      pos[s] = (int32_t)subs->size();
+      subs->push_back(s);
+    }
+  start->assign(subs->size() + 1, 0);
+  for (size_t k = 0; k < subs->size(); ++k) (*start)[k + 1] = (*start)[k] + count[(*subs)[k]];
+  std::vector<int64_t> fill(start->begin(), start->end() - 1);
+  rows->assign(n_moves, 0);
+  idx->assign(n_moves, 0);
+  for (int64_t i = 0; i < n_moves; ++i) {
+    const int32_t s = moves[2 * i], v = moves[2 * i + 1];
+    const int64_t j = fill[pos[s]]++;
+    (*rows)[j] = (int64_t)(pos[s] % S) * M + v;
+    (*idx)[j] = i;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phy_lk_spr_incr_work_size(int32_t N, const int32_t* children, int32_t root, const phy_model* m,
+                              int64_t n_pat, int32_t form, int32_t s_batch, size_t* bytes) {
+  if (!m || !bytes || !children || N < 3 || n_pat < 1 || s_batch < 1) return PHY_EINVAL;
+  try {
+    IncrTree t;
+    if (!t.load(N, children, root)) return PHY_EINVAL;
+    IncrPlan pl = incr_plan(t, m, n_pat, form, s_batch);
+    *bytes = pl.total;
+    return PHY_OK;
+  } catch (...) {
+    return PHY_ENOMEM;
+  }
+}
+
+int phy_lk_spr_scores_incr(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat,
+                           const uint64_t* tips, int64_t stride, const uint32_t* weights, int32_t N,
+                           const int32_t* children, int32_t root, const double* blen, int64_t n_moves,
+                           const int32_t* moves, int32_t s_batch, double* d_lnl, double* d_base_lnl,
+                           void* d_work, size_t work_bytes) {
+  int rc = check_scores_args(ctx, m, n_pat, tips, stride, weights, N, children, blen, n_moves, moves,
+                             s_batch, d_lnl, d_work, root, "spr incremental");
+  if (rc) return rc;
+  try {
+    IncrTree t;
+    if (!t.load(N, children, root)) return set_err(ctx, PHY_EINVAL, "spr incremental: not a rooted binary tree");
+    IncrPlan pl = incr_plan(t, m, n_pat, form, s_batch);
+    if (work_bytes < pl.total) return set_err(ctx, PHY_EINVAL, "spr incremental: workspace too small");
+    {
+      phy_topo* bt = nullptr;
+      if (phy_topo_create(nullptr, N, children, root, &bt) != PHY_OK)
+        return set_err(ctx, PHY_EINVAL, "spr incremental: base topology");
+      pl.base = make_plan(bt, m, n_pat, form & ~PHY_FUSED, 0);
+      phy_topo_destroy(bt);
+    }
+    std::vector<int32_t> subs;
+    std::vector<int64_t> start, rows, idx;
+    int64_t bad = -1;
+    if (!incr_group(t, n_moves, moves, s_batch, &subs, &start, &rows, &idx, &bad))
+      return set_err(ctx, PHY_EINVAL, "spr incremental: invalid move " + std::to_string(bad));
+    char* work = static_cast<char*>(d_work);
+    switch ((form & PHY_F32 ? 10 : 0) + m->R) {
+#define PHY_INCR(real, Rv)                                                                                   \
+  return incr_launch<real, Rv>(ctx, m, t, n_pat, tips, stride, weights, blen, pl, work, subs, start, rows, idx, \
+                               d_lnl, d_base_lnl)
+      case 1: PHY_INCR(double, 1);
+      case 2: PHY_INCR(double, 2);
+      case 4: PHY_INCR(double, 4);
+      case 8: PHY_INCR(double, 8);
+      case 11: PHY_INCR(float, 1);
+      case 12: PHY_INCR(float, 2);
+      case 14: PHY_INCR(float, 4);
+      case 18: PHY_INCR(float, 8);
+#undef PHY_INCR
+      default: return set_err(ctx, PHY_ERANGE, "spr incremental: n_rates must be 1, 2, 4 or 8");
+    }
+  } catch (...) {
+    return set_err(ctx, PHY_ENOMEM, "spr incremental: host allocation or thread start failed");
   }
 }
 

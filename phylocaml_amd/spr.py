@@ -68,6 +68,48 @@ class Scorer:
         return out[: len(mv)]
 
 
+class IncrementalScorer:
+    """Exact lnL of SPR neighbours without a tree per move
+    (phy_lk_spr_scores_incr): per pruned subtree, a repair of the base
+    tree's partials and one pre-order pass score every regraft.  Same
+    values as Scorer up to rounding; the workspace grows with the tree's
+    longest repair path."""
+
+    def __init__(self, ctx: D.Context, dm, n_leaves: int, tips, stride: int, weights,
+                 n_patterns: int, form: int = 0, s_batch: int = 16):
+        torch = __import__("torch")
+        self.ctx, self.dm, self.N = ctx, dm, int(n_leaves)
+        self.tips, self.stride, self.weights = tips, int(stride), weights
+        self.n_patterns, self.form, self.s_batch = int(n_patterns), int(form), int(s_batch)
+        self.work = torch.empty((0,), dtype=torch.uint8, device=f"cuda:{ctx.device}")
+        self.base_lnl = torch.zeros((1,), dtype=torch.float64, device=f"cuda:{ctx.device}")
+
+    def work_size(self, children, root: int) -> int:
+        ch = np.ascontiguousarray(children, np.int32)
+        nb = C.c_size_t()
+        check(lib.phy_lk_spr_incr_work_size(self.N, ptr(ch, C.c_int32), int(root), self.dm.handle,
+                                            self.n_patterns, self.form, self.s_batch, C.byref(nb)))
+        return nb.value
+
+    def score(self, children, root: int, blen, mv: np.ndarray):
+        """Device float64 tensor of lnL for each move of mv [n, 2]; the base
+        tree's lnL lands in self.base_lnl."""
+        torch = __import__("torch")
+        mv = np.ascontiguousarray(mv, np.int32)
+        ch = np.ascontiguousarray(children, np.int32)
+        bl = np.ascontiguousarray(blen, np.float64)
+        need = self.work_size(ch, root)
+        if self.work.numel() < need:
+            self.work = torch.empty((need,), dtype=torch.uint8, device=f"cuda:{self.ctx.device}")
+        out = torch.empty((max(len(mv), 1),), dtype=torch.float64, device=f"cuda:{self.ctx.device}")
+        self.ctx.check(lib.phy_lk_spr_scores_incr(
+            self.ctx.handle, self.dm.handle, self.form, self.n_patterns, D.dptr(self.tips),
+            self.stride, D.dptr(self.weights), self.N, ptr(ch, C.c_int32), int(root),
+            ptr(bl, C.c_double), len(mv), ptr(mv, C.c_int32), self.s_batch, D.dptr(out),
+            D.dptr(self.base_lnl), D.dptr(self.work), self.work.numel()), "spr_scores_incr")
+        return out[: len(mv)]
+
+
 # ---------------------------------------------------------------------------
 # Move features
 # ---------------------------------------------------------------------------
