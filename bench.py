@@ -123,6 +123,19 @@ class Clock:
         return step_ms, k_ms, n
 
 
+def kernel_totals(ctx, labels):
+    """{label: total ms} of the kernels timed since phy_ctx_kernel_timer(1)."""
+    from phylocaml_amd._lib import lib
+
+    out = {}
+    for label in labels:
+        tot, cnt = C.c_double(), C.c_int64()
+        ctx.check(lib.phy_ctx_kernel_time(ctx.handle, label.encode(), C.byref(tot), C.byref(cnt)),
+                  "kernel_time")
+        out[label] = round(tot.value, 3)
+    return out
+
+
 def stream_copy(torch, ctx, clock, gib=2):
     """The project's own STREAM-style copy kernel on `gib` GiB."""
     from phylocaml_amd import device as D
@@ -363,6 +376,7 @@ def bench_spr(torch, ctx, cpu_baseline: bool):
     even sample of 4,096 of the round's moves, FP64 and FP32-scaled; beside
     it the FP64 oracle on 8 of the moved trees, which must agree."""
     from phylocaml_amd import device as D, spr, spr_search as SS
+    from phylocaml_amd._lib import lib
 
     inst = SS.make_instance(7, 256, 50_000, 10)
     c = SS.Context(ctx, inst)
@@ -370,10 +384,12 @@ def bench_spr(torch, ctx, cpu_baseline: bool):
     sample = mv[:: max(1, len(mv) // 4096)][:4096]
     rates, scores = {}, {}
     for name, fp64 in (("f64", True), ("f32", False)):
-        c.exact(inst.children, inst.root, inst.blen, sample[:256], fp64=fp64)  # warm-up
+        c.exact(inst.children, inst.root, inst.blen, sample[:256], fp64=fp64,
+                incremental=False)  # warm-up
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        scores[name] = c.exact(inst.children, inst.root, inst.blen, sample, fp64=fp64)
+        scores[name] = c.exact(inst.children, inst.root, inst.blen, sample, fp64=fp64,
+                               incremental=False)
         torch.cuda.synchronize()
         rates[name] = len(sample) / (time.perf_counter() - t0)
     res = {"workload": "256-taxon Yule tree, 50,000 unique GTR+G4 patterns; exact lnL of "
@@ -381,6 +397,37 @@ def bench_spr(torch, ctx, cpu_baseline: bool):
            "moves_per_round": int(len(mv)), "exact_evaluations_per_s": rates}
     log(f"  spr: {len(mv)} moves; exact evaluations/s f64 {rates['f64']:.0f}, "
         f"f32 {rates['f32']:.0f}")
+    # the whole round through the incremental scorer (repair + insertion
+    # pass per pruned subtree), checked against the per-candidate scores
+    idx = np.arange(0, len(mv), max(1, len(mv) // 4096))[:4096]
+    inc = {}
+    for name, form in (("f64", 0), ("f32", 1)):
+        sc = spr.IncrementalScorer(ctx, c.dm, inst.N, c.tips, c.stride, c.wd, inst.P, form=form,
+                                   s_batch=32)
+        sc.score(inst.children, inst.root, inst.blen, mv[:2048])  # warm-up, workspace
+        torch.cuda.synchronize()
+        lib.phy_ctx_kernel_timer(ctx.handle, 0)
+        t0 = time.perf_counter()
+        got = sc.score(inst.children, inst.root, inst.blen, mv)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        # the same round again with per-launch event pairs, for the split
+        lib.phy_ctx_kernel_timer(ctx.handle, 1)
+        sc.score(inst.children, inst.root, inst.blen, mv)
+        torch.cuda.synchronize()
+        spans = kernel_totals(ctx, ("spr_insert", "spr_repair", "prune_down"))
+        lib.phy_ctx_kernel_timer(ctx.handle, 0)
+        got = got.cpu().numpy()
+        diff = float(np.max(np.abs(got[idx] - scores["f64"]) / np.abs(scores["f64"])))
+        inc[name] = {"round_s": dt, "moves_per_s": len(mv) / dt,
+                     "max_rel_diff_vs_per_candidate_f64": diff,
+                     "kernel_ms": spans,
+                     "workspace_gib": sc.work.numel() / 2**30}
+        log(f"  spr incremental {name}: whole round {len(mv)} moves in {dt * 1e3:.1f} ms = "
+            f"{len(mv) / dt:.0f} moves/s (x{len(mv) / dt / rates[name]:.0f} per-candidate), "
+            f"max rel diff {diff:.1e}; kernels {inc[name]['kernel_ms']}")
+        del sc
+    res["incremental"] = inc
     if cpu_baseline:
         from oracle import oracle as O
 

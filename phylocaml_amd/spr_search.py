@@ -80,6 +80,10 @@ class Context:
                                   form=0, chunk=chunk)
         self.exact32 = spr.Scorer(ctx, self.dm, inst.N, self.tips, self.stride, self.wd, inst.P,
                                   form=1, chunk=chunk)
+        self.inc64 = spr.IncrementalScorer(ctx, self.dm, inst.N, self.tips, self.stride, self.wd,
+                                           inst.P, form=0)
+        self.inc32 = spr.IncrementalScorer(ctx, self.dm, inst.N, self.tips, self.stride, self.wd,
+                                           inst.P, form=1)
         self.n_exact = 0
 
     def lnl(self, ch, r, bl, form=F64_FUSED) -> float:
@@ -89,8 +93,18 @@ class Context:
         self.n_exact += 1
         return float(ev.lnl.cpu()[0])
 
-    def exact(self, ch, r, bl, mv, fp64: bool = True) -> np.ndarray:
-        sc = self.exact64 if fp64 else self.exact32
+    def exact(self, ch, r, bl, mv, fp64: bool = True, incremental: bool | None = None) -> np.ndarray:
+        """Exact lnL of the moves mv [n, 2].  The incremental scorer costs
+        about two passes per distinct pruned subtree, the per-candidate
+        scorer one pass per move: by default the incremental one runs when
+        the moves average at least 4 per pruned subtree."""
+        mv = np.asarray(mv, np.int32).reshape(-1, 2)
+        if incremental is None:
+            incremental = len(mv) >= 4 * len(np.unique(mv[:, 0]))
+        if incremental:
+            sc = self.inc64 if fp64 else self.inc32
+        else:
+            sc = self.exact64 if fp64 else self.exact32
         self.n_exact += len(mv)
         return sc.score(ch, r, bl, mv).cpu().numpy()
 
