@@ -472,6 +472,57 @@ def bench_search(ctx):
     return out
 
 
+def bench_newton(torch):
+    """Branch-length Newton on a 64-taxon / 2,000-pattern GTR+G4 case, where
+    launches and host round trips weigh: time per iteration of the host loop
+    (optimize_branches: derivative pass, three copies to the host, host
+    step), of the device loop issued eagerly (phy_newton_propose, phy_lk_tree,
+    phy_newton_accept), and of the same iterations replayed from a HIP graph."""
+    from phylocaml_amd import device as D, likelihood as LK, mlmodel as MM, optimize as OPT, synth
+
+    s = torch.cuda.Stream()
+    ctx = D.Context(0, stream=s.cuda_stream)
+    N, P, K = 64, 2000, 20
+    with torch.cuda.stream(s):
+        tree = synth.yule(N, 4, 0.08)
+        pi, co = synth.gtr_params(4, 5.0, 4)
+        model = MM.create(MM.Spec(4, ("GTR", co), ("DiscreteGamma", 4, 0.5), ("Empirical", pi)))
+        st = synth.simulate(tree, model.priors, model.q, model.rates, P, 4)
+        topo = D.Topology(N, tree.children, tree.root, ctx)
+        dm = LK.DeviceModel(ctx, model)
+        tips, wd = D.to_device(synth.pack_dna(st)), D.to_device(synth.poisson_weights(P, 3.0, 4))
+        start = np.full(2 * N - 1, 0.1)
+        start[tree.root] = 0.0
+        obj = OPT.DeviceObjective(ctx, topo, tips, D.words(P), wd, P)
+        OPT.optimize_branches(obj, model, start, tree.children, N, tree.root, max_iter=2)
+        t0 = time.perf_counter()
+        host = OPT.optimize_branches(obj, model, start, tree.children, N, tree.root, max_iter=K)
+        host_ms = (time.perf_counter() - t0) * 1e3 / max(host.derivative_passes, 1)
+        dn = OPT.DeviceNewton(ctx, topo, dm, tips, D.words(P), wd, P, start)
+        dn.run(2)
+        ctx.sync()
+        t0 = time.perf_counter()
+        dn.run(K)
+        ctx.sync()
+        eager_ms = (time.perf_counter() - t0) * 1e3 / K
+        dg = OPT.DeviceNewton(ctx, topo, dm, tips, D.words(P), wd, P, start)
+        g = dg.capture(K)
+        g.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.replay()
+        torch.cuda.synchronize()
+        graph_ms = (time.perf_counter() - t0) * 1e3 / K
+        _, lg, _, _ = dg.result()
+    out = {"workload": f"{N}-taxon Yule tree, {P:,} GTR+G4 patterns, FP64, every branch from 0.1",
+           "host_loop_ms_per_pass": host_ms, "device_eager_ms_per_iteration": eager_ms,
+           "graph_ms_per_iteration": graph_ms, "host_final_lnl": host.lnl,
+           "graph_lnl_after_two_replays": lg}
+    log(f"  newton: host loop {host_ms:.3f} ms/pass, device eager {eager_ms:.3f} ms/iteration, "
+        f"graph replay {graph_ms:.3f} ms/iteration")
+    return out
+
+
 def bench_mlp(torch, ctx, clock):
     """MLP 32 -> 128 -> 128 -> 1 on 65,536 moves: forward and backward
     kernels (BF16 MFMA GEMMs, FP32 accumulation)."""
@@ -578,7 +629,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["all", "prune", "fitch", "aa", "spr", "search", "mlp",
-                                       "copy"], default="all",
+                                       "copy", "newton"], default="all",
                     help="one part only (the pruning line is always measured unless the part "
                          "is copy or mlp)")
     ap.add_argument("--aa-trees", type=int, default=2048)
@@ -614,6 +665,7 @@ def main():
     spr_res = bench_spr(torch, ctx, cpu_on) if args.only in ("all", "spr") else None
     search = bench_search(ctx) if args.only in ("all", "search") else None
     mlp = bench_mlp(torch, ctx, clock) if everything else None
+    newton = bench_newton(torch) if args.only in ("all", "newton") else None
     config0 = bench_config0(ctx) if everything and cpu_on else None
 
     cpu = None
@@ -703,6 +755,7 @@ def main():
         "spr": spr_res,
         "search": search,
         "mlp": mlp,
+        "newton": newton,
         "config0": config0,
     }
     print(json.dumps(line), flush=True)

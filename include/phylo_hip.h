@@ -444,6 +444,28 @@ int phy_lk_adjust3(phy_ctx* ctx, const phy_model* model, int32_t form,
                    int32_t max_iter, double tol, double* t_out, double* lnl_out,
                    int32_t* n_evals);
 
+/* Branch-length Newton-Raphson with its step control on the device, for the
+ * branch-length hooks the reference declares (MlModel.num_parameters /
+ * update, lib/mlModel.ml:193-196; unimplemented there).  An iteration is
+ * phy_newton_propose, phy_lk_tree with derivatives at the proposal, then
+ * phy_newton_accept: kernels only (no host copy, no synchronisation), so
+ * iterations can be captured in a HIP graph.  d_state: 4 doubles, [lnL at
+ * d_blen, 1 = next proposal is a Newton step / 0 = halve the last proposal
+ * towards d_blen, accepted steps, halvings]; phy_newton_init sets it from
+ * the lnL at the starting lengths.  Propose: every branch takes -d1/d2 where
+ * d2 < 0, else moves half its length along the gradient, clamped to
+ * [1e-8, 10]; the two root branches take one step on their sum and keep
+ * their ratio.  Accept: when lnL at the proposal is not lower, d_blen, d_d1,
+ * d_d2 and the state's lnL take the proposal's values. */
+int phy_newton_init(phy_ctx* ctx, double* d_state, const double* d_lnl);
+int phy_newton_propose(phy_ctx* ctx, const phy_topo* topo, const double* d_state,
+                       const double* d_blen, const double* d_d1, const double* d_d2,
+                       double* d_blen_new);
+int phy_newton_accept(phy_ctx* ctx, const phy_topo* topo, double* d_state,
+                      const double* d_lnl_new, double* d_blen, const double* d_blen_new,
+                      double* d_d1, const double* d_d1_new, double* d_d2,
+                      const double* d_d2_new);
+
 
 /* ------------------------------------------------------------------------- */
 /* SPR neighbourhood (TreeNeighborhoods.generate_spr, lib/treeNeighborhoods. */

@@ -93,6 +93,9 @@ SIGNATURES = {
     "phy_lk_edge": (C.c_int, [vp, vp, i32, i64, vp, vp, vp, vp, i64, f64, vp, vp]),
     "phy_lk_adjust3": (C.c_int, [vp, vp, i32, i64, vp, vp, vp, vp, i64, vp, f64, f64, f64, i32,
                                  f64, P_f64, P_f64, P_i32]),
+    "phy_newton_init": (C.c_int, [vp, vp, vp]),
+    "phy_newton_propose": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    "phy_newton_accept": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "phy_spr_moves": (C.c_int, [i32, P_i32, i32, P_i64, P_i32, i64]),
     "phy_spr_apply": (C.c_int, [i32, P_i32, i32, P_f64, i32, i32, P_i32, P_i32, P_f64]),
     "phy_tbr_moves": (C.c_int, [i32, P_i32, i32, i32, i32, P_i64, P_i32, i64]),

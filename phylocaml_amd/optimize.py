@@ -83,6 +83,77 @@ class DeviceObjective:
         return float(ev.lnl.cpu()[0]), ev.d1.cpu().numpy().copy(), ev.d2.cpu().numpy().copy()
 
 
+class DeviceNewton:
+    """The Newton iteration of ``optimize_branches`` with its step control on
+    the device (phy_newton_propose, phy_lk_tree with derivatives at the
+    proposal, phy_newton_accept): an iteration issues kernels only, so
+    ``capture(k)`` records k iterations in one HIP graph.  A proposal that
+    lowers lnL is halved towards the current lengths on the next iteration,
+    as ``optimize_branches`` does; there is no stopping test, so iterations
+    past convergence leave the lengths in place.  Capture needs ``ctx`` on
+    a side stream (not the default stream)."""
+
+    def __init__(self, ctx, topo, dm, tips, stride: int, weights, n_patterns: int, blen):
+        from . import device as D
+        from . import likelihood as LK
+        from ._lib import lib
+
+        self.ctx, self.topo, self._D, self._lib = ctx, topo, D, lib
+        self.ev = LK.Evaluator(ctx, topo, dm, n_patterns, LK.PHY_F64, derivs=True)
+        self.inputs = (tips, int(stride), weights)
+        t = np.asarray(blen, np.float64).copy()
+        t[topo.root] = 0.0
+        self.t = D.to_device(t, ctx.device)
+        self.tn = D.to_device(t, ctx.device)
+        self.d1 = D.to_device(np.zeros_like(t), ctx.device)
+        self.d2 = D.to_device(np.zeros_like(t), ctx.device)
+        self.state = D.to_device(np.array([-np.inf, 1.0, 0.0, 0.0]), ctx.device)
+        # derivatives at the start: accepted as a proposal over lnL = -inf,
+        # then the counters reset
+        self.ev.run(*self.inputs, self.t)
+        self._accept()
+        ctx.check(lib.phy_newton_init(ctx.handle, D.dptr(self.state), D.dptr(self.ev.lnl)),
+                  "newton_init")
+
+    def _accept(self):
+        D = self._D
+        self.ctx.check(self._lib.phy_newton_accept(
+            self.ctx.handle, self.topo.handle, D.dptr(self.state), D.dptr(self.ev.lnl),
+            D.dptr(self.t), D.dptr(self.tn), D.dptr(self.d1), D.dptr(self.ev.d1), D.dptr(self.d2),
+            D.dptr(self.ev.d2)), "newton_accept")
+
+    def step(self) -> None:
+        """Enqueue one iteration (propose, derivative pass, accept)."""
+        D = self._D
+        self.ctx.check(self._lib.phy_newton_propose(
+            self.ctx.handle, self.topo.handle, D.dptr(self.state), D.dptr(self.t), D.dptr(self.d1),
+            D.dptr(self.d2), D.dptr(self.tn)), "newton_propose")
+        self.ev.run(*self.inputs, self.tn)
+        self._accept()
+
+    def run(self, k: int) -> None:
+        for _ in range(int(k)):
+            self.step()
+
+    def capture(self, k: int):
+        """A torch.cuda.CUDAGraph of k iterations on the context's stream;
+        each replay advances the state by k iterations."""
+        torch = __import__("torch")
+        if not self.ctx.stream:
+            raise ValueError("graph capture needs the context on a side stream")
+        s = torch.cuda.ExternalStream(self.ctx.stream, device=f"cuda:{self.ctx.device}")
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            self.run(k)
+        return g
+
+    def result(self):
+        """(lengths, lnL, accepted steps, halvings) after a synchronisation."""
+        self.ctx.sync()
+        st = self.state.cpu().numpy()
+        return self.t.cpu().numpy(), float(st[0]), int(st[2]), int(st[3])
+
+
 def _root_children(children: np.ndarray, n_leaves: int, root: int):
     k = 2 * (root - n_leaves)
     return int(children[k]), int(children[k + 1])

@@ -118,6 +118,13 @@ def test_spr_round_256_taxa_50k_patterns(ctx):
     rel32 = np.abs(s32 - s64) / np.abs(s64)
     _say(f"FP32 vs FP64 over every move: max relative {rel32.max():.2e}")
     assert rel32.max() <= 1e-6
+    # the whole round went through the incremental scorer (moves per pruned
+    # subtree >> 4); 1,024 of its moves through the per-candidate scorer
+    sample = np.arange(0, len(mv), len(mv) // 1024)[:1024]
+    p64 = c.exact(inst.children, inst.root, inst.blen, mv[sample], fp64=True, incremental=False)
+    rel = np.abs(p64 - s64[sample]) / np.abs(p64)
+    _say(f"incremental vs per-candidate FP64 on 1,024 moves: max relative {rel.max():.2e}")
+    assert rel.max() <= 1e-10
     # sampled moves against the oracle (the best move, the worst, 14 random)
     rng = np.random.default_rng(1)
     picks = [int(np.argmax(s64)), int(np.argmin(s64))] + \
