@@ -547,6 +547,25 @@ int phy_lk_spr_scores_incr(phy_ctx* ctx, const phy_model* model, int32_t form,
                            int32_t root, const double* blen, int64_t n_moves,
                            const int32_t* moves, int32_t s_batch, double* d_lnl,
                            double* d_base_lnl, void* d_work, size_t work_bytes);
+/* The same scoring prepared once and run many times: the plan holds the
+ * moves grouped by pruned subtree and every pruned tree's programs on the
+ * device (the library allocates them); a run takes device branch lengths
+ * and a model with the plan's number of rate classes, and issues kernels
+ * only (no host work, copy or synchronisation), so it can be captured in a
+ * HIP graph and replayed after the lengths or the model change.  d_lnl
+ * receives one lnL per move of the plan, in the order given; d_base_lnl
+ * (nullable) the base tree's. */
+typedef struct phy_spr_plan phy_spr_plan;
+int phy_spr_plan_create(phy_ctx* ctx, const phy_model* model, int32_t form, int64_t n_patterns,
+                        int32_t n_leaves, const int32_t* children, int32_t root,
+                        int64_t n_moves, const int32_t* moves, int32_t s_batch,
+                        phy_spr_plan** out);
+int phy_spr_plan_work_size(const phy_spr_plan* plan, size_t* bytes);
+int phy_spr_plan_run(phy_ctx* ctx, const phy_spr_plan* plan, const phy_model* model,
+                     const uint64_t* d_tips, int64_t stride, const uint32_t* d_weights,
+                     const double* d_blen, double* d_lnl, double* d_base_lnl, void* d_work,
+                     size_t work_bytes);
+void phy_spr_plan_destroy(phy_spr_plan* plan);
 /* The same for TBR moves (s, w, v), 3 ints each (workspace as for SPR). */
 int phy_lk_tbr_scores(phy_ctx* ctx, const phy_model* model, int32_t form,
                       int64_t n_patterns, const uint64_t* d_tips, int64_t stride,

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <climits>
 #include <atomic>
+#include <memory>
 #include <cmath>
 #include <mutex>
 #include <thread>
@@ -1350,6 +1351,217 @@ __global__ __launch_bounds__(256) void k_spr_insert(
   });
 }
 
+// The insertion pass with one rate class per wave: a block is R waves over
+// the same 64 patterns (lane = pattern), so a wave's P matrices are
+// wave-uniform (scalar loads) and partial rows are read whole.  P matrices
+// instead of eigenbasis factors: Pf (full branches), Ph (half branches), Px
+// (the merged branch of x, per subtree).  The classes of a target meet per
+// entry in LDS -- exponents aligned over the non-zero terms, the FP64 rate
+// sum -- where wave 0 finishes target a and wave 1 target b (wave 0 both
+// when R = 1); double-buffered by entry parity, one barrier per entry.
+// wsum[(k * M + c) * n_blk + block]: per target and block of 64 patterns.
+template <typename real, int R>
+__global__ __launch_bounds__(64 * R) void k_spr_insert_w(
+    const int4* __restrict__ prog, int64_t prog_stride, const int32_t* __restrict__ meta,
+    const int32_t* __restrict__ ref, const real* __restrict__ Pf, const real* __restrict__ Ph,
+    const real* __restrict__ Px, const double* __restrict__ pi, const double* __restrict__ probs,
+    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
+    const real* __restrict__ part, const int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
+    double* __restrict__ wsum, int64_t n_blk, int N, int M, int up_depth) {
+  constexpr int BD = 64 * R;
+  extern __shared__ unsigned char lds_raw[];
+  const int k = blockIdx.y;
+  const int32_t* mk = meta + kIncrMeta * k;
+  const int n_ops = mk[4];
+  if (n_ops == 0) return;  // the whole block: before any barrier
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int r = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int64_t p = (int64_t)blockIdx.x * 64 + lane;
+  const bool act = p < n_pat;
+  const int64_t pc = act ? p : 0;
+  real* stk = reinterpret_cast<real*>(lds_raw);                                  // [depth][4][BD]
+  int32_t* stke = reinterpret_cast<int32_t*>(stk + (size_t)up_depth * 4 * BD);  // [depth][BD]
+  double* xv = reinterpret_cast<double*>(stke + (size_t)up_depth * BD);         // [2][2][R][64]
+  int32_t* xe = reinterpret_cast<int32_t*>(xv + 2 * 2 * R * 64);                // [2][2][R][64]
+  const int32_t* rf = ref + (size_t)k * M;
+  const int x = mk[2], s = mk[3];
+  const int4* pg = prog + (size_t)k * prog_stride;
+  double* ws = wsum + (size_t)k * M * n_blk;
+  const real* pxk = Px + ((size_t)k * R + r) * 16;
+  auto pfull = [&](int c) { return c == x ? pxk : Pf + ((size_t)c * R + r) * 16; };
+
+  // X = P(t_s) D_s: the subtree of s and its branch are untouched
+  real X[4];
+  int eX = 0;
+  {
+    real D[4];
+    if (s < N) {
+      tipvec(tip_code(tips, stride, s, pc), D);
+    } else {
+      const real* sv = part + (((size_t)(s - N) * R + r) * 4) * pitch + pc;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) D[j] = sv[(size_t)j * pitch];
+      eX = ex[((size_t)(s - N) * R + r) * pitch + pc];
+    }
+    real Ps[16];
+    load_mat<real, R>(Pf + ((size_t)s * R + r) * 16, Ps);
+    matvec4(Ps, D, X);
+  }
+  auto fetch = [&](int c, real* Dv, int* e) {
+    if (c < N) {
+      tipvec(tip_code(tips, stride, c, pc), Dv);
+      *e = 0;
+    } else {
+      const int row = rf[c];
+      const real* sv = part + (((size_t)row * R + r) * 4) * pitch + pc;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Dv[j] = sv[(size_t)j * pitch];
+      *e = ex[((size_t)row * R + r) * pitch + pc];
+    }
+  };
+  // the class's insertion term at the middle of the branch above a child
+  // with partial D, under the message A arriving at it
+  auto insert = [&](const real* Phc, const real* A, const real* D) {
+    real left[4], right[4];
+    vecmat4(Phc, A, left);
+    matvec4(Phc, D, right);
+    real sv = (real)0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sv = fma(left[j] * right[j], X[j], sv);
+    return sv;
+  };
+
+  real Dc[8];
+  int Xc[2];
+  {
+    const int4 o = pg[0];
+    fetch(o.y, Dc, &Xc[0]);
+    fetch(o.z, Dc + 4, &Xc[1]);
+  }
+  real accM[4] = {(real)0, (real)0, (real)0, (real)0};
+  int accE = 0;
+  int sp = 0;
+  for_each_entry_ahead(pg, n_ops, [&](int i, const int4 op, const int4 on) {
+    real Dn[8];
+    int Xn[2] = {0, 0};
+    const bool more = i + 1 < n_ops;
+    if (more) {
+      fetch(on.y, Dn, &Xn[0]);
+      fetch(on.z, Dn + 4, &Xn[1]);
+    }
+    const int fl = op.w, src = fl & 3;
+    const int a = op.y, b = op.z;
+    real Mv[4];
+    int eM;
+    if (src == SRC_LEAF) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Mv[j] = (real)pi[j];
+      eM = 0;
+    } else if (src == SRC_ACC) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Mv[j] = accM[j];
+      eM = accE;
+    } else {
+      --sp;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Mv[j] = stk[((size_t)sp * 4 + j) * BD + tid];
+      eM = stke[(size_t)sp * BD + tid];
+    }
+    real PA[16], PB[16];
+    load_mat<real, R>(pfull(a), PA);
+    load_mat<real, R>(pfull(b), PB);
+    real ya[4], yb[4], Aa[4], Ab[4];
+    matvec4(PA, Dc, ya);
+    matvec4(PB, Dc + 4, yb);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      Aa[j] = Mv[j] * yb[j];
+      Ab[j] = Mv[j] * ya[j];
+    }
+    int eAa = eM + Xc[1], eAb = eM + Xc[0];
+    rescale4(Aa, &eAa);
+    rescale4(Ab, &eAb);
+    real sa, sb;
+    {
+      real PH[16];
+      load_mat<real, R>(Ph + ((size_t)a * R + r) * 16, PH);
+      sa = insert(PH, Aa, Dc);
+      load_mat<real, R>(Ph + ((size_t)b * R + r) * 16, PH);
+      sb = insert(PH, Ab, Dc + 4);
+    }
+    // exchange; wave 0 finishes target a, wave 1 (wave 0 when R = 1) b
+    {
+      const int buf = i & 1;
+      double* xb0 = xv + (size_t)buf * 2 * R * 64;
+      int32_t* eb0 = xe + (size_t)buf * 2 * R * 64;
+      xb0[r * 64 + lane] = (double)sa;
+      eb0[r * 64 + lane] = eAa + Xc[0] + eX;
+      xb0[(R + r) * 64 + lane] = (double)sb;
+      eb0[(R + r) * 64 + lane] = eAb + Xc[1] + eX;
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (r != (R == 1 ? 0 : q)) continue;
+        const double* sq = xb0 + (size_t)q * R * 64;
+        const int32_t* eq = eb0 + (size_t)q * R * 64;
+        int m = kNoExp;
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr)
+          if (sq[rr * 64 + lane] != 0.0) m = max(m, eq[rr * 64 + lane]);
+        // the class sum and its log in the working precision (single: the
+        // per-pattern term to ~1e-7, far inside the path's 1e-6 on lnL);
+        // the exponent offset and the weighted sums in FP64
+        real L = (real)0;
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+          const real v = (real)sq[rr * 64 + lane];
+          if (v != (real)0) L += (real)probs[rr] * ldexp(v, eq[rr * 64 + lane] - m);
+        }
+        const double v = act ? (double)weights[p] * ((double)log(L) + (double)m * 0.6931471805599453) : 0.0;
+        const double S = wave_sum_d(v);
+        if (lane == 0) ws[(size_t)(q ? b : a) * n_blk + blockIdx.x] = S;
+      }
+    }
+    // messages to internal children: M_c = P_c^T A_c
+    real Ma[4], Mb[4];
+    int eMa = eAa, eMb = eAb;
+    if (!(fl & UP_A_LEAF)) {
+      vecmat4(PA, Aa, Ma);
+      rescale4(Ma, &eMa);
+    }
+    if (!(fl & UP_B_LEAF)) {
+      vecmat4(PB, Ab, Mb);
+      rescale4(Mb, &eMb);
+    }
+    if (fl & UP_PUSH) {
+      const bool cb = fl & UP_CONT_B;
+      const real* o = cb ? Ma : Mb;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) stk[((size_t)sp * 4 + j) * BD + tid] = o[j];
+      stke[(size_t)sp * BD + tid] = cb ? eMa : eMb;
+      ++sp;
+    }
+    if (!(fl & UP_A_LEAF) || !(fl & UP_B_LEAF)) {
+      const bool cb = fl & UP_CONT_B;
+      const real* c = cb ? Mb : Ma;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) accM[j] = c[j];
+      accE = cb ? eMb : eMa;
+    }
+    if (more) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Dc[j] = Dn[j];
+      Xc[0] = Xn[0];
+      Xc[1] = Xn[1];
+    }
+  });
+}
+
+// LDS of k_spr_insert_w: the message stack and the double-buffered exchange
+size_t insert_w_lds(int R, size_t esz, int depth) {
+  return (size_t)std::max(depth, 1) * 64 * R * (4 * esz + 4) + (size_t)2 * 2 * R * 64 * 12;
+}
+
 // out[idx[j]] = compensated sum of row rows[j] of x (n values per row); one
 // block per j
 __global__ void k_reduce_gather(const double* __restrict__ x, int64_t n, const int64_t* __restrict__ rows,
@@ -2210,67 +2422,65 @@ struct IncrTree {
   }
 };
 
+// Workspace of a run: lengths and factors, the base tree's P, the partial
+// pool, the per-target sums.  Per-subtree factors are sized for every
+// pruned subtree a tree can have (M), so the layout depends on the tree
+// only through its longest repair path.
 struct IncrPlan {
   Plan base;  // stored post-order pass of the base tree
-  size_t off_bl = 0, off_E = 0, off_Eh = 0, off_Ex = 0, off_tab = 0, off_dprog = 0, off_dfifo = 0;
-  size_t off_P = 0, off_part = 0, off_ex = 0, off_bsum = 0, off_lnl = 0;
-  size_t off_prog = 0, off_meta = 0, off_ref = 0, off_ops = 0, off_wsum = 0, off_rows = 0, total = 0;
-  int64_t pitch = 0, n_waves = 0, pool_rows = 0, max_moves = 0;
+  size_t off_bl = 0, off_E = 0, off_Eh = 0, off_Ex = 0, off_tab = 0, off_P = 0, off_Ph = 0, off_Px = 0;
+  size_t off_part = 0, off_ex = 0, off_bsum = 0, off_lnl = 0, off_wsum = 0, off_blh = 0, total = 0;
+  int64_t pitch = 0, n_waves = 0, n_blk = 0, pool_rows = 0;
   int S = 0, lmax = 0;
   size_t esz = 8;
 };
 
-IncrPlan incr_plan(const IncrTree& t, const phy_model* m, int64_t n_pat, int32_t form, int32_t S) {
+IncrPlan incr_plan(const IncrTree& t, int R, int64_t n_pat, int32_t form, int32_t S) {
   IncrPlan pl;
-  const int N = t.N, M = t.M, R = m->R, K = 4;
+  const int N = t.N, M = t.M, K = 4;
   pl.esz = (form & PHY_F32) ? 4 : 8;
   pl.S = S;
   pl.lmax = std::max(t.lmax, 1);
   pl.pitch = ((n_pat + 255) / 256) * 256;
   pl.n_waves = pl.pitch * R / 64;
+  pl.n_blk = pl.pitch / 64;
   pl.pool_rows = (int64_t)(N - 1) + (int64_t)S * pl.lmax;
-  // every move of a batch has a distinct (subtree, target) pair
-  pl.max_moves = (int64_t)S * M;
   const size_t esz = pl.esz;
   size_t o = 0;
-  pl.off_bl = o; o = align256(o + (size_t)(2 * M + S) * 8);
+  pl.off_bl = o; o = align256(o + (size_t)3 * M * 8);  // full | half | merged x of each subtree
+  pl.off_blh = o; o = align256(o + (size_t)M * 8);     // host lengths of a one-shot call
   pl.off_E = o; o = align256(o + (size_t)M * R * K * esz);
   pl.off_Eh = o; o = align256(o + (size_t)M * R * K * esz);
-  pl.off_Ex = o; o = align256(o + (size_t)S * R * K * esz);
+  pl.off_Ex = o; o = align256(o + (size_t)M * R * K * esz);
   pl.off_tab = o; o = align256(o + (size_t)(2 * K * K + R * K) * esz);
-  pl.off_dprog = o; o = align256(o + (size_t)(N - 1) * 16);
-  pl.off_dfifo = o; o = align256(o + phy_fifo_padded(N) * 4);
   pl.off_P = o; o = align256(o + (size_t)M * R * K * K * esz);
+  pl.off_Ph = o; o = align256(o + (size_t)M * R * K * K * esz);
+  pl.off_Px = o; o = align256(o + (size_t)M * R * K * K * esz);
   pl.off_part = o; o = align256(o + (size_t)pl.pool_rows * R * K * pl.pitch * esz);
   pl.off_ex = o; o = align256(o + (size_t)pl.pool_rows * R * pl.pitch * 4);
   pl.off_bsum = o; o = align256(o + (size_t)(pl.pitch / 64) * 8);
   pl.off_lnl = o; o = align256(o + 8);
-  pl.off_prog = o; o = align256(o + (size_t)S * (N - 1) * 16);
-  pl.off_meta = o; o = align256(o + (size_t)S * kIncrMeta * 4);
-  pl.off_ref = o; o = align256(o + (size_t)S * M * 4);
-  pl.off_ops = o; o = align256(o + (size_t)S * pl.lmax * 16);
-  pl.off_wsum = o; o = align256(o + (size_t)S * M * pl.n_waves * 8);
-  pl.off_rows = o; o = align256(o + (size_t)pl.max_moves * 16);
+  pl.off_wsum = o; o = align256(o + (size_t)S * M * std::max(pl.n_waves, pl.n_blk) * 8);
   pl.total = o;
   return pl;
 }
 
 // Per pruned subtree: the pre-order program of the pruned tree on the
-// original node ids, pool rows, repair ops and the merged branch of x.
+// original node ids, pool rows, repair ops, and x and p (the merged branch
+// t_x + t_p; x = -1 when p is the root).
 struct IncrSub {
-  int32_t s = -1, x = -1, n_ops = 0, depth = 0;
-  double merged = 0.0;
+  int32_t s = -1, x = -1, p = -1, n_ops = 0, depth = 0;
   std::vector<int32_t> prog, ref, ops;  // ops: {y, a, b, flags} bottom-up
 };
 
-void incr_prepare(const IncrTree& t, const double* blen, int32_t s, int slot, int lmax, IncrSub* o) {
+void incr_prepare(const IncrTree& t, int32_t s, int slot, int lmax, IncrSub* o) {
   const int32_t N = t.N, M = t.M;
   const int32_t p = t.parent[s], x = t.sibling(s), g = t.parent[p];
   std::vector<int32_t> ch = t.ch, size = t.size;
   int32_t top = x;
   o->s = s;
   o->x = -1;
-  o->merged = 0.0;
+  o->p = p;
   o->ops.clear();
   o->ref.resize(M);
   for (int32_t c = 0; c < M; ++c) o->ref[c] = c >= N ? c - N : 0;
@@ -2278,7 +2488,6 @@ void incr_prepare(const IncrTree& t, const double* blen, int32_t s, int slot, in
     (ch[2 * (g - N)] == p ? ch[2 * (g - N)] : ch[2 * (g - N) + 1]) = x;
     top = t.root;
     o->x = x;
-    o->merged = blen[x] + blen[p];
     int32_t prev = -1, j = 0;
     for (int32_t y = g; y >= 0; y = t.parent[y], ++j) {
       size[y] -= t.size[s];
@@ -2292,158 +2501,6 @@ void incr_prepare(const IncrTree& t, const double* blen, int32_t s, int slot, in
   o->depth = 0;
   if (top >= N) phy_compile_up_program(N, ch, size, top, &o->prog, &o->depth);
   o->n_ops = (int32_t)o->prog.size() / 4;
-}
-
-template <typename real, int R>
-int incr_launch(phy_ctx* ctx, const phy_model* m, const IncrTree& t, int64_t n_pat, const uint64_t* tips,
-                int64_t stride, const uint32_t* weights, const double* blen, const IncrPlan& pl,
-                char* work, const std::vector<int32_t>& subs, const std::vector<int64_t>& mv_start,
-                const std::vector<int64_t>& mv_rows, const std::vector<int64_t>& mv_idx, double* d_lnl,
-                double* d_base) {
-  const int N = t.N, M = t.M, K = 4;
-  const double* cst = m->d_consts;
-  const double* d_pi = cst + 2 * K * K + K + R;
-  const double* d_probs = d_pi + K;
-  double* bl = reinterpret_cast<double*>(work + pl.off_bl);
-  real* E = reinterpret_cast<real*>(work + pl.off_E);
-  real* Eh = reinterpret_cast<real*>(work + pl.off_Eh);
-  real* Ex = reinterpret_cast<real*>(work + pl.off_Ex);
-  real* tab = reinterpret_cast<real*>(work + pl.off_tab);
-  real* part = reinterpret_cast<real*>(work + pl.off_part);
-  int32_t* ex = reinterpret_cast<int32_t*>(work + pl.off_ex);
-  double* wsum = reinterpret_cast<double*>(work + pl.off_wsum);
-  int64_t* rows = reinterpret_cast<int64_t*>(work + pl.off_rows);
-  int64_t* idx = rows + pl.max_moves;
-  std::vector<double> hb(2 * (size_t)M);
-  for (int c = 0; c < M; ++c) {
-    hb[c] = blen[c];
-    hb[M + c] = 0.5 * blen[c];
-  }
-  PHY_HIP(ctx, hipMemcpyAsync(bl, hb.data(), hb.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-  const int64_t ne = std::max<int64_t>((int64_t)M * R * K, 32 + 4 * R);
-  hipLaunchKernelGGL((k_eigen_exp<real>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, ctx->stream, cst,
-                     R, bl, M, E, tab);
-  PHY_LAUNCHED(ctx);
-  hipLaunchKernelGGL((k_eigen_exp<real>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, ctx->stream, cst,
-                     R, bl + M, M, Eh, tab);
-  PHY_LAUNCHED(ctx);
-
-  // the base tree's partials: stored post-order pass into pool rows 0 .. N-2
-  {
-    phy_topo* bt = nullptr;
-    int rc = phy_topo_create(nullptr, N, t.ch.data(), t.root, &bt);
-    if (rc) return set_err(ctx, rc, "spr incremental: base topology");
-    std::vector<int32_t> fifo(phy_fifo_padded(N), bt->down_fifo[0]);
-    std::copy(bt->down_fifo.begin(), bt->down_fifo.end(), fifo.begin());
-    hipError_t e = hipMemcpyAsync(work + pl.off_dprog, bt->down_prog.data(), bt->down_prog.size() * 4,
-                                  hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(work + pl.off_dfifo, fifo.data(), fifo.size() * 4, hipMemcpyHostToDevice, ctx->stream);
-    // the host sources (and hb) go out of scope: wait for the copies
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) {
-      phy_topo_destroy(bt);
-      return set_err(ctx, PHY_EDEVICE, std::string("spr incremental: ") + hipGetErrorString(e));
-    }
-    bt->d_down_prog = reinterpret_cast<int4*>(work + pl.off_dprog);
-    bt->d_down_fifo = reinterpret_cast<int32_t*>(work + pl.off_dfifo);
-    Plan bp = pl.base;
-    bp.off_P = pl.off_P;
-    bp.off_part = pl.off_part;
-    bp.off_ex = pl.off_ex;
-    bp.off_bsum = pl.off_bsum;
-    double* base_lnl = d_base ? d_base : reinterpret_cast<double*>(work + pl.off_lnl);
-    rc = launch_tree<real, R>(ctx, bt, m, false, false, n_pat, tips, stride, weights, bl, base_lnl, nullptr,
-                              nullptr, nullptr, work, bp);
-    bt->d_down_prog = nullptr;
-    bt->d_down_fifo = nullptr;
-    phy_topo_destroy(bt);
-    if (rc) return rc;
-  }
-
-  // per batch of pruned subtrees
-  const int n_sub = (int)subs.size();
-  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  std::vector<IncrSub> sub(pl.S);
-  std::vector<int32_t> hprog, hmeta, href, hops;
-  std::vector<double> hmerged;
-  const int dbs_want = env_block("PHY_SPR_INCR_BLOCK", 256);
-  for (int b0 = 0; b0 < n_sub; b0 += pl.S) {
-    const int S = std::min(pl.S, n_sub - b0);
-    {
-      std::vector<std::thread> th;
-      for (unsigned w = 0; w < std::min<unsigned>(nt, S); ++w)
-        th.emplace_back([&, w] {
-          for (int k = (int)w; k < S; k += (int)nt) incr_prepare(t, blen, subs[b0 + k], k, pl.lmax, &sub[k]);
-        });
-      for (auto& x : th) x.join();
-    }
-    hprog.assign((size_t)S * (N - 1) * 4, 0);
-    hmeta.assign((size_t)S * kIncrMeta, 0);
-    href.assign((size_t)S * M, 0);
-    hops.assign((size_t)S * pl.lmax * 4, 0);
-    hmerged.assign(S, 0.0);
-    int depth = 1;
-    for (int k = 0; k < S; ++k) {
-      const IncrSub& q = sub[k];
-      std::copy(q.prog.begin(), q.prog.end(), hprog.begin() + (size_t)k * (N - 1) * 4);
-      std::copy(q.ref.begin(), q.ref.end(), href.begin() + (size_t)k * M);
-      std::copy(q.ops.begin(), q.ops.end(), hops.begin() + (size_t)k * pl.lmax * 4);
-      int32_t* mk = &hmeta[(size_t)k * kIncrMeta];
-      mk[0] = k * pl.lmax;
-      mk[1] = (int32_t)q.ops.size() / 4;
-      mk[2] = q.x;
-      mk[3] = q.s;
-      mk[4] = q.n_ops;
-      mk[5] = q.depth;
-      hmerged[k] = q.merged;
-      depth = std::max(depth, q.depth);
-    }
-    const int64_t m0 = mv_start[b0], m1 = mv_start[b0 + S];
-    PHY_HIP(ctx, hipMemcpyAsync(work + pl.off_prog, hprog.data(), hprog.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    PHY_HIP(ctx, hipMemcpyAsync(work + pl.off_meta, hmeta.data(), hmeta.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    PHY_HIP(ctx, hipMemcpyAsync(work + pl.off_ref, href.data(), href.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    PHY_HIP(ctx, hipMemcpyAsync(work + pl.off_ops, hops.data(), hops.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    PHY_HIP(ctx, hipMemcpyAsync(bl + 2 * M, hmerged.data(), (size_t)S * 8, hipMemcpyHostToDevice, ctx->stream));
-    const int64_t nx = std::max<int64_t>((int64_t)S * R * K, 32 + 4 * R);
-    hipLaunchKernelGGL((k_eigen_exp<real>), dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, ctx->stream,
-                       cst, R, bl + 2 * M, S, Ex, tab);
-    PHY_LAUNCHED(ctx);
-    const auto* d_meta = reinterpret_cast<const int32_t*>(work + pl.off_meta);
-    const auto* d_ref = reinterpret_cast<const int32_t*>(work + pl.off_ref);
-    {
-      KernelSpan span(ctx, "spr_repair");
-      hipLaunchKernelGGL((k_spr_repair<real, R>), dim3((unsigned)(pl.pitch * R / 256), (unsigned)S), dim3(256), 0,
-                         ctx->stream, reinterpret_cast<const int4*>(work + pl.off_ops), d_meta, d_ref, tab, E,
-                         Ex, tips, stride, n_pat, pl.pitch, part, ex, N, M);
-    }
-    PHY_LAUNCHED(ctx);
-    const int dbs = fit_block(dbs_want, [&](int bs) { return deriv_lds(R, sizeof(real), bs, depth); });
-    if (!dbs) return set_err(ctx, PHY_ERANGE, "spr incremental: LDS stack too deep");
-    {
-      KernelSpan span(ctx, "spr_insert");
-      hipLaunchKernelGGL((k_spr_insert<real, R>), dim3((unsigned)(pl.pitch * R / dbs), (unsigned)S), dim3(dbs),
-                         deriv_lds(R, sizeof(real), dbs, depth), ctx->stream,
-                         reinterpret_cast<const int4*>(work + pl.off_prog), (int64_t)(N - 1), d_meta, d_ref,
-                         tab, E, Eh, Ex, d_pi, d_probs, tips, stride, n_pat, pl.pitch, part, ex, weights, wsum,
-                         pl.n_waves, N, M, depth);
-    }
-    PHY_LAUNCHED(ctx);
-    // per-move sums, max_moves at a time (a list may repeat a move); the
-    // sources stay alive for the whole call and stream order keeps each
-    // upload behind the previous gather
-    for (int64_t c0 = m0; c0 < m1; c0 += pl.max_moves) {
-      const int64_t c = std::min<int64_t>(pl.max_moves, m1 - c0);
-      PHY_HIP(ctx, hipMemcpyAsync(rows, mv_rows.data() + c0, (size_t)c * 8, hipMemcpyHostToDevice, ctx->stream));
-      PHY_HIP(ctx, hipMemcpyAsync(idx, mv_idx.data() + c0, (size_t)c * 8, hipMemcpyHostToDevice, ctx->stream));
-      hipLaunchKernelGGL(k_reduce_gather, dim3((unsigned)c), dim3(256), 0, ctx->stream, wsum, pl.n_waves, rows,
-                         idx, d_lnl);
-      PHY_LAUNCHED(ctx);
-    }
-    // the host staging vectors are refilled by the next batch
-    PHY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  }
-  return PHY_OK;
 }
 
 // Moves grouped by pruned subtree (ascending s): subs, the first move of
@@ -2490,6 +2547,276 @@ bool incr_group(const IncrTree& t, int64_t n_moves, const int32_t* moves, int S,
   return true;
 }
 
+// lengths of a run from the device blen: full | half | merged x of each
+// pruned subtree (t_x + t_p, 0 when p is the root)
+__global__ void k_incr_lengths(const double* __restrict__ blen, int M, const int32_t* __restrict__ xp,
+                               int n_sub, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < M) {
+    out[i] = blen[i];
+    out[M + i] = 0.5 * blen[i];
+  }
+  if (i < n_sub) {
+    const int x = xp[2 * i], p = xp[2 * i + 1];
+    out[2 * M + i] = x >= 0 ? blen[x] + blen[p] : 0.0;
+  }
+}
+
+}  // namespace
+
+// A prepared incremental SPR scoring: the moves grouped by pruned subtree,
+// each pruned tree's pre-order program, pool-row map and repair ops, the
+// base tree's post-order program, all uploaded once; a run issues kernels
+// only.
+struct phy_spr_plan {
+  IncrTree t;
+  IncrPlan pl;
+  int32_t form = 0, R = 0;
+  int64_t n_pat = 0, n_moves = 0;
+  int n_sub = 0;
+  phy_topo* base = nullptr;               // host programs; device copies in dmem
+  std::vector<int64_t> mv_start;          // first grouped move of each subtree (n_sub + 1)
+  std::vector<int32_t> batch_depth;       // message stack depth of each batch
+  char* dmem = nullptr;
+  size_t o_dprog = 0, o_dfifo = 0, o_prog = 0, o_meta = 0, o_ref = 0, o_ops = 0, o_xp = 0, o_rows = 0,
+         o_idx = 0;
+  ~phy_spr_plan() {
+    if (base) {
+      base->d_down_prog = nullptr;
+      base->d_down_fifo = nullptr;
+      phy_topo_destroy(base);
+    }
+    if (dmem) (void)hipFree(dmem);
+  }
+};
+
+namespace {
+
+template <typename real, int R>
+int incr_run(phy_ctx* ctx, const phy_spr_plan* sp, const phy_model* m, const uint64_t* tips, int64_t stride,
+             const uint32_t* weights, const double* d_blen, double* d_lnl, double* d_base, char* work) {
+  const IncrTree& t = sp->t;
+  const IncrPlan& pl = sp->pl;
+  const int N = t.N, M = t.M, K = 4, S = pl.S, n_sub = sp->n_sub;
+  const double* cst = m->d_consts;
+  const double* d_pi = cst + 2 * K * K + K + R;
+  const double* d_probs = d_pi + K;
+  double* bl = reinterpret_cast<double*>(work + pl.off_bl);
+  real* E = reinterpret_cast<real*>(work + pl.off_E);
+  real* Eh = reinterpret_cast<real*>(work + pl.off_Eh);
+  real* Ex = reinterpret_cast<real*>(work + pl.off_Ex);
+  real* tab = reinterpret_cast<real*>(work + pl.off_tab);
+  real* Pf = reinterpret_cast<real*>(work + pl.off_P);
+  real* Ph = reinterpret_cast<real*>(work + pl.off_Ph);
+  real* Px = reinterpret_cast<real*>(work + pl.off_Px);
+  real* part = reinterpret_cast<real*>(work + pl.off_part);
+  int32_t* ex = reinterpret_cast<int32_t*>(work + pl.off_ex);
+  double* wsum = reinterpret_cast<double*>(work + pl.off_wsum);
+  const char* dm = sp->dmem;
+  const auto* d_prog = reinterpret_cast<const int4*>(dm + sp->o_prog);
+  const auto* d_meta = reinterpret_cast<const int32_t*>(dm + sp->o_meta);
+  const auto* d_ref = reinterpret_cast<const int32_t*>(dm + sp->o_ref);
+  const auto* d_ops = reinterpret_cast<const int4*>(dm + sp->o_ops);
+  const auto* d_xp = reinterpret_cast<const int32_t*>(dm + sp->o_xp);
+  const auto* d_rows = reinterpret_cast<const int64_t*>(dm + sp->o_rows);
+  const auto* d_idx = reinterpret_cast<const int64_t*>(dm + sp->o_idx);
+
+  // lengths, eigen factors and P matrices of every branch and merged branch
+  const int nl = std::max(M, n_sub);
+  hipLaunchKernelGGL(k_incr_lengths, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, ctx->stream, d_blen, M,
+                     d_xp, n_sub, bl);
+  PHY_LAUNCHED(ctx);
+  auto eig = [&](const double* lens, int nb, real* out) {
+    const int64_t ne = std::max<int64_t>((int64_t)nb * R * K, 32 + 4 * R);
+    hipLaunchKernelGGL((k_eigen_exp<real>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, ctx->stream,
+                       cst, R, lens, nb, out, tab);
+  };
+  auto pmat = [&](const double* lens, int nb, real* out) {
+    const int64_t n = (int64_t)nb * R * K * K;
+    hipLaunchKernelGGL((k_pmats<real>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, cst,
+                       K, R, lens, HostLens{}, nb, out, (real*)nullptr, (real*)nullptr);
+  };
+  eig(bl, M, E);
+  PHY_LAUNCHED(ctx);
+  eig(bl + M, M, Eh);
+  PHY_LAUNCHED(ctx);
+  pmat(bl + M, M, Ph);
+  PHY_LAUNCHED(ctx);
+  if (n_sub > 0) {
+    eig(bl + 2 * M, n_sub, Ex);
+    PHY_LAUNCHED(ctx);
+    pmat(bl + 2 * M, n_sub, Px);
+    PHY_LAUNCHED(ctx);
+  }
+  // the base tree's partials: stored post-order pass into pool rows 0 .. N-2
+  {
+    Plan bp = pl.base;
+    bp.off_P = pl.off_P;
+    bp.off_part = pl.off_part;
+    bp.off_ex = pl.off_ex;
+    bp.off_bsum = pl.off_bsum;
+    double* base_lnl = d_base ? d_base : reinterpret_cast<double*>(work + pl.off_lnl);
+    int rc = launch_tree<real, R>(ctx, sp->base, m, false, false, sp->n_pat, tips, stride, weights, bl, base_lnl,
+                                  nullptr, nullptr, nullptr, work, bp);
+    if (rc) return rc;
+  }
+  const int64_t n_pat = sp->n_pat;
+  for (int b0 = 0, bi = 0; b0 < n_sub; b0 += S, ++bi) {
+    const int Sb = std::min(S, n_sub - b0);
+    const int depth = std::max(sp->batch_depth[bi], 1);
+    const int32_t* meta = d_meta + (size_t)b0 * kIncrMeta;
+    const int32_t* ref = d_ref + (size_t)b0 * M;
+    {
+      KernelSpan span(ctx, "spr_repair");
+      hipLaunchKernelGGL((k_spr_repair<real, R>), dim3((unsigned)(pl.pitch * R / 256), (unsigned)Sb), dim3(256),
+                         0, ctx->stream, d_ops + (size_t)b0 * pl.lmax, meta, ref, tab, E,
+                         Ex + (size_t)b0 * R * K, tips, stride, n_pat, pl.pitch, part, ex, N, M);
+    }
+    PHY_LAUNCHED(ctx);
+    const int4* prog = d_prog + (size_t)b0 * (N - 1);
+    const size_t wl = insert_w_lds(R, sizeof(real), depth);
+    int64_t n_slot;
+    if (wl <= kLdsMax) {
+      n_slot = pl.n_blk;
+      KernelSpan span(ctx, "spr_insert");
+      hipLaunchKernelGGL((k_spr_insert_w<real, R>), dim3((unsigned)pl.n_blk, (unsigned)Sb), dim3(64 * R), wl,
+                         ctx->stream, prog, (int64_t)(N - 1), meta, ref, Pf, Ph, Px + (size_t)b0 * R * 16, d_pi,
+                         d_probs, tips, stride, n_pat, pl.pitch, part, ex, weights, wsum, pl.n_blk, N, M, depth);
+    } else {
+      // a stack too deep for one class per wave: the (pattern, rate) form
+      const int dbs = fit_block(env_block("PHY_SPR_INCR_BLOCK", 256),
+                                [&](int bs) { return deriv_lds(R, sizeof(real), bs, depth); });
+      if (!dbs) return set_err(ctx, PHY_ERANGE, "spr incremental: LDS stack too deep");
+      n_slot = pl.n_waves;
+      KernelSpan span(ctx, "spr_insert");
+      hipLaunchKernelGGL((k_spr_insert<real, R>), dim3((unsigned)(pl.pitch * R / dbs), (unsigned)Sb), dim3(dbs),
+                         deriv_lds(R, sizeof(real), dbs, depth), ctx->stream, prog, (int64_t)(N - 1), meta, ref,
+                         tab, E, Eh, Ex + (size_t)b0 * R * K, d_pi, d_probs, tips, stride, n_pat, pl.pitch, part,
+                         ex, weights, wsum, pl.n_waves, N, M, depth);
+    }
+    PHY_LAUNCHED(ctx);
+    const int64_t m0 = sp->mv_start[b0], m1 = sp->mv_start[b0 + Sb];
+    if (m1 > m0) {
+      hipLaunchKernelGGL(k_reduce_gather, dim3((unsigned)(m1 - m0)), dim3(256), 0, ctx->stream, wsum, n_slot,
+                         d_rows + m0, d_idx + m0, d_lnl);
+      PHY_LAUNCHED(ctx);
+    }
+  }
+  return PHY_OK;
+}
+
+int incr_dispatch(phy_ctx* ctx, const phy_spr_plan* sp, const phy_model* m, const uint64_t* tips, int64_t stride,
+                  const uint32_t* weights, const double* d_blen, double* d_lnl, double* d_base, char* work) {
+  switch ((sp->form & PHY_F32 ? 10 : 0) + m->R) {
+#define PHY_INCR(real, Rv) return incr_run<real, Rv>(ctx, sp, m, tips, stride, weights, d_blen, d_lnl, d_base, work)
+    case 1: PHY_INCR(double, 1);
+    case 2: PHY_INCR(double, 2);
+    case 4: PHY_INCR(double, 4);
+    case 8: PHY_INCR(double, 8);
+    case 11: PHY_INCR(float, 1);
+    case 12: PHY_INCR(float, 2);
+    case 14: PHY_INCR(float, 4);
+    case 18: PHY_INCR(float, 8);
+#undef PHY_INCR
+    default: return set_err(ctx, PHY_ERANGE, "spr incremental: n_rates must be 1, 2, 4 or 8");
+  }
+}
+
+int plan_create(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat, int32_t N, const int32_t* children,
+                int32_t root, int64_t n_moves, const int32_t* moves, int32_t s_batch, phy_spr_plan** out) {
+  std::unique_ptr<phy_spr_plan> sp(new phy_spr_plan());
+  if (!sp->t.load(N, children, root)) return set_err(ctx, PHY_EINVAL, "spr incremental: not a rooted binary tree");
+  const IncrTree& t = sp->t;
+  const int M = t.M;
+  sp->form = form;
+  sp->R = m->R;
+  sp->n_pat = n_pat;
+  sp->n_moves = n_moves;
+  sp->pl = incr_plan(t, m->R, n_pat, form, s_batch);
+  if (phy_topo_create(nullptr, N, children, root, &sp->base) != PHY_OK)
+    return set_err(ctx, PHY_EINVAL, "spr incremental: base topology");
+  sp->pl.base = make_plan(sp->base, m, n_pat, form & ~PHY_FUSED, 0);
+  std::vector<int32_t> subs;
+  std::vector<int64_t> rows, idx;
+  int64_t bad = -1;
+  if (!incr_group(t, n_moves, moves, s_batch, &subs, &sp->mv_start, &rows, &idx, &bad))
+    return set_err(ctx, PHY_EINVAL, "spr incremental: invalid move " + std::to_string(bad));
+  const int n_sub = sp->n_sub = (int)subs.size();
+  const int S = sp->pl.S, lmax = sp->pl.lmax;
+  std::vector<IncrSub> sub(n_sub);
+  {
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned w = 0; w < std::min<unsigned>(nt, std::max(n_sub, 1)); ++w)
+      th.emplace_back([&, w] {
+        for (int k = (int)w; k < n_sub; k += (int)nt) incr_prepare(t, subs[k], k % S, lmax, &sub[k]);
+      });
+    for (auto& x : th) x.join();
+  }
+  std::vector<int32_t> hprog((size_t)std::max(n_sub, 1) * (N - 1) * 4, 0), hmeta((size_t)std::max(n_sub, 1) * kIncrMeta, 0),
+      href((size_t)std::max(n_sub, 1) * M, 0), hops((size_t)std::max(n_sub, 1) * lmax * 4, 0),
+      hxp((size_t)std::max(n_sub, 1) * 2, 0);
+  sp->batch_depth.assign((n_sub + S - 1) / S, 1);
+  for (int k = 0; k < n_sub; ++k) {
+    const IncrSub& q = sub[k];
+    std::copy(q.prog.begin(), q.prog.end(), hprog.begin() + (size_t)k * (N - 1) * 4);
+    std::copy(q.ref.begin(), q.ref.end(), href.begin() + (size_t)k * M);
+    std::copy(q.ops.begin(), q.ops.end(), hops.begin() + (size_t)k * lmax * 4);
+    int32_t* mk = &hmeta[(size_t)k * kIncrMeta];
+    mk[0] = (k % S) * lmax;
+    mk[1] = (int32_t)q.ops.size() / 4;
+    mk[2] = q.x;
+    mk[3] = q.s;
+    mk[4] = q.n_ops;
+    mk[5] = q.depth;
+    hxp[2 * k] = q.x;
+    hxp[2 * k + 1] = q.p;
+    sp->batch_depth[k / S] = std::max(sp->batch_depth[k / S], q.depth);
+  }
+  std::vector<int32_t> fifo(phy_fifo_padded(N), sp->base->down_fifo[0]);
+  std::copy(sp->base->down_fifo.begin(), sp->base->down_fifo.end(), fifo.begin());
+  // one device allocation for every prepared array
+  size_t o = 0;
+  auto put = [&](size_t bytes) {
+    const size_t at = o;
+    o = align256(o + std::max<size_t>(bytes, 8));
+    return at;
+  };
+  sp->o_dprog = put(sp->base->down_prog.size() * 4);
+  sp->o_dfifo = put(fifo.size() * 4);
+  sp->o_prog = put(hprog.size() * 4);
+  sp->o_meta = put(hmeta.size() * 4);
+  sp->o_ref = put(href.size() * 4);
+  sp->o_ops = put(hops.size() * 4);
+  sp->o_xp = put(hxp.size() * 4);
+  sp->o_rows = put(rows.size() * 8);
+  sp->o_idx = put(idx.size() * 8);
+  PHY_HIP(ctx, hipMalloc((void**)&sp->dmem, o));
+  auto up = [&](size_t off, const void* src, size_t bytes) {
+    return bytes ? hipMemcpy(sp->dmem + off, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+  };
+  PHY_HIP(ctx, up(sp->o_dprog, sp->base->down_prog.data(), sp->base->down_prog.size() * 4));
+  PHY_HIP(ctx, up(sp->o_dfifo, fifo.data(), fifo.size() * 4));
+  PHY_HIP(ctx, up(sp->o_prog, hprog.data(), hprog.size() * 4));
+  PHY_HIP(ctx, up(sp->o_meta, hmeta.data(), hmeta.size() * 4));
+  PHY_HIP(ctx, up(sp->o_ref, href.data(), href.size() * 4));
+  PHY_HIP(ctx, up(sp->o_ops, hops.data(), hops.size() * 4));
+  PHY_HIP(ctx, up(sp->o_xp, hxp.data(), hxp.size() * 4));
+  PHY_HIP(ctx, up(sp->o_rows, rows.data(), rows.size() * 8));
+  PHY_HIP(ctx, up(sp->o_idx, idx.data(), idx.size() * 8));
+  sp->base->d_down_prog = reinterpret_cast<int4*>(sp->dmem + sp->o_dprog);
+  sp->base->d_down_fifo = reinterpret_cast<int32_t*>(sp->dmem + sp->o_dfifo);
+  *out = sp.release();
+  return PHY_OK;
+}
+
+int check_plan_model(phy_ctx* ctx, const phy_model* m) {
+  if (m->K != 4) return set_err(ctx, PHY_EINVAL, "spr incremental: 4-state models only");
+  if (m->R != 1 && m->R != 2 && m->R != 4 && m->R != 8)
+    return set_err(ctx, PHY_ERANGE, "spr incremental: n_rates must be 1, 2, 4 or 8");
+  return PHY_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2500,13 +2827,48 @@ int phy_lk_spr_incr_work_size(int32_t N, const int32_t* children, int32_t root, 
   try {
     IncrTree t;
     if (!t.load(N, children, root)) return PHY_EINVAL;
-    IncrPlan pl = incr_plan(t, m, n_pat, form, s_batch);
-    *bytes = pl.total;
+    *bytes = incr_plan(t, m->R, n_pat, form, s_batch).total;
     return PHY_OK;
   } catch (...) {
     return PHY_ENOMEM;
   }
 }
+
+int phy_spr_plan_create(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat, int32_t N,
+                        const int32_t* children, int32_t root, int64_t n_moves, const int32_t* moves,
+                        int32_t s_batch, phy_spr_plan** out) {
+  if (!ctx || !m || !out) return PHY_EINVAL;
+  *out = nullptr;
+  if (N < 3 || n_pat < 1 || s_batch < 1 || n_moves < 0 || !children || (n_moves > 0 && !moves))
+    return set_err(ctx, PHY_EINVAL, "spr plan: bad argument");
+  int rc = check_plan_model(ctx, m);
+  if (rc) return rc;
+  try {
+    return plan_create(ctx, m, form, n_pat, N, children, root, n_moves, moves, s_batch, out);
+  } catch (...) {
+    return set_err(ctx, PHY_ENOMEM, "spr plan: host allocation or thread start failed");
+  }
+}
+
+int phy_spr_plan_work_size(const phy_spr_plan* plan, size_t* bytes) {
+  if (!plan || !bytes) return PHY_EINVAL;
+  *bytes = plan->pl.total;
+  return PHY_OK;
+}
+
+int phy_spr_plan_run(phy_ctx* ctx, const phy_spr_plan* plan, const phy_model* m, const uint64_t* tips,
+                     int64_t stride, const uint32_t* weights, const double* d_blen, double* d_lnl,
+                     double* d_base_lnl, void* d_work, size_t work_bytes) {
+  if (!ctx || !plan || !m || !tips || !weights || !d_blen || !d_lnl || !d_work) return PHY_EINVAL;
+  int rc = check_plan_model(ctx, m);
+  if (rc) return rc;
+  if (m->R != plan->R) return set_err(ctx, PHY_EINVAL, "spr plan: model has another number of rate classes");
+  if (stride < phy_words(plan->n_pat)) return set_err(ctx, PHY_EINVAL, "spr plan: tip stride < words");
+  if (work_bytes < plan->pl.total) return set_err(ctx, PHY_EINVAL, "spr plan: workspace too small");
+  return incr_dispatch(ctx, plan, m, tips, stride, weights, d_blen, d_lnl, d_base_lnl, static_cast<char*>(d_work));
+}
+
+void phy_spr_plan_destroy(phy_spr_plan* plan) { delete plan; }
 
 int phy_lk_spr_scores_incr(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat,
                            const uint64_t* tips, int64_t stride, const uint32_t* weights, int32_t N,
@@ -2516,39 +2878,26 @@ int phy_lk_spr_scores_incr(phy_ctx* ctx, const phy_model* m, int32_t form, int64
   int rc = check_scores_args(ctx, m, n_pat, tips, stride, weights, N, children, blen, n_moves, moves,
                              s_batch, d_lnl, d_work, root, "spr incremental");
   if (rc) return rc;
+  rc = check_plan_model(ctx, m);
+  if (rc) return rc;
   try {
     IncrTree t;
     if (!t.load(N, children, root)) return set_err(ctx, PHY_EINVAL, "spr incremental: not a rooted binary tree");
-    IncrPlan pl = incr_plan(t, m, n_pat, form, s_batch);
-    if (work_bytes < pl.total) return set_err(ctx, PHY_EINVAL, "spr incremental: workspace too small");
-    {
-      phy_topo* bt = nullptr;
-      if (phy_topo_create(nullptr, N, children, root, &bt) != PHY_OK)
-        return set_err(ctx, PHY_EINVAL, "spr incremental: base topology");
-      pl.base = make_plan(bt, m, n_pat, form & ~PHY_FUSED, 0);
-      phy_topo_destroy(bt);
-    }
-    std::vector<int32_t> subs;
-    std::vector<int64_t> start, rows, idx;
-    int64_t bad = -1;
-    if (!incr_group(t, n_moves, moves, s_batch, &subs, &start, &rows, &idx, &bad))
-      return set_err(ctx, PHY_EINVAL, "spr incremental: invalid move " + std::to_string(bad));
+    const IncrPlan lay = incr_plan(t, m->R, n_pat, form, s_batch);
+    if (work_bytes < lay.total) return set_err(ctx, PHY_EINVAL, "spr incremental: workspace too small");
+    phy_spr_plan* sp = nullptr;
+    rc = plan_create(ctx, m, form, n_pat, N, children, root, n_moves, moves, s_batch, &sp);
+    if (rc) return rc;
+    std::unique_ptr<phy_spr_plan> hold(sp);
     char* work = static_cast<char*>(d_work);
-    switch ((form & PHY_F32 ? 10 : 0) + m->R) {
-#define PHY_INCR(real, Rv)                                                                                   \
-  return incr_launch<real, Rv>(ctx, m, t, n_pat, tips, stride, weights, blen, pl, work, subs, start, rows, idx, \
-                               d_lnl, d_base_lnl)
-      case 1: PHY_INCR(double, 1);
-      case 2: PHY_INCR(double, 2);
-      case 4: PHY_INCR(double, 4);
-      case 8: PHY_INCR(double, 8);
-      case 11: PHY_INCR(float, 1);
-      case 12: PHY_INCR(float, 2);
-      case 14: PHY_INCR(float, 4);
-      case 18: PHY_INCR(float, 8);
-#undef PHY_INCR
-      default: return set_err(ctx, PHY_ERANGE, "spr incremental: n_rates must be 1, 2, 4 or 8");
-    }
+    double* dbl = reinterpret_cast<double*>(work + lay.off_blh);
+    PHY_HIP(ctx, hipMemcpyAsync(dbl, blen, (size_t)t.M * 8, hipMemcpyHostToDevice, ctx->stream));
+    rc = incr_dispatch(ctx, sp, m, tips, stride, weights, dbl, d_lnl, d_base_lnl, work);
+    // the plan's device arrays are freed on return
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (rc) return rc;
+    if (e != hipSuccess) return set_err(ctx, PHY_EDEVICE, std::string("spr incremental: ") + hipGetErrorString(e));
+    return PHY_OK;
   } catch (...) {
     return set_err(ctx, PHY_ENOMEM, "spr incremental: host allocation or thread start failed");
   }

@@ -110,6 +110,66 @@ class IncrementalScorer:
         return out[: len(mv)]
 
 
+class SprPlan:
+    """Incremental scoring of a fixed move list on a fixed topology,
+    prepared once (phy_spr_plan_create) and run on device branch lengths
+    (phy_spr_plan_run): a run issues kernels only, so ``capture`` records it
+    in a HIP graph that rescoring replays after the lengths (written into
+    the same device tensor) change.  Capture needs ``ctx`` on a side
+    stream."""
+
+    def __init__(self, ctx: D.Context, dm, n_leaves: int, tips, stride: int, weights,
+                 n_patterns: int, children, root: int, mv: np.ndarray, form: int = 0,
+                 s_batch: int = 16):
+        torch = __import__("torch")
+        self.ctx, self.dm, self.N = ctx, dm, int(n_leaves)
+        self.tips, self.stride, self.weights = tips, int(stride), weights
+        self.mv = np.ascontiguousarray(mv, np.int32).reshape(-1, 2)
+        ch = np.ascontiguousarray(children, np.int32)
+        h = C.c_void_p()
+        ctx.check(lib.phy_spr_plan_create(ctx.handle, dm.handle, int(form), int(n_patterns), self.N,
+                                          ptr(ch, C.c_int32), int(root), len(self.mv),
+                                          ptr(self.mv, C.c_int32), int(s_batch), C.byref(h)),
+                  "spr_plan_create")
+        self.handle = h
+        nb = C.c_size_t()
+        check(lib.phy_spr_plan_work_size(h, C.byref(nb)))
+        dev = f"cuda:{ctx.device}"
+        self.work = torch.empty((max(nb.value, 8),), dtype=torch.uint8, device=dev)
+        self.lnl = torch.zeros((max(len(self.mv), 1),), dtype=torch.float64, device=dev)
+        self.base_lnl = torch.zeros((1,), dtype=torch.float64, device=dev)
+
+    def run(self, blen_dev, dm=None) -> None:
+        """Enqueue the scores of every move at the device lengths blen_dev
+        into self.lnl (and the base tree's into self.base_lnl)."""
+        dm = dm or self.dm
+        self.ctx.check(lib.phy_spr_plan_run(
+            self.ctx.handle, self.handle, dm.handle, D.dptr(self.tips), self.stride,
+            D.dptr(self.weights), D.dptr(blen_dev), D.dptr(self.lnl), D.dptr(self.base_lnl),
+            D.dptr(self.work), self.work.numel()), "spr_plan_run")
+
+    def capture(self, blen_dev):
+        torch = __import__("torch")
+        if not self.ctx.stream:
+            raise ValueError("graph capture needs the context on a side stream")
+        s = torch.cuda.ExternalStream(self.ctx.stream, device=f"cuda:{self.ctx.device}")
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            self.run(blen_dev)
+        return g
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.phy_spr_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------------------
 # Move features
 # ---------------------------------------------------------------------------

@@ -199,3 +199,46 @@ def test_incremental_invalid_moves(ctx):
     # the scorer still works after the errors
     got = inc.score(tree.children, tree.root, tree.blen, mv[:5]).cpu().numpy()
     assert np.all(np.isfinite(got))
+
+
+def test_plan_runs_equal_the_one_shot_scores_and_replay_from_a_graph():
+    """A prepared plan (phy_spr_plan_*) scores like the one-shot call bit for
+    bit; captured in a HIP graph, replays after the branch lengths change in
+    place equal eager runs at the new lengths, with eager work between."""
+    from phylocaml_amd import device as D, likelihood as LK, spr, synth
+
+    N, P = 40, 1500
+    tree = synth.yule(N, 31, 0.08)
+    model = _model(31)
+    st, w = _data(tree, model, P, 31)
+    mv = spr.moves(N, tree.children, tree.root)
+    s = torch.cuda.Stream()
+    ctx = D.Context(0, stream=s.cuda_stream)
+    with torch.cuda.stream(s):
+        dm = LK.DeviceModel(ctx, model)
+        tips = D.to_device(synth.pack_dna(st))
+        wd = D.to_device(w)
+        for form in (F64, F32):
+            one = spr.IncrementalScorer(ctx, dm, N, tips, D.words(P), wd, P, form=form, s_batch=6)
+            ref0 = one.score(tree.children, tree.root, tree.blen, mv).cpu().numpy()
+            plan = spr.SprPlan(ctx, dm, N, tips, D.words(P), wd, P, tree.children, tree.root, mv,
+                               form=form, s_batch=6)
+            bl = D.to_device(np.asarray(tree.blen, np.float64))
+            plan.run(bl)
+            ctx.sync()
+            assert np.array_equal(plan.lnl.cpu().numpy(), ref0), form
+            g = plan.capture(bl)
+            rng = np.random.default_rng(form)
+            for rep in range(3):
+                new = np.asarray(tree.blen) * rng.uniform(0.5, 1.5, 2 * N - 1)
+                new[tree.root] = 0.0
+                bl.copy_(torch.from_numpy(new))
+                torch.cuda.synchronize()
+                g.replay()
+                torch.cuda.synchronize()
+                got = plan.lnl.cpu().numpy().copy()
+                want = one.score(tree.children, tree.root, new, mv).cpu().numpy()  # eager
+                print(f"form {form} replay {rep}: max |diff| {np.max(np.abs(got - want)):.3g}",
+                      flush=True)
+                assert np.array_equal(got, want), (form, rep)
+            plan.close()
