@@ -11,37 +11,45 @@ namespace phy {
 
 static const double kComposeEps = 1e-10;  // lib/mlmodel.h EPSILON
 
+// Regularized lower incomplete gamma P(a, x) = gamma(a, x) / Gamma(a), with
+// F = x^a e^-x / Gamma(a):
+//  * x < a + 1: the power series P = F * sum_{n>=0} x^n Gamma(a) / Gamma(a+n+1)
+//    (Abramowitz & Stegun 6.5.29), whose terms t_0 = 1/a,
+//    t_n = t_{n-1} x / (a + n) fall off fast there;
+//  * otherwise Q = 1 - P = F / f with the continued fraction
+//    f = b_0 + a_1 / (b_1 + a_2 / (b_2 + ...)),  b_k = x + 2k + 1 - a,
+//    a_k = -k (k - a)  (the even form of A&S 6.5.31), evaluated forward by the
+//    modified Lentz method (W. J. Lentz, Appl. Opt. 15 (1976) 668;
+//    I. J. Thompson and A. R. Barnett, J. Comput. Phys. 64 (1986) 490).
 double gammp(double a, double x) {
   if (x <= 0.0) return 0.0;
   if (std::isinf(x)) return 1.0;
-  const double gln = std::lgamma(a);
+  const double log_f = a * std::log(x) - x - std::lgamma(a);
   if (x < a + 1.0) {
-    // series: P = e^-x x^a / Gamma(a+1) * sum_n x^n / ((a+1)...(a+n))
-    double ap = a, sum = 1.0 / a, del = sum;
-    for (int n = 0; n < 100000; ++n) {
-      ap += 1.0;
-      del *= x / ap;
-      sum += del;
-      if (std::fabs(del) < std::fabs(sum) * 1e-17) break;
+    double term = 1.0 / a, total = term;
+    for (int n = 1; n < 100000; ++n) {
+      term *= x / (a + n);
+      total += term;
+      if (std::fabs(term) < std::fabs(total) * 1e-17) break;
     }
-    return sum * std::exp(-x + a * std::log(x) - gln);
+    return total * std::exp(log_f);
   }
-  // continued fraction for Q = 1 - P (modified Lentz)
-  const double fpmin = 1e-300;
-  double b = x + 1.0 - a, c = 1.0 / fpmin, d = 1.0 / b, h = d;
-  for (int i = 1; i < 100000; ++i) {
-    const double an = -i * (i - a);
-    b += 2.0;
-    d = an * d + b;
-    if (std::fabs(d) < fpmin) d = fpmin;
-    c = b + an / c;
-    if (std::fabs(c) < fpmin) c = fpmin;
-    d = 1.0 / d;
-    const double del = d * c;
-    h *= del;
-    if (std::fabs(del - 1.0) < 1e-17) break;
+  const double tiny = 1e-300;  // stands in for a zero denominator
+  double f = x + 1.0 - a;
+  if (std::fabs(f) < tiny) f = tiny;
+  double C = f, D = 0.0;
+  for (int k = 1; k < 100000; ++k) {
+    const double ak = -k * (k - a), bk = x + 2.0 * k + 1.0 - a;
+    D = bk + ak * D;
+    if (std::fabs(D) < tiny) D = tiny;
+    D = 1.0 / D;
+    C = bk + ak / C;
+    if (std::fabs(C) < tiny) C = tiny;
+    const double step = C * D;
+    f *= step;
+    if (std::fabs(step - 1.0) < 1e-17) break;
   }
-  return 1.0 - std::exp(-x + a * std::log(x) - gln) * h;
+  return 1.0 - std::exp(log_f) / f;
 }
 
 double gammp_inv(double a, double p) {
