@@ -60,18 +60,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(prefix):
-    """HBM bytes per launch of the kernels whose name starts with `prefix`, from
-    the rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in
-    profiles/hbm_traffic.json (already scaled by the copy calibration); None
-    when that kernel was not profiled."""
+def measured_traffic(prefix, grid=None):
+    """HBM bytes per launch of the kernels whose name starts with `prefix`
+    (and whose launch has `grid` work-items, when given), from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes summarised in profiles/hbm_traffic.json
+    (already scaled by the copy calibration); None when that kernel was not
+    profiled."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         with open(path) as f:
             k = json.load(f)["kernels"]
     except (OSError, ValueError, KeyError):
         return None
-    hits = [v for name, v in k.items() if name.startswith(prefix)]
+    hits = [v for name, v in k.items()
+            if name.startswith(prefix) and (grid is None or v.get("grid") == grid)]
     if not hits:
         return None
     return sum(v["read_bytes"] + v["write_bytes"] for v in hits)
@@ -230,9 +232,9 @@ def bench_prune(torch, ctx, clock, steps, warm, cpu_baseline):
             "oracle_inputs": (tree, model, st, w)}
 
 
-def deriv_roofline(k_ms, units, peak, nbytes_prefix):
+def deriv_roofline(k_ms, units, peak, nbytes_prefix, grid=None):
     tf = DERIV_FLOPS * units / (k_ms / 1e3) / 1e12
-    tr = measured_traffic(nbytes_prefix)
+    tr = measured_traffic(nbytes_prefix, grid)
     return {"bound": "valu", "achieved": tf, "peak": peak, "unit": "TFLOP/s", "frac": tf / peak,
             "traffic": tr,
             "hbm_GBs": None if tr is None else tr / (k_ms / 1e3) / 1e9,
@@ -331,7 +333,9 @@ def bench_aa(torch, ctx, clock, n_trees, steps, cpu_baseline):
            "pattern_nodes": pn, "ms": ms, "kernel_ms": k_ms, "launches": tb.launches,
            "pattern_node_per_s": pn / (ms / 1e3), "autotune": tune,
            "finite_and_nonpositive": bool(np.all(np.isfinite(got)) and np.all(got <= 0)),
-           "roofline": {"bound": "mfma", "traffic": measured_traffic("k_aa_prune<"),
+           "roofline": {"bound": "mfma",
+                        "traffic": sum(measured_traffic(f"k_aa_prune<{b.waves}, ") or 0.0
+                                       for _, b in tb.parts) or None,
                         "achieved": useful_tf, "peak": BF16_PEAK_TF, "unit": "TFLOP/s",
                         "frac": useful_tf / BF16_PEAK_TF, "executed_tflops": executed_tf,
                         "note": "6,400 useful BF16 MFMA flop per internal pattern.node "
@@ -702,7 +706,9 @@ def main():
     ms_main, k_main = pr["step_ms"]["f32_fused"], pr["kernel_ms"]["f32_fused"]
     value = units / (ms_main / 1e3)
     tflops = PRUNE_FLOPS * units / (k_main / 1e3) / 1e12
-    traffic = measured_traffic("k_prune_down<float, 4, 1, true,")
+    # the 512 x 200k launch: pitch / 64 blocks of 64 R threads
+    grid = ((pr["P"] + 255) // 256) * 256 * 4
+    traffic = measured_traffic("k_prune_down<float, 4, 1, true,", grid)
     line = {
         "metric": "FP32-scaled vs FP64 pruning updates/s at rel ΔlnL≤1e-6; "
                   "Fitch site-node/s",
@@ -746,7 +752,7 @@ def main():
             "derivatives": {
                 name: {**d, "roofline": deriv_roofline(
                     d["kernel_ms"], units, FP32_PEAK_TF if name == "f32" else FP64_PEAK_TF,
-                    f"k_prune_deriv<{'float' if name == 'f32' else 'double'},")}
+                    f"k_prune_deriv<{'float' if name == 'f32' else 'double'},", grid)}
                 for name, d in pr["deriv"].items()},
         },
         "copy_stream": copy,
