@@ -312,11 +312,12 @@ __device__ __forceinline__ void load_mat(const real* __restrict__ P, real* m) {
   for (int k = 0; k < 16; ++k) m[k] = P[k];
 }
 
-// exact power-of-two rescale of one lane's 4 values
+// exact power-of-two rescale of one lane's 4 values (an all-zero vector
+// keeps its values and exponent: frexp(0) gives exponent 0)
 template <typename real>
 __device__ __forceinline__ void rescale4(real* v, int* e) {
   const real mx = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
-  if (mx < Thr<real>::lo && mx > (real)0) {
+  if (mx < Thr<real>::lo) {
     int ex;
     (void)frexp(mx, &ex);
 #pragma unroll

@@ -37,7 +37,7 @@ __device__ __forceinline__ void for_each_entry(const int4* __restrict__ prog, in
   int4 cur = prog[min(lane, n_ops - 1)];
   int4 nxt = prog[min(64 + lane, n_ops - 1)];
   for (int base = 0; base < n_ops; base += 64) {
-    const int cnt = min(64, n_ops - base);
+    const int cnt = __builtin_amdgcn_readfirstlane(min(64, n_ops - base));  // wave-uniform
     for (int j = 0; j < cnt; ++j) {
       int4 op;
       op.x = __builtin_amdgcn_readlane(cur.x, j);
@@ -60,7 +60,7 @@ __device__ __forceinline__ void for_each_entry_ahead(const int4* __restrict__ pr
   int4 cur = prog[min(lane, n_ops - 1)];
   int4 nxt = prog[min(64 + lane, n_ops - 1)];
   for (int base = 0; base < n_ops; base += 64) {
-    const int cnt = min(64, n_ops - base);
+    const int cnt = __builtin_amdgcn_readfirstlane(min(64, n_ops - base));  // wave-uniform
     for (int j = 0; j < cnt; ++j) {
       int4 op, on;
       op.x = __builtin_amdgcn_readlane(cur.x, j);
