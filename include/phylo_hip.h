@@ -573,6 +573,33 @@ int phy_lk_tbr_scores(phy_ctx* ctx, const phy_model* model, int32_t form,
                       int32_t root, const double* blen, int64_t n_moves,
                       const int32_t* moves, int32_t chunk, double* d_lnl, void* d_work,
                       size_t work_bytes);
+/* The TBR scores without rebuilding a tree per move (form PHY_F32 or
+ * PHY_F64, n_rates 1, 2, 4 or 8), the incremental SPR scoring extended to
+ * rerooted subtrees: moves are grouped into slots (a pruned subtree with up
+ * to 32 of its rerootings); per slot the ancestors of s's parent are
+ * repaired, one pre-order pass over the subtree of s gives the root vector
+ * of each needed rerooting, and one pre-order pass over the pruned tree
+ * scores every needed (target, rerooting) pair, keeping the attachment
+ * (v = -1) included.  Moves are validated as phy_tbr_apply does; the lnL are
+ * those of the trees phy_tbr_apply builds.  The workspace depends on the
+ * moves (slots, rerootings per slot, pairs per batch of s_batch slots).
+ * A plan made by phy_tbr_plan_create is run, sized and destroyed by the
+ * phy_spr_plan functions above.  Replaces the reference's per-candidate TBR
+ * scoring through lib/treeNeighborhoods.ml:27-75 and lib/tree.ml:400-494. */
+int phy_lk_tbr_incr_work_size(int32_t n_leaves, const int32_t* children, int32_t root,
+                              const phy_model* model, int64_t n_patterns, int32_t form,
+                              int64_t n_moves, const int32_t* moves, int32_t s_batch,
+                              size_t* bytes);
+int phy_lk_tbr_scores_incr(phy_ctx* ctx, const phy_model* model, int32_t form,
+                           int64_t n_patterns, const uint64_t* d_tips, int64_t stride,
+                           const uint32_t* d_weights, int32_t n_leaves, const int32_t* children,
+                           int32_t root, const double* blen, int64_t n_moves,
+                           const int32_t* moves, int32_t s_batch, double* d_lnl,
+                           double* d_base_lnl, void* d_work, size_t work_bytes);
+int phy_tbr_plan_create(phy_ctx* ctx, const phy_model* model, int32_t form, int64_t n_patterns,
+                        int32_t n_leaves, const int32_t* children, int32_t root,
+                        int64_t n_moves, const int32_t* moves, int32_t s_batch,
+                        phy_spr_plan** out);
 
 /* ------------------------------------------------------------------------- */
 /* Learned SPR proposal: a small MLP (n_in -> H -> H -> 1, GELU) trained and */

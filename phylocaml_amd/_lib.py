@@ -113,6 +113,11 @@ SIGNATURES = {
     "phy_spr_plan_work_size": (C.c_int, [vp, C.POINTER(sz)]),
     "phy_spr_plan_run": (C.c_int, [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, sz]),
     "phy_spr_plan_destroy": (None, [vp]),
+    "phy_lk_tbr_incr_work_size": (C.c_int, [i32, P_i32, i32, vp, i64, i32, i64, P_i32, i32,
+                                            C.POINTER(sz)]),
+    "phy_lk_tbr_scores_incr": (C.c_int, [vp, vp, i32, i64, vp, i64, vp, i32, P_i32, i32, P_f64,
+                                         i64, P_i32, i32, vp, vp, vp, sz]),
+    "phy_tbr_plan_create": (C.c_int, [vp, vp, i32, i64, i32, P_i32, i32, i64, P_i32, i32, PP]),
     "phy_mlp_create": (C.c_int, [vp, i32, i32, u64, PP]),
     "phy_mlp_destroy": (None, [vp]),
     "phy_mlp_n_params": (i64, [vp]),

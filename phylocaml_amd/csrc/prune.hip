@@ -1558,6 +1558,426 @@ __global__ __launch_bounds__(64 * R) void k_spr_insert_w(
   });
 }
 
+// ---------------------------------------------------------------------------
+// Incremental TBR scoring.  The move (s, w, v) reroots the subtree of s at
+// the middle of the branch above w (w in the subtree; its old root s
+// suppressed, s's two branches one edge) and regrafts it as the SPR move
+// (s, v), or keeps it in place (v = -1).  Per rerooting the subtree's root
+// vector is
+//   X_w = P(t_s) R_w,  R_w = (P(t_w/2) D_w) (*) (P(t_w/2) B_w),
+// B_w the conditional likelihood of the rest of the subtree seen at w's
+// parent, from a pre-order pass over the subtree that starts from C_s = 1:
+//   B_a = (P_b D_b) (*) C_n,  C_a = P_a B_a   (entry n with children a, b);
+// X_s = P(t_s) D_s (no rerooting).  The insertion pass over the pruned tree
+// then scores every needed (target, rerooting) pair; keeping the attachment
+// is the insertion on x's merged branch at its original split (P(t_p) above,
+// P(t_x) below), or, when p is the root, the root of the pruned tree joined
+// to the subtree (pi, then P(t_x)).
+//
+// tbr meta[k]: {subtree program entries, s, 0, subtree stack depth, x (the
+// sibling of s), g >= 0, 0, 0}.  wslot[k][c]: X row of the rerooting at c
+// (-1: not needed); the rows of batch slot k are k * wmax .. k * wmax + 31.
+// ---------------------------------------------------------------------------
+template <typename real, int R>
+__global__ __launch_bounds__(64 * R) void k_tbr_reroot(
+    const int4* __restrict__ prog, int64_t prog_stride, const int32_t* __restrict__ meta,
+    const int32_t* __restrict__ wslot, const real* __restrict__ Pf, const real* __restrict__ Ph,
+    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
+    const real* __restrict__ part, const int32_t* __restrict__ ex, real* __restrict__ xv,
+    int32_t* __restrict__ xe, int N, int M, int up_depth) {
+  constexpr int BD = 64 * R;
+  extern __shared__ unsigned char lds_raw[];
+  const int k = blockIdx.y;
+  const int32_t* mk = meta + kIncrMeta * k;
+  const int n_ops = mk[0], s = mk[1];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int r = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int64_t p = (int64_t)blockIdx.x * 64 + lane;
+  const bool act = p < n_pat;
+  const int64_t pc = act ? p : 0;
+  const int32_t* ws = wslot + (size_t)k * M;
+  real* stk = reinterpret_cast<real*>(lds_raw);                                  // [depth][4][BD]
+  int32_t* stke = reinterpret_cast<int32_t*>(stk + (size_t)up_depth * 4 * BD);  // [depth][BD]
+  real Ps[16];
+  load_mat<real, R>(Pf + ((size_t)s * R + r) * 16, Ps);
+  auto fetch = [&](int c, real* Dv, int* e) {
+    if (c < N) {
+      tipvec(tip_code(tips, stride, c, pc), Dv);
+      *e = 0;
+    } else {
+      const real* sv = part + (((size_t)(c - N) * R + r) * 4) * pitch + pc;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Dv[j] = sv[(size_t)j * pitch];
+      *e = ex[((size_t)(c - N) * R + r) * pitch + pc];
+    }
+  };
+  auto store = [&](int row, const real* R4, int e) {
+    real X[4];
+    int eX = e;
+    matvec4(Ps, R4, X);
+    rescale4(X, &eX);
+    if (act) {
+      real* d = xv + (((size_t)row * R + r) * 4) * pitch + p;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) d[(size_t)j * pitch] = X[j];
+      xe[((size_t)row * R + r) * pitch + p] = eX;
+    }
+  };
+  if (ws[s] >= 0) {  // no rerooting
+    real D[4];
+    int e;
+    fetch(s, D, &e);
+    store(ws[s], D, e);
+  }
+  if (n_ops == 0) return;  // no barriers in this kernel
+  real Dc[8];
+  int Xc[2];
+  {
+    const int4 o = prog[(size_t)k * prog_stride];
+    fetch(o.y, Dc, &Xc[0]);
+    fetch(o.z, Dc + 4, &Xc[1]);
+  }
+  real accC[4] = {(real)0, (real)0, (real)0, (real)0};
+  int accE = 0;
+  int sp = 0;
+  for_each_entry_ahead(prog + (size_t)k * prog_stride, n_ops, [&](int i, const int4 op, const int4 on) {
+    real Dn[8];
+    int Xn[2] = {0, 0};
+    const bool more = i + 1 < n_ops;
+    if (more) {
+      fetch(on.y, Dn, &Xn[0]);
+      fetch(on.z, Dn + 4, &Xn[1]);
+    }
+    const int fl = op.w, src = fl & 3;
+    const int a = op.y, b = op.z;
+    real Cv[4];
+    int eC;
+    if (src == SRC_LEAF) {  // the subtree's root: nothing outside it
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Cv[j] = (real)1;
+      eC = 0;
+    } else if (src == SRC_ACC) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Cv[j] = accC[j];
+      eC = accE;
+    } else {
+      --sp;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Cv[j] = stk[((size_t)sp * 4 + j) * BD + tid];
+      eC = stke[(size_t)sp * BD + tid];
+    }
+    real PA[16], PB[16];
+    load_mat<real, R>(Pf + ((size_t)a * R + r) * 16, PA);
+    load_mat<real, R>(Pf + ((size_t)b * R + r) * 16, PB);
+    real ya[4], yb[4], Ba[4], Bb[4];
+    matvec4(PA, Dc, ya);
+    matvec4(PB, Dc + 4, yb);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      Ba[j] = yb[j] * Cv[j];
+      Bb[j] = ya[j] * Cv[j];
+    }
+    int eBa = eC + Xc[1], eBb = eC + Xc[0];
+    rescale4(Ba, &eBa);
+    rescale4(Bb, &eBb);
+    // rerootings at a and b
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int c = q ? b : a;
+      const int row = ws[c];
+      if (row < 0) continue;
+      real PH[16], u[4], v[4], Rw[4];
+      load_mat<real, R>(Ph + ((size_t)c * R + r) * 16, PH);
+      matvec4(PH, q ? Dc + 4 : Dc, u);
+      matvec4(PH, q ? Bb : Ba, v);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Rw[j] = u[j] * v[j];
+      int eR = (q ? Xc[1] + eBb : Xc[0] + eBa);
+      rescale4(Rw, &eR);
+      store(row, Rw, eR);
+    }
+    // outside vectors at internal children: C_c = P_c B_c
+    real Ca[4], Cb[4];
+    int eCa = eBa, eCb = eBb;
+    if (!(fl & UP_A_LEAF)) {
+      matvec4(PA, Ba, Ca);
+      rescale4(Ca, &eCa);
+    }
+    if (!(fl & UP_B_LEAF)) {
+      matvec4(PB, Bb, Cb);
+      rescale4(Cb, &eCb);
+    }
+    if (fl & UP_PUSH) {
+      const bool cb = fl & UP_CONT_B;
+      const real* o = cb ? Ca : Cb;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) stk[((size_t)sp * 4 + j) * BD + tid] = o[j];
+      stke[(size_t)sp * BD + tid] = cb ? eCa : eCb;
+      ++sp;
+    }
+    if (!(fl & UP_A_LEAF) || !(fl & UP_B_LEAF)) {
+      const bool cb = fl & UP_CONT_B;
+      const real* c = cb ? Cb : Ca;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) accC[j] = c[j];
+      accE = cb ? eCb : eCa;
+    }
+    if (more) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Dc[j] = Dn[j];
+      Xc[0] = Xn[0];
+      Xc[1] = Xn[1];
+    }
+  });
+}
+
+// The insertion pass of TBR moves over each pruned tree: at every entry,
+// for each child c with needed pairs (tmask[k][c], one bit per rerooting
+// row w of the subtree), q = (P(t_c/2)^T A_c) (*) (P(t_c/2) D_c) and the
+// class term q . X_w of each needed w; the classes of a pair meet in LDS
+// (double-buffered by pair parity, one barrier per pair, the pair's wave
+// r = pair % R finishing it).  Keeping the attachment (tmask[k][M]) is the
+// child x of g with P(t_p) above and P(t_x) below (Pk[k]), or, when p is the
+// root, the first entry (x, the pruned tree's root) with pi above.
+// wsum row of (k, c, w): rb[k][c] + the rank of w among c's needed bits.
+template <typename real, int R>
+__global__ __launch_bounds__(64 * R) void k_tbr_insert_w(
+    const int4* __restrict__ prog, int64_t prog_stride, const int32_t* __restrict__ meta,
+    const int32_t* __restrict__ tmeta, const int32_t* __restrict__ ref, const uint32_t* __restrict__ tmask,
+    const int32_t* __restrict__ rbase, const real* __restrict__ Pf, const real* __restrict__ Ph,
+    const real* __restrict__ Px, const real* __restrict__ Pk, const real* __restrict__ xv,
+    const int32_t* __restrict__ xe, int wmax, const double* __restrict__ pi, const double* __restrict__ probs,
+    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
+    const real* __restrict__ part, const int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
+    double* __restrict__ wsum, int64_t n_blk, int N, int M, int up_depth) {
+  constexpr int BD = 64 * R;
+  extern __shared__ unsigned char lds_raw[];
+  const int k = blockIdx.y;
+  const int32_t* mk = meta + kIncrMeta * k;
+  const int32_t* tk = tmeta + kIncrMeta * k;
+  const int n_ops = mk[4];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int r = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int64_t p = (int64_t)blockIdx.x * 64 + lane;
+  const bool act = p < n_pat;
+  const int64_t pc = act ? p : 0;
+  real* stk = reinterpret_cast<real*>(lds_raw);                                  // [depth][4][BD]
+  int32_t* stke = reinterpret_cast<int32_t*>(stk + (size_t)up_depth * 4 * BD);  // [depth][BD]
+  double* xvl = reinterpret_cast<double*>(stke + (size_t)up_depth * BD);        // [2][R][64]
+  int32_t* xel = reinterpret_cast<int32_t*>(xvl + 2 * R * 64);                  // [2][R][64]
+  const int32_t* rf = ref + (size_t)k * M;
+  const uint32_t* tm = tmask + (size_t)k * (M + 1);
+  const int32_t* rb = rbase + (size_t)k * (M + 1);
+  const int x = mk[2];
+  const bool has_g = tk[5] != 0;
+  const int xrow0 = k * wmax;
+  const int4* pg = prog + (size_t)k * prog_stride;
+  const real* pxk = Px + ((size_t)k * R + r) * 16;
+  auto pfull = [&](int c) { return c == x ? pxk : Pf + ((size_t)c * R + r) * 16; };
+  int pair = 0;  // pairs finished so far (uniform)
+  // the class terms of the needed pairs (c, w) of q, finished in LDS
+  auto pairs = [&](uint32_t mask, int row0, const real* q, int eq) {
+    mask = __builtin_amdgcn_readfirstlane(mask);  // the same word in every lane
+    while (mask) {
+      const int w = __builtin_ctz(mask);
+      mask &= mask - 1;
+      const int xr = xrow0 + w;
+      const real* xs = xv + (((size_t)xr * R + r) * 4) * pitch + pc;
+      real sv = (real)0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sv = fma(q[j], xs[(size_t)j * pitch], sv);
+      const int E = eq + xe[((size_t)xr * R + r) * pitch + pc];
+      const int buf = pair & 1;
+      xvl[(buf * R + r) * 64 + lane] = (double)sv;
+      xel[(buf * R + r) * 64 + lane] = E;
+      __syncthreads();
+      if (r == pair % R) {
+        const double* sq = xvl + buf * R * 64;
+        const int32_t* eqq = xel + buf * R * 64;
+        int m = kNoExp;
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr)
+          if (sq[rr * 64 + lane] != 0.0) m = max(m, eqq[rr * 64 + lane]);
+        real L = (real)0;
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+          const real v = (real)sq[rr * 64 + lane];
+          if (v != (real)0) L += (real)probs[rr] * ldexp(v, eqq[rr * 64 + lane] - m);
+        }
+        const double v = act ? (double)weights[p] * ((double)log(L) + (double)m * 0.6931471805599453) : 0.0;
+        const double S = wave_sum_d(v);
+        if (lane == 0) wsum[(size_t)row0 * n_blk + blockIdx.x] = S;
+      }
+      ++row0;
+      ++pair;
+    }
+  };
+  auto fetch = [&](int c, real* Dv, int* e) {
+    if (c < N) {
+      tipvec(tip_code(tips, stride, c, pc), Dv);
+      *e = 0;
+    } else {
+      const int row = rf[c];
+      const real* sv = part + (((size_t)row * R + r) * 4) * pitch + pc;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Dv[j] = sv[(size_t)j * pitch];
+      *e = ex[((size_t)row * R + r) * pitch + pc];
+    }
+  };
+  if (n_ops == 0) {  // block-uniform
+    // the pruned tree is the leaf x alone (p the root): only keeping the
+    // attachment, pi above x's tip through P(t_x)
+    if (!has_g && tm[M]) {
+      real D[4], PK[16], right[4], q[4];
+      tipvec(tip_code(tips, stride, tk[4], pc), D);
+      load_mat<real, R>(Pk + (((size_t)k * 2 + 1) * R + r) * 16, PK);
+      matvec4(PK, D, right);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) q[j] = (real)pi[j] * right[j];
+      pairs(tm[M], rb[M], q, 0);
+    }
+    return;
+  }
+  real Dc[8];
+  int Xc[2];
+  {
+    const int4 o = pg[0];
+    fetch(o.y, Dc, &Xc[0]);
+    fetch(o.z, Dc + 4, &Xc[1]);
+  }
+  real accM[4] = {(real)0, (real)0, (real)0, (real)0};
+  int accE = 0;
+  int sp = 0;
+  for_each_entry_ahead(pg, n_ops, [&](int i, const int4 op, const int4 on) {
+    real Dn[8];
+    int Xn[2] = {0, 0};
+    const bool more = i + 1 < n_ops;
+    if (more) {
+      fetch(on.y, Dn, &Xn[0]);
+      fetch(on.z, Dn + 4, &Xn[1]);
+    }
+    const int fl = op.w, src = fl & 3;
+    const int a = op.y, b = op.z;
+    real Mv[4];
+    int eM;
+    if (src == SRC_LEAF) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Mv[j] = (real)pi[j];
+      eM = 0;
+    } else if (src == SRC_ACC) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Mv[j] = accM[j];
+      eM = accE;
+    } else {
+      --sp;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Mv[j] = stk[((size_t)sp * 4 + j) * BD + tid];
+      eM = stke[(size_t)sp * BD + tid];
+    }
+    real PA[16], PB[16];
+    load_mat<real, R>(pfull(a), PA);
+    load_mat<real, R>(pfull(b), PB);
+    real ya[4], yb[4], Aa[4], Ab[4];
+    matvec4(PA, Dc, ya);
+    matvec4(PB, Dc + 4, yb);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      Aa[j] = Mv[j] * yb[j];
+      Ab[j] = Mv[j] * ya[j];
+    }
+    int eAa = eM + Xc[1], eAb = eM + Xc[0];
+    rescale4(Aa, &eAa);
+    rescale4(Ab, &eAb);
+    // keeping the attachment when p is the root: the pruned tree's root x
+    // (this first entry) joined to the subtree, pi above
+    if (i == 0 && !has_g && tm[M]) {
+      real D[4], right[4], q[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) D[j] = ya[j] * yb[j];
+      int eD = Xc[0] + Xc[1];
+      rescale4(D, &eD);
+      real PK[16];
+      load_mat<real, R>(Pk + (((size_t)k * 2 + 1) * R + r) * 16, PK);
+      matvec4(PK, D, right);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) q[j] = (real)pi[j] * right[j];
+      pairs(tm[M], rb[M], q, eD);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int c = t ? b : a;
+      const real* A = t ? Ab : Aa;
+      const real* D = t ? Dc + 4 : Dc;
+      const int eA = t ? eAb : eAa, eD = Xc[t];
+      if (tm[c]) {
+        real PH[16], left[4], right[4], q[4];
+        load_mat<real, R>(Ph + ((size_t)c * R + r) * 16, PH);
+        vecmat4(PH, A, left);
+        matvec4(PH, D, right);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q[j] = left[j] * right[j];
+        int eq = eA + eD;
+        rescale4(q, &eq);
+        pairs(tm[c], rb[c], q, eq);
+      }
+      // keeping the attachment: x's merged branch at its original split
+      if (c == x && has_g && tm[M]) {
+        real PU[16], PD[16], left[4], right[4], q[4];
+        load_mat<real, R>(Pk + (((size_t)k * 2 + 0) * R + r) * 16, PU);
+        load_mat<real, R>(Pk + (((size_t)k * 2 + 1) * R + r) * 16, PD);
+        vecmat4(PU, A, left);
+        matvec4(PD, D, right);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q[j] = left[j] * right[j];
+        int eq = eA + eD;
+        rescale4(q, &eq);
+        pairs(tm[M], rb[M], q, eq);
+      }
+    }
+    // messages to internal children: M_c = P_c^T A_c
+    real Ma[4], Mb[4];
+    int eMa = eAa, eMb = eAb;
+    if (!(fl & UP_A_LEAF)) {
+      vecmat4(PA, Aa, Ma);
+      rescale4(Ma, &eMa);
+    }
+    if (!(fl & UP_B_LEAF)) {
+      vecmat4(PB, Ab, Mb);
+      rescale4(Mb, &eMb);
+    }
+    if (fl & UP_PUSH) {
+      const bool cb = fl & UP_CONT_B;
+      const real* o = cb ? Ma : Mb;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) stk[((size_t)sp * 4 + j) * BD + tid] = o[j];
+      stke[(size_t)sp * BD + tid] = cb ? eMa : eMb;
+      ++sp;
+    }
+    if (!(fl & UP_A_LEAF) || !(fl & UP_B_LEAF)) {
+      const bool cb = fl & UP_CONT_B;
+      const real* c = cb ? Mb : Ma;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) accM[j] = c[j];
+      accE = cb ? eMb : eMa;
+    }
+    if (more) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Dc[j] = Dn[j];
+      Xc[0] = Xn[0];
+      Xc[1] = Xn[1];
+    }
+  });
+}
+
+// LDS of k_tbr_insert_w: the message stack and the double-buffered exchange
+size_t tbr_insert_lds(int R, size_t esz, int depth) {
+  return (size_t)std::max(depth, 1) * 64 * R * (4 * esz + 4) + (size_t)2 * R * 64 * 12;
+}
+// LDS of k_tbr_reroot: the outside-vector stack
+size_t tbr_reroot_lds(int R, size_t esz, int depth) {
+  return (size_t)std::max(depth, 1) * 64 * R * (4 * esz + 4);
+}
+
 // LDS of k_spr_insert_w: the message stack and the double-buffered exchange
 size_t insert_w_lds(int R, size_t esz, int depth) {
   return (size_t)std::max(depth, 1) * 64 * R * (4 * esz + 4) + (size_t)2 * 2 * R * 64 * 12;
@@ -2430,38 +2850,58 @@ struct IncrTree {
 struct IncrPlan {
   Plan base;  // stored post-order pass of the base tree
   size_t off_bl = 0, off_E = 0, off_Eh = 0, off_Ex = 0, off_tab = 0, off_P = 0, off_Ph = 0, off_Px = 0;
-  size_t off_part = 0, off_ex = 0, off_bsum = 0, off_lnl = 0, off_wsum = 0, off_blh = 0, total = 0;
+  size_t off_part = 0, off_ex = 0, off_bsum = 0, off_lnl = 0, off_wsum = 0, off_blh = 0;
+  size_t off_xv = 0, off_xe = 0, off_Pk = 0, total = 0;  // TBR: rerooted subtree vectors, keep matrices
   int64_t pitch = 0, n_waves = 0, n_blk = 0, pool_rows = 0;
-  int S = 0, lmax = 0;
+  int S = 0, lmax = 0, slots = 0, wmax = 0;
   size_t esz = 8;
 };
 
-IncrPlan incr_plan(const IncrTree& t, int R, int64_t n_pat, int32_t form, int32_t S) {
+// Workspace of a run: lengths and factors, the base tree's P, the partial
+// pool, the per-target sums; per-slot factors for `slots` pruned subtrees
+// (every pruned subtree a tree can have, M, for SPR).  TBR adds the
+// rerooted subtree vectors of a batch (S slots x wmax rerootings), the keep
+// matrices P(t_p), P(t_x) of each slot, and per-pair sums (max_pairs pairs
+// in a batch).
+IncrPlan incr_plan(const IncrTree& t, int R, int64_t n_pat, int32_t form, int32_t S, int slots = -1,
+                   int wmax = 0, int64_t max_pairs = 0) {
   IncrPlan pl;
   const int N = t.N, M = t.M, K = 4;
   pl.esz = (form & PHY_F32) ? 4 : 8;
   pl.S = S;
+  pl.slots = slots < 0 ? M : std::max(slots, 1);
+  pl.wmax = wmax;
   pl.lmax = std::max(t.lmax, 1);
   pl.pitch = ((n_pat + 255) / 256) * 256;
   pl.n_waves = pl.pitch * R / 64;
   pl.n_blk = pl.pitch / 64;
   pl.pool_rows = (int64_t)(N - 1) + (int64_t)S * pl.lmax;
   const size_t esz = pl.esz;
+  const size_t ns = (size_t)pl.slots;
   size_t o = 0;
-  pl.off_bl = o; o = align256(o + (size_t)3 * M * 8);  // full | half | merged x of each subtree
-  pl.off_blh = o; o = align256(o + (size_t)M * 8);     // host lengths of a one-shot call
+  // full | half | merged x of each slot | keep (t_p, t_x) of each slot
+  pl.off_bl = o; o = align256(o + ((size_t)2 * M + 3 * ns) * 8);
+  pl.off_blh = o; o = align256(o + (size_t)M * 8);  // host lengths of a one-shot call
   pl.off_E = o; o = align256(o + (size_t)M * R * K * esz);
   pl.off_Eh = o; o = align256(o + (size_t)M * R * K * esz);
-  pl.off_Ex = o; o = align256(o + (size_t)M * R * K * esz);
+  pl.off_Ex = o; o = align256(o + ns * R * K * esz);
   pl.off_tab = o; o = align256(o + (size_t)(2 * K * K + R * K) * esz);
   pl.off_P = o; o = align256(o + (size_t)M * R * K * K * esz);
   pl.off_Ph = o; o = align256(o + (size_t)M * R * K * K * esz);
-  pl.off_Px = o; o = align256(o + (size_t)M * R * K * K * esz);
+  pl.off_Px = o; o = align256(o + ns * R * K * K * esz);
   pl.off_part = o; o = align256(o + (size_t)pl.pool_rows * R * K * pl.pitch * esz);
   pl.off_ex = o; o = align256(o + (size_t)pl.pool_rows * R * pl.pitch * 4);
   pl.off_bsum = o; o = align256(o + (size_t)(pl.pitch / 64) * 8);
   pl.off_lnl = o; o = align256(o + 8);
-  pl.off_wsum = o; o = align256(o + (size_t)S * M * std::max(pl.n_waves, pl.n_blk) * 8);
+  // per-target sums: SPR one row per (subtree of a batch, target), TBR one
+  // per needed pair of a batch
+  const size_t rows = wmax > 0 ? (size_t)max_pairs * pl.n_blk : (size_t)S * M * std::max(pl.n_waves, pl.n_blk);
+  pl.off_wsum = o; o = align256(o + rows * 8);
+  if (wmax > 0) {
+    pl.off_xv = o; o = align256(o + (size_t)S * wmax * R * K * pl.pitch * esz);
+    pl.off_xe = o; o = align256(o + (size_t)S * wmax * R * pl.pitch * 4);
+    pl.off_Pk = o; o = align256(o + 2 * ns * R * K * K * esz);
+  }
   pl.total = o;
   return pl;
 }
@@ -2549,9 +2989,11 @@ bool incr_group(const IncrTree& t, int64_t n_moves, const int32_t* moves, int S,
 }
 
 // lengths of a run from the device blen: full | half | merged x of each
-// pruned subtree (t_x + t_p, 0 when p is the root)
+// pruned subtree (t_x + t_p, 0 when p is the root) | with keep (TBR), the
+// original split of x's branch of each pruned subtree (t_p, t_x; kx holds x
+// also when p is the root)
 __global__ void k_incr_lengths(const double* __restrict__ blen, int M, const int32_t* __restrict__ xp,
-                               int n_sub, double* __restrict__ out) {
+                               int n_sub, int slots, const int32_t* __restrict__ kx, double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < M) {
     out[i] = blen[i];
@@ -2560,6 +3002,10 @@ __global__ void k_incr_lengths(const double* __restrict__ blen, int M, const int
   if (i < n_sub) {
     const int x = xp[2 * i], p = xp[2 * i + 1];
     out[2 * M + i] = x >= 0 ? blen[x] + blen[p] : 0.0;
+    if (kx) {
+      out[2 * M + slots + 2 * i] = blen[p];
+      out[2 * M + slots + 2 * i + 1] = blen[kx[i]];
+    }
   }
 }
 
@@ -2578,9 +3024,12 @@ struct phy_spr_plan {
   phy_topo* base = nullptr;               // host programs; device copies in dmem
   std::vector<int64_t> mv_start;          // first grouped move of each subtree (n_sub + 1)
   std::vector<int32_t> batch_depth;       // message stack depth of each batch
+  // TBR: the n_sub slots are (pruned subtree, up to 32 rerootings) pairs
+  bool tbr = false;
+  std::vector<int32_t> batch_rdepth;      // stack depth of the rerooting pass of each batch
   char* dmem = nullptr;
   size_t o_dprog = 0, o_dfifo = 0, o_prog = 0, o_meta = 0, o_ref = 0, o_ops = 0, o_xp = 0, o_rows = 0,
-         o_idx = 0;
+         o_idx = 0, o_tprog = 0, o_tmeta = 0, o_wslot = 0, o_tmask = 0, o_rbase = 0, o_kx = 0;
   ~phy_spr_plan() {
     if (base) {
       base->d_down_prog = nullptr;
@@ -2624,8 +3073,9 @@ int incr_run(phy_ctx* ctx, const phy_spr_plan* sp, const phy_model* m, const uin
 
   // lengths, eigen factors and P matrices of every branch and merged branch
   const int nl = std::max(M, n_sub);
+  const auto* d_kx = sp->tbr ? reinterpret_cast<const int32_t*>(dm + sp->o_kx) : nullptr;
   hipLaunchKernelGGL(k_incr_lengths, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, ctx->stream, d_blen, M,
-                     d_xp, n_sub, bl);
+                     d_xp, n_sub, pl.slots, d_kx, bl);
   PHY_LAUNCHED(ctx);
   auto eig = [&](const double* lens, int nb, real* out) {
     const int64_t ne = std::max<int64_t>((int64_t)nb * R * K, 32 + 4 * R);
@@ -2648,6 +3098,10 @@ int incr_run(phy_ctx* ctx, const phy_spr_plan* sp, const phy_model* m, const uin
     PHY_LAUNCHED(ctx);
     pmat(bl + 2 * M, n_sub, Px);
     PHY_LAUNCHED(ctx);
+    if (sp->tbr) {
+      pmat(bl + 2 * M + pl.slots, 2 * n_sub, reinterpret_cast<real*>(work + pl.off_Pk));
+      PHY_LAUNCHED(ctx);
+    }
   }
   // the base tree's partials: stored post-order pass into pool rows 0 .. N-2
   {
@@ -2675,6 +3129,41 @@ int incr_run(phy_ctx* ctx, const phy_spr_plan* sp, const phy_model* m, const uin
     }
     PHY_LAUNCHED(ctx);
     const int4* prog = d_prog + (size_t)b0 * (N - 1);
+    const int64_t m0 = sp->mv_start[b0], m1 = sp->mv_start[b0 + Sb];
+    if (sp->tbr) {
+      const int32_t* tmeta = reinterpret_cast<const int32_t*>(dm + sp->o_tmeta) + (size_t)b0 * kIncrMeta;
+      real* xv = reinterpret_cast<real*>(work + pl.off_xv);
+      int32_t* xe = reinterpret_cast<int32_t*>(work + pl.off_xe);
+      const int rdepth = std::max(sp->batch_rdepth[bi], 1);
+      {
+        KernelSpan span(ctx, "tbr_reroot");
+        hipLaunchKernelGGL((k_tbr_reroot<real, R>), dim3((unsigned)pl.n_blk, (unsigned)Sb), dim3(64 * R),
+                           tbr_reroot_lds(R, sizeof(real), rdepth), ctx->stream,
+                           reinterpret_cast<const int4*>(dm + sp->o_tprog) + (size_t)b0 * (N - 1),
+                           (int64_t)(N - 1), tmeta,
+                           reinterpret_cast<const int32_t*>(dm + sp->o_wslot) + (size_t)b0 * M, Pf, Ph, tips,
+                           stride, n_pat, pl.pitch, part, ex, xv, xe, N, M, rdepth);
+      }
+      PHY_LAUNCHED(ctx);
+      {
+        KernelSpan span(ctx, "tbr_insert");
+        hipLaunchKernelGGL((k_tbr_insert_w<real, R>), dim3((unsigned)pl.n_blk, (unsigned)Sb), dim3(64 * R),
+                           tbr_insert_lds(R, sizeof(real), depth), ctx->stream, prog, (int64_t)(N - 1), meta,
+                           tmeta, ref, reinterpret_cast<const uint32_t*>(dm + sp->o_tmask) + (size_t)b0 * (M + 1),
+                           reinterpret_cast<const int32_t*>(dm + sp->o_rbase) + (size_t)b0 * (M + 1), Pf, Ph,
+                           Px + (size_t)b0 * R * 16,
+                           reinterpret_cast<const real*>(work + pl.off_Pk) + (size_t)b0 * 2 * R * 16, xv, xe,
+                           pl.wmax, d_pi, d_probs, tips, stride, n_pat, pl.pitch, part, ex, weights, wsum,
+                           pl.n_blk, N, M, depth);
+      }
+      PHY_LAUNCHED(ctx);
+      if (m1 > m0) {
+        hipLaunchKernelGGL(k_reduce_gather, dim3((unsigned)(m1 - m0)), dim3(256), 0, ctx->stream, wsum, pl.n_blk,
+                           d_rows + m0, d_idx + m0, d_lnl);
+        PHY_LAUNCHED(ctx);
+      }
+      continue;
+    }
     const size_t wl = insert_w_lds(R, sizeof(real), depth);
     int64_t n_slot;
     if (wl <= kLdsMax) {
@@ -2696,7 +3185,6 @@ int incr_run(phy_ctx* ctx, const phy_spr_plan* sp, const phy_model* m, const uin
                          ex, weights, wsum, pl.n_waves, N, M, depth);
     }
     PHY_LAUNCHED(ctx);
-    const int64_t m0 = sp->mv_start[b0], m1 = sp->mv_start[b0 + Sb];
     if (m1 > m0) {
       hipLaunchKernelGGL(k_reduce_gather, dim3((unsigned)(m1 - m0)), dim3(256), 0, ctx->stream, wsum, n_slot,
                          d_rows + m0, d_idx + m0, d_lnl);
@@ -2811,6 +3299,254 @@ int plan_create(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat, i
   return PHY_OK;
 }
 
+// TBR moves (s, w, v) grouped into slots: a pruned subtree with up to 32 of
+// its rerootings (w = s: none), ascending s.  A slot scores every needed
+// (target, rerooting) pair; its pairs take consecutive wsum rows of the
+// batch, by target (0 .. M - 1, then M for keeping the attachment) and
+// rerooting bit.  Validation follows phy_tbr_apply: w = s or w under s with
+// s internal; v = -1 (keep) only with w != s; any other v a valid SPR target.
+struct TbrGroups {
+  int n_sub = 0, wmax = 1;
+  int64_t max_pairs = 1;
+  std::vector<int32_t> slot_s;          // subtree of each slot
+  std::vector<int64_t> slot_first;      // first (s, w) of each slot in uniq (n_sub + 1)
+  std::vector<int64_t> uniq;            // distinct s * M + w, ascending
+  std::vector<uint32_t> tmask;          // [slot][M + 1] rerooting bits of the needed pairs
+  std::vector<int32_t> rbase;           // [slot][M + 1] wsum row of the first such pair
+  std::vector<int64_t> mv_start, rows, idx;  // moves grouped by slot: wsum row, caller's index
+};
+
+// 0, or 1 with *bad the first invalid move, or 2 for a batch of more than
+// 2^31 pairs
+int tbr_group(const IncrTree& t, int64_t n_moves, const int32_t* moves, int S, TbrGroups* g, int64_t* bad) {
+  const int32_t N = t.N, M = t.M;
+  std::vector<int64_t> sw(n_moves);
+  for (int64_t i = 0; i < n_moves; ++i) {
+    const int32_t s = moves[3 * i], w = moves[3 * i + 1], v = moves[3 * i + 2];
+    bool ok = s >= 0 && s < M && s != t.root && w >= 0 && w < M;
+    if (ok && w != s) ok = s >= N && t.below(s, w);
+    if (ok) {
+      if (v == -1) {
+        ok = w != s;
+      } else {
+        const int32_t p = t.parent[s], x = t.sibling(s);
+        const int32_t new_root = (p == t.root) ? x : t.root;
+        ok = v >= 0 && v < M && !t.below(s, v) && v != p && v != x && v != new_root;
+      }
+    }
+    if (!ok) {
+      *bad = i;
+      return 1;
+    }
+    sw[i] = (int64_t)s * M + w;
+  }
+  g->uniq = sw;
+  std::sort(g->uniq.begin(), g->uniq.end());
+  g->uniq.erase(std::unique(g->uniq.begin(), g->uniq.end()), g->uniq.end());
+  // slots: runs of at most 32 rerootings of one subtree
+  g->slot_s.clear();
+  g->slot_first.clear();
+  for (size_t j = 0; j < g->uniq.size(); ++j) {
+    const int32_t s = (int32_t)(g->uniq[j] / M);
+    if (g->slot_s.empty() || g->slot_s.back() != s || (int64_t)j - g->slot_first.back() == 32) {
+      g->slot_s.push_back(s);
+      g->slot_first.push_back((int64_t)j);
+    }
+  }
+  const int n_sub = g->n_sub = (int)g->slot_s.size();
+  g->slot_first.push_back((int64_t)g->uniq.size());
+  g->wmax = 1;
+  for (int k = 0; k < n_sub; ++k) g->wmax = std::max(g->wmax, (int)(g->slot_first[k + 1] - g->slot_first[k]));
+  std::vector<int32_t> pair_slot(g->uniq.size());
+  for (int k = 0; k < n_sub; ++k)
+    for (int64_t j = g->slot_first[k]; j < g->slot_first[k + 1]; ++j) pair_slot[j] = k;
+  // needed pairs per slot and target; their rows
+  const size_t M1 = (size_t)M + 1;
+  g->tmask.assign((size_t)std::max(n_sub, 1) * M1, 0);
+  std::vector<int32_t> mv_slot(n_moves), mv_bit(n_moves), mv_c(n_moves);
+  for (int64_t i = 0; i < n_moves; ++i) {
+    const int64_t j = std::lower_bound(g->uniq.begin(), g->uniq.end(), sw[i]) - g->uniq.begin();
+    const int k = pair_slot[j];
+    const int bit = (int)(j - g->slot_first[k]);
+    const int32_t v = moves[3 * i + 2];
+    const int c = v < 0 ? M : v;
+    g->tmask[(size_t)k * M1 + c] |= 1u << bit;
+    mv_slot[i] = k;
+    mv_bit[i] = bit;
+    mv_c[i] = c;
+  }
+  g->rbase.assign((size_t)std::max(n_sub, 1) * M1, 0);
+  g->max_pairs = 1;
+  for (int b0 = 0; b0 < n_sub; b0 += S) {
+    int64_t cur = 0;
+    for (int k = b0; k < std::min(n_sub, b0 + S); ++k)
+      for (size_t c = 0; c < M1; ++c) {
+        g->rbase[(size_t)k * M1 + c] = (int32_t)cur;
+        cur += __builtin_popcount(g->tmask[(size_t)k * M1 + c]);
+      }
+    if (cur > 0x7fffffff) return 2;
+    g->max_pairs = std::max(g->max_pairs, cur);
+  }
+  // moves grouped by slot, with their wsum rows
+  g->mv_start.assign(n_sub + 1, 0);
+  for (int64_t i = 0; i < n_moves; ++i) ++g->mv_start[mv_slot[i] + 1];
+  for (int k = 0; k < n_sub; ++k) g->mv_start[k + 1] += g->mv_start[k];
+  g->rows.assign(n_moves, 0);
+  g->idx.assign(n_moves, 0);
+  std::vector<int64_t> fill(g->mv_start.begin(), g->mv_start.end() - 1);
+  for (int64_t i = 0; i < n_moves; ++i) {
+    const int k = mv_slot[i];
+    const uint32_t mk = g->tmask[(size_t)k * M1 + mv_c[i]];
+    const int64_t j = fill[k]++;
+    g->rows[j] = g->rbase[(size_t)k * M1 + mv_c[i]] + __builtin_popcount(mk & ((1u << mv_bit[i]) - 1u));
+    g->idx[j] = i;
+  }
+  return 0;
+}
+
+int tbr_plan_create(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat, int32_t N,
+                    const int32_t* children, int32_t root, int64_t n_moves, const int32_t* moves, int32_t s_batch,
+                    phy_spr_plan** out) {
+  std::unique_ptr<phy_spr_plan> sp(new phy_spr_plan());
+  if (!sp->t.load(N, children, root)) return set_err(ctx, PHY_EINVAL, "tbr incremental: not a rooted binary tree");
+  const IncrTree& t = sp->t;
+  const int M = t.M, S = s_batch, R = m->R;
+  sp->tbr = true;
+  sp->form = form;
+  sp->R = R;
+  sp->n_pat = n_pat;
+  sp->n_moves = n_moves;
+  TbrGroups grp;
+  int64_t bad = -1;
+  const int gr = tbr_group(t, n_moves, moves, S, &grp, &bad);
+  if (gr == 1) return set_err(ctx, PHY_EINVAL, "tbr incremental: invalid move " + std::to_string(bad));
+  if (gr == 2) return set_err(ctx, PHY_ERANGE, "tbr incremental: too many pairs in a batch");
+  const int n_sub = sp->n_sub = grp.n_sub, wmax = grp.wmax;
+  const std::vector<int32_t>& slot_s = grp.slot_s;
+  const std::vector<int64_t>& slot_first = grp.slot_first;
+  const std::vector<int64_t>& uniq = grp.uniq;
+  const std::vector<uint32_t>& tmask = grp.tmask;
+  const std::vector<int32_t>& rbase = grp.rbase;
+  const std::vector<int64_t>& rows = grp.rows;
+  const std::vector<int64_t>& idx = grp.idx;
+  sp->mv_start = grp.mv_start;
+  const int64_t max_pairs = grp.max_pairs;
+  sp->pl = incr_plan(t, R, n_pat, form, S, n_sub, wmax, max_pairs);
+  if (phy_topo_create(nullptr, N, children, root, &sp->base) != PHY_OK)
+    return set_err(ctx, PHY_EINVAL, "tbr incremental: base topology");
+  sp->pl.base = make_plan(sp->base, m, n_pat, form & ~PHY_FUSED, 0);
+  const int lmax = sp->pl.lmax;
+  // per slot: the repair and insertion of its pruned tree, the pre-order
+  // program of its subtree (shared by the slots of one subtree)
+  std::vector<IncrSub> sub(n_sub);
+  std::vector<std::vector<int32_t>> tprog(n_sub);
+  std::vector<int32_t> tdepth(n_sub, 0);
+  {
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned w = 0; w < std::min<unsigned>(nt, std::max(n_sub, 1)); ++w)
+      th.emplace_back([&, w] {
+        for (int k = (int)w; k < n_sub; k += (int)nt) {
+          incr_prepare(t, slot_s[k], k % S, lmax, &sub[k]);
+          if (slot_s[k] >= N) phy_compile_up_program(N, t.ch, t.size, slot_s[k], &tprog[k], &tdepth[k]);
+        }
+      });
+    for (auto& x : th) x.join();
+  }
+  const size_t ns = (size_t)std::max(n_sub, 1);
+  std::vector<int32_t> hprog(ns * (N - 1) * 4, 0), hmeta(ns * kIncrMeta, 0), href(ns * M, 0),
+      hops(ns * lmax * 4, 0), hxp(ns * 2, 0), htprog(ns * (N - 1) * 4, 0), htmeta(ns * kIncrMeta, 0),
+      hwslot(ns * M, -1), hkx(ns, 0);
+  sp->batch_depth.assign((n_sub + S - 1) / S, 1);
+  sp->batch_rdepth.assign((n_sub + S - 1) / S, 1);
+  for (int k = 0; k < n_sub; ++k) {
+    const IncrSub& q = sub[k];
+    const int32_t s = slot_s[k];
+    std::copy(q.prog.begin(), q.prog.end(), hprog.begin() + (size_t)k * (N - 1) * 4);
+    std::copy(q.ref.begin(), q.ref.end(), href.begin() + (size_t)k * M);
+    std::copy(q.ops.begin(), q.ops.end(), hops.begin() + (size_t)k * lmax * 4);
+    std::copy(tprog[k].begin(), tprog[k].end(), htprog.begin() + (size_t)k * (N - 1) * 4);
+    int32_t* mk = &hmeta[(size_t)k * kIncrMeta];
+    mk[0] = (k % S) * lmax;
+    mk[1] = (int32_t)q.ops.size() / 4;
+    mk[2] = q.x;
+    mk[3] = s;
+    mk[4] = q.n_ops;
+    mk[5] = q.depth;
+    int32_t* tk = &htmeta[(size_t)k * kIncrMeta];
+    // a slot whose only rerooting is w = s needs no pass over its subtree
+    const bool only_s = slot_first[k + 1] - slot_first[k] == 1 && uniq[slot_first[k]] % M == s;
+    tk[0] = only_s ? 0 : (int32_t)tprog[k].size() / 4;
+    tk[1] = s;
+    tk[3] = tdepth[k];
+    tk[4] = t.sibling(s);
+    tk[5] = t.parent[t.parent[s]] >= 0 ? 1 : 0;
+    hxp[2 * k] = q.x;
+    hxp[2 * k + 1] = q.p;
+    hkx[k] = t.sibling(s);
+    for (int64_t j = slot_first[k]; j < slot_first[k + 1]; ++j)
+      hwslot[(size_t)k * M + (size_t)(uniq[j] % M)] = (k % S) * wmax + (int32_t)(j - slot_first[k]);
+    sp->batch_depth[k / S] = std::max(sp->batch_depth[k / S], q.depth);
+    if (!only_s) sp->batch_rdepth[k / S] = std::max(sp->batch_rdepth[k / S], tdepth[k]);
+  }
+  {
+    int lds_max = 64 * 1024, dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    const size_t esz = sp->pl.esz;
+    for (size_t b = 0; b < sp->batch_depth.size(); ++b)
+      if (tbr_insert_lds(R, esz, sp->batch_depth[b]) > (size_t)lds_max ||
+          tbr_reroot_lds(R, esz, sp->batch_rdepth[b]) > (size_t)lds_max)
+        return set_err(ctx, PHY_ERANGE, "tbr incremental: LDS stack too deep");
+  }
+  std::vector<int32_t> fifo(phy_fifo_padded(N), sp->base->down_fifo[0]);
+  std::copy(sp->base->down_fifo.begin(), sp->base->down_fifo.end(), fifo.begin());
+  size_t o = 0;
+  auto put = [&](size_t bytes) {
+    const size_t at = o;
+    o = align256(o + std::max<size_t>(bytes, 8));
+    return at;
+  };
+  sp->o_dprog = put(sp->base->down_prog.size() * 4);
+  sp->o_dfifo = put(fifo.size() * 4);
+  sp->o_prog = put(hprog.size() * 4);
+  sp->o_meta = put(hmeta.size() * 4);
+  sp->o_ref = put(href.size() * 4);
+  sp->o_ops = put(hops.size() * 4);
+  sp->o_xp = put(hxp.size() * 4);
+  sp->o_rows = put(rows.size() * 8);
+  sp->o_idx = put(idx.size() * 8);
+  sp->o_tprog = put(htprog.size() * 4);
+  sp->o_tmeta = put(htmeta.size() * 4);
+  sp->o_wslot = put(hwslot.size() * 4);
+  sp->o_tmask = put(tmask.size() * 4);
+  sp->o_rbase = put(rbase.size() * 4);
+  sp->o_kx = put(hkx.size() * 4);
+  PHY_HIP(ctx, hipMalloc((void**)&sp->dmem, o));
+  auto up = [&](size_t off, const void* src, size_t bytes) {
+    return bytes ? hipMemcpy(sp->dmem + off, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+  };
+  PHY_HIP(ctx, up(sp->o_dprog, sp->base->down_prog.data(), sp->base->down_prog.size() * 4));
+  PHY_HIP(ctx, up(sp->o_dfifo, fifo.data(), fifo.size() * 4));
+  PHY_HIP(ctx, up(sp->o_prog, hprog.data(), hprog.size() * 4));
+  PHY_HIP(ctx, up(sp->o_meta, hmeta.data(), hmeta.size() * 4));
+  PHY_HIP(ctx, up(sp->o_ref, href.data(), href.size() * 4));
+  PHY_HIP(ctx, up(sp->o_ops, hops.data(), hops.size() * 4));
+  PHY_HIP(ctx, up(sp->o_xp, hxp.data(), hxp.size() * 4));
+  PHY_HIP(ctx, up(sp->o_rows, rows.data(), rows.size() * 8));
+  PHY_HIP(ctx, up(sp->o_idx, idx.data(), idx.size() * 8));
+  PHY_HIP(ctx, up(sp->o_tprog, htprog.data(), htprog.size() * 4));
+  PHY_HIP(ctx, up(sp->o_tmeta, htmeta.data(), htmeta.size() * 4));
+  PHY_HIP(ctx, up(sp->o_wslot, hwslot.data(), hwslot.size() * 4));
+  PHY_HIP(ctx, up(sp->o_tmask, tmask.data(), tmask.size() * 4));
+  PHY_HIP(ctx, up(sp->o_rbase, rbase.data(), rbase.size() * 4));
+  PHY_HIP(ctx, up(sp->o_kx, hkx.data(), hkx.size() * 4));
+  sp->base->d_down_prog = reinterpret_cast<int4*>(sp->dmem + sp->o_dprog);
+  sp->base->d_down_fifo = reinterpret_cast<int32_t*>(sp->dmem + sp->o_dfifo);
+  *out = sp.release();
+  return PHY_OK;
+}
+
 int check_plan_model(phy_ctx* ctx, const phy_model* m) {
   if (m->K != 4) return set_err(ctx, PHY_EINVAL, "spr incremental: 4-state models only");
   if (m->R != 1 && m->R != 2 && m->R != 4 && m->R != 8)
@@ -2901,6 +3637,71 @@ int phy_lk_spr_scores_incr(phy_ctx* ctx, const phy_model* m, int32_t form, int64
     return PHY_OK;
   } catch (...) {
     return set_err(ctx, PHY_ENOMEM, "spr incremental: host allocation or thread start failed");
+  }
+}
+
+int phy_lk_tbr_incr_work_size(int32_t N, const int32_t* children, int32_t root, const phy_model* m,
+                              int64_t n_pat, int32_t form, int64_t n_moves, const int32_t* moves, int32_t s_batch,
+                              size_t* bytes) {
+  if (!m || !bytes || !children || N < 3 || n_pat < 1 || s_batch < 1 || n_moves < 0 || (n_moves > 0 && !moves))
+    return PHY_EINVAL;
+  try {
+    IncrTree t;
+    if (!t.load(N, children, root)) return PHY_EINVAL;
+    TbrGroups g;
+    int64_t bad = -1;
+    const int gr = tbr_group(t, n_moves, moves, s_batch, &g, &bad);
+    if (gr) return gr == 1 ? PHY_EINVAL : PHY_ERANGE;
+    *bytes = incr_plan(t, m->R, n_pat, form, s_batch, g.n_sub, g.wmax, g.max_pairs).total;
+    return PHY_OK;
+  } catch (...) {
+    return PHY_ENOMEM;
+  }
+}
+
+int phy_tbr_plan_create(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat, int32_t N,
+                        const int32_t* children, int32_t root, int64_t n_moves, const int32_t* moves,
+                        int32_t s_batch, phy_spr_plan** out) {
+  if (!ctx || !m || !out) return PHY_EINVAL;
+  *out = nullptr;
+  if (N < 3 || n_pat < 1 || s_batch < 1 || n_moves < 0 || !children || (n_moves > 0 && !moves))
+    return set_err(ctx, PHY_EINVAL, "tbr plan: bad argument");
+  int rc = check_plan_model(ctx, m);
+  if (rc) return rc;
+  try {
+    return tbr_plan_create(ctx, m, form, n_pat, N, children, root, n_moves, moves, s_batch, out);
+  } catch (...) {
+    return set_err(ctx, PHY_ENOMEM, "tbr plan: host allocation or thread start failed");
+  }
+}
+
+int phy_lk_tbr_scores_incr(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat,
+                           const uint64_t* tips, int64_t stride, const uint32_t* weights, int32_t N,
+                           const int32_t* children, int32_t root, const double* blen, int64_t n_moves,
+                           const int32_t* moves, int32_t s_batch, double* d_lnl, double* d_base_lnl,
+                           void* d_work, size_t work_bytes) {
+  int rc = check_scores_args(ctx, m, n_pat, tips, stride, weights, N, children, blen, n_moves, moves,
+                             s_batch, d_lnl, d_work, root, "tbr incremental");
+  if (rc) return rc;
+  rc = check_plan_model(ctx, m);
+  if (rc) return rc;
+  try {
+    phy_spr_plan* sp = nullptr;
+    rc = tbr_plan_create(ctx, m, form, n_pat, N, children, root, n_moves, moves, s_batch, &sp);
+    if (rc) return rc;
+    std::unique_ptr<phy_spr_plan> hold(sp);
+    if (work_bytes < sp->pl.total) return set_err(ctx, PHY_EINVAL, "tbr incremental: workspace too small");
+    char* work = static_cast<char*>(d_work);
+    double* dbl = reinterpret_cast<double*>(work + sp->pl.off_blh);
+    PHY_HIP(ctx, hipMemcpyAsync(dbl, blen, (size_t)sp->t.M * 8, hipMemcpyHostToDevice, ctx->stream));
+    rc = incr_dispatch(ctx, sp, m, tips, stride, weights, dbl, d_lnl, d_base_lnl, work);
+    // the plan's device arrays are freed on return
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (rc) return rc;
+    if (e != hipSuccess) return set_err(ctx, PHY_EDEVICE, std::string("tbr incremental: ") + hipGetErrorString(e));
+    return PHY_OK;
+  } catch (...) {
+    return set_err(ctx, PHY_ENOMEM, "tbr incremental: host allocation or thread start failed");
   }
 }
 

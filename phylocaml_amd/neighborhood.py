@@ -166,31 +166,73 @@ class Neighborhood:
 
 
 class Scorer:
-    """Exact lnL of neighbours: many candidate trees per fused pruning launch
-    (phy_lk_spr_scores for pairs, phy_lk_tbr_scores for triples)."""
+    """Exact lnL of neighbours.  Two exact scorers: one candidate tree per
+    move, many per fused pruning launch (phy_lk_spr_scores for pairs,
+    phy_lk_tbr_scores for triples), or the incremental one
+    (phy_lk_spr_scores_incr / phy_lk_tbr_scores_incr: per pruned subtree one
+    repair of the base tree's partials, one pass over the subtree for its
+    rerootings and one over the pruned tree for every regraft).
+    ``incremental=None`` picks the incremental scorer for 4-state models
+    with 1, 2, 4 or 8 rate classes when the moves average at least four per
+    pruned subtree."""
 
     def __init__(self, ctx: D.Context, dm, n_leaves: int, tips, stride: int, weights,
-                 n_patterns: int, form: int = 0, chunk: int = 2048):
+                 n_patterns: int, form: int = 0, chunk: int = 2048,
+                 incremental: bool | None = None, s_batch: int = 16):
         torch = __import__("torch")
         self.ctx, self.dm, self.N = ctx, dm, int(n_leaves)
         self.tips, self.stride, self.weights = tips, int(stride), weights
         self.n_patterns, self.form, self.chunk = int(n_patterns), int(form), int(chunk)
+        self.incremental, self.s_batch = incremental, int(s_batch)
         nb = C.c_size_t()
         check(lib.phy_lk_spr_work_size(self.N, dm.handle, self.n_patterns, self.form, self.chunk,
                                        C.byref(nb)))
         self.work_bytes = nb.value
-        self.work = torch.empty((self.work_bytes,), dtype=torch.uint8,
-                                device=f"cuda:{ctx.device}")
+        dev = f"cuda:{ctx.device}"
+        self.work = torch.empty((self.work_bytes,), dtype=torch.uint8, device=dev)
+        self.iwork = torch.empty((0,), dtype=torch.uint8, device=dev)
+        self.base_lnl = torch.zeros((1,), dtype=torch.float64, device=dev)
+
+    def uses_incremental(self, mv: np.ndarray) -> bool:
+        if self.incremental is not None:
+            return bool(self.incremental)
+        if getattr(self.dm, "K", 4) != 4 or getattr(self.dm, "R", 4) not in (1, 2, 4, 8):
+            return False
+        return len(mv) > 0 and len(mv) >= 4 * len(np.unique(mv[:, 0]))
 
     def score(self, children, root: int, blen, mv: np.ndarray):
-        """Device float64 tensor of lnL for each delta of mv [n, 2] or [n, 3]."""
+        """Device float64 tensor of lnL for each delta of mv [n, 2] or [n, 3]
+        (with the incremental scorer, the base tree's lnL in self.base_lnl)."""
         torch = __import__("torch")
         mv = np.ascontiguousarray(mv, np.int32)
         ch = np.ascontiguousarray(children, np.int32)
         bl = np.ascontiguousarray(blen, np.float64)
         out = torch.empty((max(len(mv), 1),), dtype=torch.float64,
                           device=f"cuda:{self.ctx.device}")
-        fn = lib.phy_lk_tbr_scores if mv.ndim == 2 and mv.shape[1] == 3 else lib.phy_lk_spr_scores
+        tbr = mv.ndim == 2 and mv.shape[1] == 3
+        if len(mv) and self.uses_incremental(mv):
+            nb = C.c_size_t()
+            if tbr:
+                check(lib.phy_lk_tbr_incr_work_size(
+                    self.N, ptr(ch, C.c_int32), int(root), self.dm.handle, self.n_patterns,
+                    self.form, len(mv), ptr(mv, C.c_int32), self.s_batch, C.byref(nb)),
+                    None, "tbr incremental work size")
+            else:
+                check(lib.phy_lk_spr_incr_work_size(
+                    self.N, ptr(ch, C.c_int32), int(root), self.dm.handle, self.n_patterns,
+                    self.form, self.s_batch, C.byref(nb)), None, "spr incremental work size")
+            if self.iwork.numel() < nb.value:
+                self.iwork = torch.empty((nb.value,), dtype=torch.uint8,
+                                         device=f"cuda:{self.ctx.device}")
+            fn = lib.phy_lk_tbr_scores_incr if tbr else lib.phy_lk_spr_scores_incr
+            self.ctx.check(fn(self.ctx.handle, self.dm.handle, self.form, self.n_patterns,
+                              D.dptr(self.tips), self.stride, D.dptr(self.weights), self.N,
+                              ptr(ch, C.c_int32), int(root), ptr(bl, C.c_double), len(mv),
+                              ptr(mv, C.c_int32), self.s_batch, D.dptr(out),
+                              D.dptr(self.base_lnl), D.dptr(self.iwork), self.iwork.numel()),
+                           "neighbour scores (incremental)")
+            return out[: len(mv)]
+        fn = lib.phy_lk_tbr_scores if tbr else lib.phy_lk_spr_scores
         self.ctx.check(fn(self.ctx.handle, self.dm.handle, self.form, self.n_patterns,
                           D.dptr(self.tips), self.stride, D.dptr(self.weights), self.N,
                           ptr(ch, C.c_int32), int(root), ptr(bl, C.c_double), len(mv),

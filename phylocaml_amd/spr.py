@@ -112,11 +112,12 @@ class IncrementalScorer:
 
 class SprPlan:
     """Incremental scoring of a fixed move list on a fixed topology,
-    prepared once (phy_spr_plan_create) and run on device branch lengths
-    (phy_spr_plan_run): a run issues kernels only, so ``capture`` records it
-    in a HIP graph that rescoring replays after the lengths (written into
-    the same device tensor) change.  Capture needs ``ctx`` on a side
-    stream."""
+    prepared once (phy_spr_plan_create for SPR pairs [n, 2],
+    phy_tbr_plan_create for TBR triples [n, 3]) and run on device branch
+    lengths (phy_spr_plan_run): a run issues kernels only, so ``capture``
+    records it in a HIP graph that rescoring replays after the lengths
+    (written into the same device tensor) change.  Capture needs ``ctx`` on
+    a side stream."""
 
     def __init__(self, ctx: D.Context, dm, n_leaves: int, tips, stride: int, weights,
                  n_patterns: int, children, root: int, mv: np.ndarray, form: int = 0,
@@ -124,13 +125,16 @@ class SprPlan:
         torch = __import__("torch")
         self.ctx, self.dm, self.N = ctx, dm, int(n_leaves)
         self.tips, self.stride, self.weights = tips, int(stride), weights
-        self.mv = np.ascontiguousarray(mv, np.int32).reshape(-1, 2)
+        mv = np.ascontiguousarray(mv, np.int32)
+        tbr = mv.ndim == 2 and mv.shape[1] == 3
+        self.mv = mv if tbr else mv.reshape(-1, 2)
         ch = np.ascontiguousarray(children, np.int32)
         h = C.c_void_p()
-        ctx.check(lib.phy_spr_plan_create(ctx.handle, dm.handle, int(form), int(n_patterns), self.N,
-                                          ptr(ch, C.c_int32), int(root), len(self.mv),
-                                          ptr(self.mv, C.c_int32), int(s_batch), C.byref(h)),
-                  "spr_plan_create")
+        create = lib.phy_tbr_plan_create if tbr else lib.phy_spr_plan_create
+        ctx.check(create(ctx.handle, dm.handle, int(form), int(n_patterns), self.N,
+                         ptr(ch, C.c_int32), int(root), len(self.mv), ptr(self.mv, C.c_int32),
+                         int(s_batch), C.byref(h)),
+                  "tbr_plan_create" if tbr else "spr_plan_create")
         self.handle = h
         nb = C.c_size_t()
         check(lib.phy_spr_plan_work_size(h, C.byref(nb)))
