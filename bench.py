@@ -460,6 +460,79 @@ def bench_spr(torch, ctx, cpu_baseline: bool):
     return res
 
 
+def bench_tbr(torch, ctx):
+    """Exact scores of TBR neighbours on the same instance (256-taxon tree,
+    50,000 GTR+G4 patterns): the neighbourhood within join and reroot
+    distance 3 through the per-candidate scorer (an even sample of 2,048
+    moves) and whole through the incremental scorer, which must agree; then
+    the neighbourhood with unlimited join distance (reroot <= 3) through the
+    incremental scorer alone."""
+    from phylocaml_amd import neighborhood as NB, spr_search as SS
+    from phylocaml_amd._lib import lib
+
+    inst = SS.make_instance(7, 256, 50_000, 10)
+    c = SS.Context(ctx, inst)
+    mv = NB.tbr_moves(inst.N, inst.children, inst.root, 3, 3)
+    idx = np.arange(0, len(mv), max(1, len(mv) // 2048))[:2048]
+    sample = mv[idx]
+    rates, ref = {}, {}
+    for name, form in (("f64", 0), ("f32", 1)):
+        sc = NB.Scorer(ctx, c.dm, inst.N, c.tips, c.stride, c.wd, inst.P, form=form, chunk=1024,
+                       incremental=False)
+        sc.score(inst.children, inst.root, inst.blen, sample[:256])  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ref[name] = sc.score(inst.children, inst.root, inst.blen, sample).cpu().numpy()
+        rates[name] = len(sample) / (time.perf_counter() - t0)
+        del sc
+    res = {"workload": "256-taxon Yule tree, 50,000 unique GTR+G4 patterns; exact lnL of the "
+                       f"{len(mv)} TBR neighbours within join and reroot distance 3",
+           "moves": int(len(mv)), "per_candidate_evaluations_per_s": rates}
+    inc = {}
+    for name, form in (("f64", 0), ("f32", 1)):
+        sc = NB.Scorer(ctx, c.dm, inst.N, c.tips, c.stride, c.wd, inst.P, form=form,
+                       incremental=True, s_batch=32)
+        sc.score(inst.children, inst.root, inst.blen, mv[:4096])  # warm-up, workspace
+        sc.score(inst.children, inst.root, inst.blen, mv)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        got = sc.score(inst.children, inst.root, inst.blen, mv)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        lib.phy_ctx_kernel_timer(ctx.handle, 1)
+        sc.score(inst.children, inst.root, inst.blen, mv)
+        torch.cuda.synchronize()
+        spans = kernel_totals(ctx, ("tbr_insert", "tbr_reroot", "spr_repair", "prune_down"))
+        lib.phy_ctx_kernel_timer(ctx.handle, 0)
+        got = got.cpu().numpy()
+        diff = float(np.max(np.abs(got[idx] - ref["f64"]) / np.abs(ref["f64"])))
+        inc[name] = {"round_s": dt, "moves_per_s": len(mv) / dt,
+                     "vs_per_candidate": len(mv) / dt / rates[name],
+                     "max_rel_diff_vs_per_candidate_f64": diff, "kernel_ms": spans,
+                     "workspace_gib": sc.iwork.numel() / 2**30}
+        log(f"  tbr incremental {name}: {len(mv)} moves in {dt * 1e3:.1f} ms = "
+            f"{len(mv) / dt:.0f} moves/s (x{len(mv) / dt / rates[name]:.0f} per-candidate "
+            f"{rates[name]:.0f}/s), max rel diff {diff:.1e}; kernels {spans}")
+        del sc
+    res["incremental"] = inc
+    # unlimited join distance: every regraft of every rerooting within 3
+    big = NB.tbr_moves(inst.N, inst.children, inst.root, -1, 3)
+    sc = NB.Scorer(ctx, c.dm, inst.N, c.tips, c.stride, c.wd, inst.P, form=1, incremental=True,
+                   s_batch=32)
+    sc.score(inst.children, inst.root, inst.blen, big)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    got = sc.score(inst.children, inst.root, inst.blen, big).cpu().numpy()
+    dt = time.perf_counter() - t0
+    res["incremental_unlimited_join_f32"] = {
+        "moves": int(len(big)), "round_s": dt, "moves_per_s": len(big) / dt,
+        "finite": bool(np.all(np.isfinite(got))), "workspace_gib": sc.iwork.numel() / 2**30}
+    log(f"  tbr incremental f32, join unlimited: {len(big)} moves in {dt * 1e3:.1f} ms = "
+        f"{len(big) / dt:.0f} moves/s")
+    del sc
+    return res
+
+
 def bench_search(ctx):
     """Learned proposal against exhaustive exact scoring, both to convergence,
     on the bounded instance (64 taxa, 5,000 patterns, start = true tree after
@@ -632,8 +705,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["all", "prune", "fitch", "aa", "spr", "search", "mlp",
-                                       "copy", "newton"], default="all",
+    ap.add_argument("--only", choices=["all", "prune", "fitch", "aa", "spr", "tbr", "search",
+                                       "mlp", "copy", "newton"], default="all",
                     help="one part only (the pruning line is always measured unless the part "
                          "is copy or mlp)")
     ap.add_argument("--aa-trees", type=int, default=2048)
@@ -667,6 +740,7 @@ def main():
     aa_res = bench_aa(torch, ctx, clock, args.aa_trees, max(2, steps // 5), cpu_on) \
         if args.only in ("all", "aa") else None
     spr_res = bench_spr(torch, ctx, cpu_on) if args.only in ("all", "spr") else None
+    tbr_res = bench_tbr(torch, ctx) if args.only in ("all", "tbr") else None
     search = bench_search(ctx) if args.only in ("all", "search") else None
     mlp = bench_mlp(torch, ctx, clock) if everything else None
     newton = bench_newton(torch) if args.only in ("all", "newton") else None
@@ -759,6 +833,7 @@ def main():
         "fitch": fitch,
         "aa": aa_res,
         "spr": spr_res,
+        "tbr": tbr_res,
         "search": search,
         "mlp": mlp,
         "newton": newton,
