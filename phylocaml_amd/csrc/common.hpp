@@ -143,9 +143,13 @@ int phy_compile_topo(phy_topo* t);
 // tree with N leaves; nodes not below `top` are ignored, so a tree with
 // nodes removed (a pruned subtree) compiles on the original node ids.  size
 // orders the walk (the smaller subtree continues in the accumulator); depth
-// receives the stack depth.  Host only.
+// receives the stack depth.  With `enter`, the walk enters an internal child
+// only where enter[child] is set; the others are flagged UP_A_CUT /
+// UP_B_CUT (their partials are still read, their subtrees not walked).
+// Host only.
 void phy_compile_up_program(int32_t N, const std::vector<int32_t>& ch, const std::vector<int32_t>& size,
-                            int32_t top, std::vector<int32_t>* prog, int32_t* depth);
+                            int32_t top, std::vector<int32_t>* prog, int32_t* depth,
+                            const std::vector<char>* enter = nullptr);
 
 // Device copies of leaf / node id streams are padded past their end with a
 // valid id (the stream's first), so kernels prefetch whole batches of ids and

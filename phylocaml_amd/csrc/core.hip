@@ -103,8 +103,12 @@ int phy_ctx_kernel_time(phy_ctx* ctx, const char* label, double* total_ms, int64
   return PHY_OK;
 }
 
-// STREAM-style copy: each lane moves 16-byte vectors with non-temporal loads
-// and stores (grid-stride), the tail byte by byte.
+// STREAM-style copy: one 16-byte vector per lane, non-temporal load and
+// store, a grid that covers the buffer (the loop runs once unless the grid
+// is capped), the tail byte by byte.  Measured on a 2 GiB copy: 6.46 TB/s
+// read + write, against 5.58 TB/s for a grid-stride loop over 4 workgroups
+// per CU and 5.89 TB/s for 16 KiB chunks per workgroup (waves dispatched in
+// address order keep the DRAM pages they open busy).
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __global__ void k_copy_stream(const u32x4* __restrict__ src, u32x4* __restrict__ dst, int64_t n16,
                               const unsigned char* __restrict__ src_tail,
@@ -124,16 +128,8 @@ int phy_copy_stream(phy_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) 
     return set_err(ctx, PHY_EINVAL, "copy_stream: pointers must be 16-byte aligned");
   if (bytes == 0) return PHY_OK;
   const int64_t n16 = (int64_t)(bytes / 16), n_tail = (int64_t)(bytes % 16);
-  int n_cu = 256;
-  (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
   const int bs = 256;
-  // workgroups of 4 waves per CU (PHY_COPY_WG_PER_CU, default 4: measured
-  // 5.57 TB/s for a 2 GiB copy against 3.39 / 5.21 / 4.92 / 4.79 / 4.95 TB/s
-  // at 1 / 2 / 8 / 16 / 32)
-  int per_cu = 4;
-  if (const char* e = getenv("PHY_COPY_WG_PER_CU")) per_cu = std::max(1, std::min(64, atoi(e)));
-  const int64_t grid =
-      std::max<int64_t>(1, std::min<int64_t>((n16 + bs - 1) / bs, (int64_t)n_cu * per_cu));
+  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((n16 + bs - 1) / bs, 0x7fffffff));
   KernelSpan span(ctx, "copy_stream");
   hipLaunchKernelGGL(k_copy_stream, dim3((unsigned)grid), dim3(bs), 0, ctx->stream,
                      (const u32x4*)d_src, (u32x4*)d_dst, n16,
@@ -365,10 +361,16 @@ std::vector<int32_t> choose_cuts(int32_t N, const std::vector<int32_t>& ch,
 }  // namespace
 
 void phy_compile_up_program(int32_t N, const std::vector<int32_t>& ch, const std::vector<int32_t>& size,
-                            int32_t top, std::vector<int32_t>* prog, int32_t* depth) {
+                            int32_t top, std::vector<int32_t>* prog, int32_t* depth,
+                            const std::vector<char>* enter) {
   std::vector<int32_t> fifo;
-  const Stop is_leaf = [N](int32_t v) { return v < N; };
-  compile_up(N, ch, size, top, is_leaf, SRC_LEAF, prog, &fifo, depth);
+  if (enter) {
+    const Stop cut = [N, enter](int32_t v) { return v < N || !(*enter)[v]; };
+    compile_up(N, ch, size, top, cut, SRC_LEAF, prog, &fifo, depth);
+  } else {
+    const Stop is_leaf = [N](int32_t v) { return v < N; };
+    compile_up(N, ch, size, top, is_leaf, SRC_LEAF, prog, &fifo, depth);
+  }
 }
 
 int phy_compile_topo(phy_topo* t) {

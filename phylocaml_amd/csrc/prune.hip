@@ -1306,7 +1306,7 @@ __global__ __launch_bounds__(256) void k_spr_insert(
     // messages to internal children, as in the derivative pass
     real Ma[4], Mb[4];
     int eMa = eM + Xc[1], eMb = eM + Xc[0];
-    if (!(fl & UP_A_LEAF)) {
+    if (!(fl & (UP_A_LEAF | UP_A_CUT))) {
       real t[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) t[j] = Ec[j] * ga[j];
@@ -1315,7 +1315,7 @@ __global__ __launch_bounds__(256) void k_spr_insert(
       for (int j = 0; j < 4; ++j) Ma[j] += Aa[j];
       rescale4(Ma, &eMa);
     }
-    if (!(fl & UP_B_LEAF)) {
+    if (!(fl & (UP_B_LEAF | UP_B_CUT))) {
       real t[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) t[j] = Ec[4 + j] * gb[j];
@@ -1332,7 +1332,7 @@ __global__ __launch_bounds__(256) void k_spr_insert(
       stke[(size_t)sp * bd + tid] = cb ? eMa : eMb;
       ++sp;
     }
-    if (!(fl & UP_A_LEAF) || !(fl & UP_B_LEAF)) {
+    if (!(fl & (UP_A_LEAF | UP_A_CUT)) || !(fl & (UP_B_LEAF | UP_B_CUT))) {
       const bool cb = fl & UP_CONT_B;
       const real* c = cb ? Mb : Ma;
 #pragma unroll
@@ -1526,11 +1526,11 @@ __global__ __launch_bounds__(64 * R) void k_spr_insert_w(
     // messages to internal children: M_c = P_c^T A_c
     real Ma[4], Mb[4];
     int eMa = eAa, eMb = eAb;
-    if (!(fl & UP_A_LEAF)) {
+    if (!(fl & (UP_A_LEAF | UP_A_CUT))) {
       vecmat4(PA, Aa, Ma);
       rescale4(Ma, &eMa);
     }
-    if (!(fl & UP_B_LEAF)) {
+    if (!(fl & (UP_B_LEAF | UP_B_CUT))) {
       vecmat4(PB, Ab, Mb);
       rescale4(Mb, &eMb);
     }
@@ -1542,7 +1542,7 @@ __global__ __launch_bounds__(64 * R) void k_spr_insert_w(
       stke[(size_t)sp * BD + tid] = cb ? eMa : eMb;
       ++sp;
     }
-    if (!(fl & UP_A_LEAF) || !(fl & UP_B_LEAF)) {
+    if (!(fl & (UP_A_LEAF | UP_A_CUT)) || !(fl & (UP_B_LEAF | UP_B_CUT))) {
       const bool cb = fl & UP_CONT_B;
       const real* c = cb ? Mb : Ma;
 #pragma unroll
@@ -1937,11 +1937,11 @@ __global__ __launch_bounds__(64 * R) void k_tbr_insert_w(
     // messages to internal children: M_c = P_c^T A_c
     real Ma[4], Mb[4];
     int eMa = eAa, eMb = eAb;
-    if (!(fl & UP_A_LEAF)) {
+    if (!(fl & (UP_A_LEAF | UP_A_CUT))) {
       vecmat4(PA, Aa, Ma);
       rescale4(Ma, &eMa);
     }
-    if (!(fl & UP_B_LEAF)) {
+    if (!(fl & (UP_B_LEAF | UP_B_CUT))) {
       vecmat4(PB, Ab, Mb);
       rescale4(Mb, &eMb);
     }
@@ -1953,7 +1953,7 @@ __global__ __launch_bounds__(64 * R) void k_tbr_insert_w(
       stke[(size_t)sp * BD + tid] = cb ? eMa : eMb;
       ++sp;
     }
-    if (!(fl & UP_A_LEAF) || !(fl & UP_B_LEAF)) {
+    if (!(fl & (UP_A_LEAF | UP_A_CUT)) || !(fl & (UP_B_LEAF | UP_B_CUT))) {
       const bool cb = fl & UP_CONT_B;
       const real* c = cb ? Mb : Ma;
 #pragma unroll
@@ -2914,7 +2914,30 @@ struct IncrSub {
   std::vector<int32_t> prog, ref, ops;  // ops: {y, a, b, flags} bottom-up
 };
 
-void incr_prepare(const IncrTree& t, int32_t s, int slot, int lmax, IncrSub* o) {
+// Internal nodes of the pruned tree of s that the insertion walk enters: the
+// proper ancestors, in the pruned tree, of each target (x stands for keeping
+// the attachment; x hangs from g there, or is the root when p is).
+void incr_enter(const IncrTree& t, int32_t s, const std::vector<int32_t>& targets, std::vector<char>* enter) {
+  const int32_t p = t.parent[s], g = t.parent[p];
+  enter->assign(t.M, 0);
+  for (int32_t c : targets) {
+    int32_t y = t.parent[c];
+    if (y == p) y = g;
+    while (y >= 0 && !(*enter)[y]) {
+      (*enter)[y] = 1;
+      y = t.parent[y];
+      if (y == p) y = g;
+    }
+  }
+}
+
+// enter: the nodes the insertion walk enters, tgt: the targets (nullptr:
+// every node).  With both, the repair stops at the highest ancestor whose
+// new partial the walk uses (a target, or the sibling of a target or of an
+// entered node); the ancestors above keep their base rows, which the walk
+// reads only into products it discards.
+void incr_prepare(const IncrTree& t, int32_t s, int slot, int lmax, IncrSub* o,
+                  const std::vector<char>* enter = nullptr, const std::vector<char>* tgt = nullptr) {
   const int32_t N = t.N, M = t.M;
   const int32_t p = t.parent[s], x = t.sibling(s), g = t.parent[p];
   std::vector<int32_t> ch = t.ch, size = t.size;
@@ -2929,9 +2952,20 @@ void incr_prepare(const IncrTree& t, int32_t s, int slot, int lmax, IncrSub* o) 
     (ch[2 * (g - N)] == p ? ch[2 * (g - N)] : ch[2 * (g - N) + 1]) = x;
     top = t.root;
     o->x = x;
+    int32_t last = (int32_t)M;  // highest path position to repair
+    if (enter && tgt) {
+      last = -1;
+      int32_t j = 0;
+      for (int32_t y = g; y >= 0; y = t.parent[y], ++j) {
+        const int32_t up = t.parent[y];
+        const int32_t z = up >= 0 ? t.sibling(y) : -1;
+        if ((*tgt)[y] || (z >= 0 && ((*tgt)[z] || (*enter)[z]))) last = j;
+      }
+    }
     int32_t prev = -1, j = 0;
     for (int32_t y = g; y >= 0; y = t.parent[y], ++j) {
       size[y] -= t.size[s];
+      if (j > last) continue;
       o->ref[y] = (N - 1) + slot * lmax + j;
       const int32_t a = ch[2 * (y - N)], b = ch[2 * (y - N) + 1];
       o->ops.insert(o->ops.end(), {y, a, b, (a == prev ? 1 : 0) | (b == prev ? 2 : 0)});
@@ -2940,7 +2974,7 @@ void incr_prepare(const IncrTree& t, int32_t s, int slot, int lmax, IncrSub* o) 
   }
   o->prog.clear();
   o->depth = 0;
-  if (top >= N) phy_compile_up_program(N, ch, size, top, &o->prog, &o->depth);
+  if (top >= N) phy_compile_up_program(N, ch, size, top, &o->prog, &o->depth, enter);
   o->n_ops = (int32_t)o->prog.size() / 4;
 }
 
@@ -3234,11 +3268,21 @@ int plan_create(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat, i
   const int S = sp->pl.S, lmax = sp->pl.lmax;
   std::vector<IncrSub> sub(n_sub);
   {
+    // each pruned tree's walk enters only the nodes above its moves' targets
     const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::thread> th;
     for (unsigned w = 0; w < std::min<unsigned>(nt, std::max(n_sub, 1)); ++w)
       th.emplace_back([&, w] {
-        for (int k = (int)w; k < n_sub; k += (int)nt) incr_prepare(t, subs[k], k % S, lmax, &sub[k]);
+        std::vector<int32_t> targets;
+        std::vector<char> enter, tgt(M, 0);
+        for (int k = (int)w; k < n_sub; k += (int)nt) {
+          targets.clear();
+          for (int64_t j = sp->mv_start[k]; j < sp->mv_start[k + 1]; ++j) targets.push_back(moves[2 * idx[j] + 1]);
+          for (int32_t c : targets) tgt[c] = 1;
+          incr_enter(t, subs[k], targets, &enter);
+          incr_prepare(t, subs[k], k % S, lmax, &sub[k], &enter, &tgt);
+          for (int32_t c : targets) tgt[c] = 0;
+        }
       });
     for (auto& x : th) x.join();
   }
@@ -3446,8 +3490,17 @@ int tbr_plan_create(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pa
     std::vector<std::thread> th;
     for (unsigned w = 0; w < std::min<unsigned>(nt, std::max(n_sub, 1)); ++w)
       th.emplace_back([&, w] {
+        std::vector<int32_t> targets;
+        std::vector<char> enter, tgt(M, 0);
         for (int k = (int)w; k < n_sub; k += (int)nt) {
-          incr_prepare(t, slot_s[k], k % S, lmax, &sub[k]);
+          targets.clear();
+          for (int c = 0; c < M; ++c)
+            if (tmask[(size_t)k * (M + 1) + c]) targets.push_back(c);
+          if (tmask[(size_t)k * (M + 1) + M]) targets.push_back(t.sibling(slot_s[k]));  // keeping
+          for (int32_t c : targets) tgt[c] = 1;
+          incr_enter(t, slot_s[k], targets, &enter);
+          incr_prepare(t, slot_s[k], k % S, lmax, &sub[k], &enter, &tgt);
+          for (int32_t c : targets) tgt[c] = 0;
           if (slot_s[k] >= N) phy_compile_up_program(N, t.ch, t.size, slot_s[k], &tprog[k], &tdepth[k]);
         }
       });

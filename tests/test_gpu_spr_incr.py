@@ -134,6 +134,33 @@ def test_incremental_rate_classes_and_shapes(ctx, rates, ref, kind):
                                   ref_v[err.argmax()])
 
 
+@pytest.mark.parametrize("kind", ["random", "yule", "caterpillar"])
+def test_incremental_few_targets_per_subtree(ctx, kind):
+    """Short move lists (the NNI moves, and two random regrafts per pruned
+    subtree): each pruned tree's walk then enters only the nodes above its
+    targets and reads the partials of the subtrees it skips."""
+    from phylocaml_amd import neighborhood as NB, spr, synth
+
+    N, P = 20, 500
+    ch, root, bl = _tree(kind, N, 17)
+    model = _model(17)
+    st, w = _data(synth.Tree(N, ch, root, bl), model, P, 17)
+    full = spr.moves(N, ch, root)
+    rng = np.random.default_rng(5)
+    pick = []
+    for s in np.unique(full[:, 0]):
+        rows = np.flatnonzero(full[:, 0] == s)
+        pick += list(rng.choice(rows, min(2, len(rows)), replace=False))
+    mv = np.concatenate([NB.nni_moves(N, ch, root), full[np.array(pick)]])
+    ref_v = _oracle_all(N, ch, root, bl, model, st, w, mv)
+    for form, tol in ((F64, 1e-10), (F32, 1e-6)):
+        inc, _, _ = _scorers(ctx, model, st, w, N, form, s_batch=3)
+        _say(f"incremental few targets {kind} form {form}: {len(mv)} moves")
+        got = inc.score(ch, root, bl, mv).cpu().numpy()
+        err = np.abs(got - ref_v) / np.abs(ref_v)
+        assert err.max() <= tol, (kind, form, mv[err.argmax()], got[err.argmax()], ref_v[err.argmax()])
+
+
 def test_incremental_three_taxa(ctx):
     from phylocaml_amd import spr, synth
 
