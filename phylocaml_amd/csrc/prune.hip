@@ -1558,6 +1558,9 @@ __global__ __launch_bounds__(64 * R) void k_spr_insert_w(
   });
 }
 
+// pairs per LDS exchange of k_tbr_insert_w
+constexpr int kTbrChunk = 4;
+
 // ---------------------------------------------------------------------------
 // Incremental TBR scoring.  The move (s, w, v) reroots the subtree of s at
 // the middle of the branch above w (w in the subtree; its old root s
@@ -1735,8 +1738,8 @@ __global__ __launch_bounds__(64 * R) void k_tbr_reroot(
 // for each child c with needed pairs (tmask[k][c], one bit per rerooting
 // row w of the subtree), q = (P(t_c/2)^T A_c) (*) (P(t_c/2) D_c) and the
 // class term q . X_w of each needed w; the classes of a pair meet in LDS
-// (double-buffered by pair parity, one barrier per pair, the pair's wave
-// r = pair % R finishing it).  Keeping the attachment (tmask[k][M]) is the
+// (kTbrChunk pairs per exchange, double-buffered by exchange parity, one
+// barrier per exchange, wave r finishing pairs r, r + R, ... of it).  Keeping the attachment (tmask[k][M]) is the
 // child x of g with P(t_p) above and P(t_x) below (Pk[k]), or, when p is the
 // root, the first entry (x, the pruned tree's root) with pi above.
 // wsum row of (k, c, w): rb[k][c] + the rank of w among c's needed bits.
@@ -1763,8 +1766,8 @@ __global__ __launch_bounds__(64 * R) void k_tbr_insert_w(
   const int64_t pc = act ? p : 0;
   real* stk = reinterpret_cast<real*>(lds_raw);                                  // [depth][4][BD]
   int32_t* stke = reinterpret_cast<int32_t*>(stk + (size_t)up_depth * 4 * BD);  // [depth][BD]
-  double* xvl = reinterpret_cast<double*>(stke + (size_t)up_depth * BD);        // [2][R][64]
-  int32_t* xel = reinterpret_cast<int32_t*>(xvl + 2 * R * 64);                  // [2][R][64]
+  double* xvl = reinterpret_cast<double*>(stke + (size_t)up_depth * BD);        // [2][chunk][R][64]
+  int32_t* xel = reinterpret_cast<int32_t*>(xvl + 2 * kTbrChunk * R * 64);      // [2][chunk][R][64]
   const int32_t* rf = ref + (size_t)k * M;
   const uint32_t* tm = tmask + (size_t)k * (M + 1);
   const int32_t* rb = rbase + (size_t)k * (M + 1);
@@ -1774,26 +1777,35 @@ __global__ __launch_bounds__(64 * R) void k_tbr_insert_w(
   const int4* pg = prog + (size_t)k * prog_stride;
   const real* pxk = Px + ((size_t)k * R + r) * 16;
   auto pfull = [&](int c) { return c == x ? pxk : Pf + ((size_t)c * R + r) * 16; };
-  int pair = 0;  // pairs finished so far (uniform)
-  // the class terms of the needed pairs (c, w) of q, finished in LDS
+  int chunk = 0;  // exchanges so far (uniform)
+  // the class terms of the needed pairs (c, w) of q, kTbrChunk pairs per
+  // exchange: every wave forms its class's terms of the chunk (the X rows
+  // loaded together), one barrier, then wave r finishes pairs r, r + R, ...
   auto pairs = [&](uint32_t mask, int row0, const real* q, int eq) {
     mask = __builtin_amdgcn_readfirstlane(mask);  // the same word in every lane
     while (mask) {
-      const int w = __builtin_ctz(mask);
-      mask &= mask - 1;
-      const int xr = xrow0 + w;
-      const real* xs = xv + (((size_t)xr * R + r) * 4) * pitch + pc;
-      real sv = (real)0;
+      const int buf = chunk & 1;
+      int n = 0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) sv = fma(q[j], xs[(size_t)j * pitch], sv);
-      const int E = eq + xe[((size_t)xr * R + r) * pitch + pc];
-      const int buf = pair & 1;
-      xvl[(buf * R + r) * 64 + lane] = (double)sv;
-      xel[(buf * R + r) * 64 + lane] = E;
+      for (int j = 0; j < kTbrChunk; ++j) {
+        if (mask) {
+          const int w = __builtin_ctz(mask);
+          mask &= mask - 1;
+          const int xr = xrow0 + w;
+          const real* xs = xv + (((size_t)xr * R + r) * 4) * pitch + pc;
+          real sv = (real)0;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) sv = fma(q[i], xs[(size_t)i * pitch], sv);
+          const int slot = ((buf * kTbrChunk + j) * R + r) * 64 + lane;
+          xvl[slot] = (double)sv;
+          xel[slot] = eq + xe[((size_t)xr * R + r) * pitch + pc];
+          ++n;
+        }
+      }
       __syncthreads();
-      if (r == pair % R) {
-        const double* sq = xvl + buf * R * 64;
-        const int32_t* eqq = xel + buf * R * 64;
+      for (int j = r; j < n; j += R) {
+        const double* sq = xvl + (buf * kTbrChunk + j) * R * 64;
+        const int32_t* eqq = xel + (buf * kTbrChunk + j) * R * 64;
         int m = kNoExp;
 #pragma unroll
         for (int rr = 0; rr < R; ++rr)
@@ -1806,10 +1818,10 @@ __global__ __launch_bounds__(64 * R) void k_tbr_insert_w(
         }
         const double v = act ? (double)weights[p] * ((double)log(L) + (double)m * 0.6931471805599453) : 0.0;
         const double S = wave_sum_d(v);
-        if (lane == 0) wsum[(size_t)row0 * n_blk + blockIdx.x] = S;
+        if (lane == 0) wsum[(size_t)(row0 + j) * n_blk + blockIdx.x] = S;
       }
-      ++row0;
-      ++pair;
+      row0 += n;
+      ++chunk;
     }
   };
   auto fetch = [&](int c, real* Dv, int* e) {
@@ -1971,7 +1983,7 @@ __global__ __launch_bounds__(64 * R) void k_tbr_insert_w(
 
 // LDS of k_tbr_insert_w: the message stack and the double-buffered exchange
 size_t tbr_insert_lds(int R, size_t esz, int depth) {
-  return (size_t)std::max(depth, 1) * 64 * R * (4 * esz + 4) + (size_t)2 * R * 64 * 12;
+  return (size_t)std::max(depth, 1) * 64 * R * (4 * esz + 4) + (size_t)2 * kTbrChunk * R * 64 * 12;
 }
 // LDS of k_tbr_reroot: the outside-vector stack
 size_t tbr_reroot_lds(int R, size_t esz, int depth) {
