@@ -758,28 +758,28 @@ int uppass_launch(phy_ctx* ctx, const phy_topo* topo, const uint32_t* s32, uint3
   return PHY_OK;
 }
 
-// 32-bit words per thread: 1, 2 or 4 (PHY_FITCH_WORDS, default 2: 8-byte
-// accesses)
-int words_env() {
+// 32-bit words per thread: 1, 2 or 4 (PHY_FITCH_WORDS overrides the pass's
+// default)
+int words_env(int dflt) {
   const char* s = getenv("PHY_FITCH_WORDS");
-  const int v = s ? atoi(s) : 2;
-  return (v == 1 || v == 4) ? v : 2;
+  const int v = s ? atoi(s) : dflt;
+  return (v == 1 || v == 2 || v == 4) ? v : dflt;
 }
 
-// operand prefetch depth (leaves / node sets per batch): 8 for the
-// down-pass, 4 for the up-pass (whose LDS stack holds two sets a level, so a
-// shorter ring keeps more waves resident; measured 0.487 against 0.503 ms at
-// 1,024 x 10^6), PHY_FITCH_BATCH=4|8|16|32 overrides both
+// operand prefetch depth (leaves / node sets per batch), 4 for both passes
+// (a short ring keeps more waves resident); PHY_FITCH_BATCH=4|8|16|32
+// overrides both
 int batch_env(int dflt) {
   const char* s = getenv("PHY_FITCH_BATCH");
   const int v = s ? atoi(s) : dflt;
   return (v == 4 || v == 8 || v == 16 || v == 32) ? v : dflt;
 }
 
-// non-temporal row loads and stores (PHY_FITCH_NT=1)
-bool nt_env() {
+// non-temporal row loads and stores (PHY_FITCH_NT=0|1 overrides the pass's
+// default)
+bool nt_env(bool dflt) {
   const char* s = getenv("PHY_FITCH_NT");
-  return s && s[0] == '1';
+  return s ? s[0] == '1' : dflt;
 }
 
 // the widest lane (<= want words) whose accesses stay aligned and whose
@@ -812,10 +812,14 @@ int phy_fitch_downpass(phy_ctx* ctx, const phy_topo* topo, uint64_t* sets, int64
   uint32_t* s32 = reinterpret_cast<uint32_t*>(sets);
   auto* nc = reinterpret_cast<unsigned long long*>(node_cost);
   auto* tt = reinterpret_cast<unsigned long long*>(total);
-  const int v = lane_words(words_env(), s32, 2 * stride, W32);
-#define PHY_DOWN(B, VV) (nt_env() ? downpass_launch<B, VV, true>(ctx, topo, s32, 2 * stride, W32, n_sites, wt, site_cost, nc, tt, bs) \
+  // 16-byte non-temporal lanes and a 4-deep ring: at 1,024 taxa x 10^6
+  // sites 0.243 ms against 0.266 ms for 8-byte cached lanes and an 8-deep
+  // ring (sweep over words 1 / 2 / 4, ring 4 / 8 / 16, non-temporal or not,
+  // blocks of 128 / 256, 8 / 16 / 32 parts)
+  const int v = lane_words(words_env(4), s32, 2 * stride, W32);
+#define PHY_DOWN(B, VV) (nt_env(true) ? downpass_launch<B, VV, true>(ctx, topo, s32, 2 * stride, W32, n_sites, wt, site_cost, nc, tt, bs) \
                               : downpass_launch<B, VV, false>(ctx, topo, s32, 2 * stride, W32, n_sites, wt, site_cost, nc, tt, bs))
-  switch (batch_env(8)) {
+  switch (batch_env(4)) {
     case 4: return v == 4 ? PHY_DOWN(4, 4) : v == 2 ? PHY_DOWN(4, 2) : PHY_DOWN(4, 1);
     case 8: return v == 4 ? PHY_DOWN(8, 4) : v == 2 ? PHY_DOWN(8, 2) : PHY_DOWN(8, 1);
     case 32: return v == 4 ? PHY_DOWN(32, 4) : v == 2 ? PHY_DOWN(32, 2) : PHY_DOWN(32, 1);
@@ -836,9 +840,11 @@ int phy_fitch_uppass(phy_ctx* ctx, const phy_topo* topo, const uint64_t* sets, u
   const int64_t W32 = 2 * W;
   const uint32_t* s32 = reinterpret_cast<const uint32_t*>(sets);
   uint32_t* f32 = reinterpret_cast<uint32_t*>(fin);
-  const int v = std::min(lane_words(words_env(), s32, 2 * stride, W32),
-                         lane_words(words_env(), f32, 2 * stride, W32));
-#define PHY_UP(B, VV) (nt_env() ? uppass_launch<B, VV, true>(ctx, topo, s32, f32, 2 * stride, W32, bs) \
+  // 8-byte cached lanes, 4-deep ring: 0.485 ms at 1,024 x 10^6 (16-byte
+  // non-temporal lanes 0.491 ms)
+  const int v = std::min(lane_words(words_env(2), s32, 2 * stride, W32),
+                         lane_words(words_env(2), f32, 2 * stride, W32));
+#define PHY_UP(B, VV) (nt_env(false) ? uppass_launch<B, VV, true>(ctx, topo, s32, f32, 2 * stride, W32, bs) \
                             : uppass_launch<B, VV, false>(ctx, topo, s32, f32, 2 * stride, W32, bs))
   switch (batch_env(4)) {
     case 4: return v == 4 ? PHY_UP(4, 4) : v == 2 ? PHY_UP(4, 2) : PHY_UP(4, 1);

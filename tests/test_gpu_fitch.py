@@ -97,17 +97,20 @@ def test_tree_passes_match_oracle(ctx, rng, N, S):
         _tree_case(ctx, N, ch, root, leaves, w)
 
 
+@pytest.mark.parametrize("nt", ["0", "1"])
 @pytest.mark.parametrize("words", ["1", "2", "4"])
 @pytest.mark.parametrize("batch", ["4", "8", "16", "32"])
 @pytest.mark.parametrize("split", ["0", "1"])
-def test_prefetch_depth_and_split_forms(ctx, rng, monkeypatch, batch, split, words):
+def test_prefetch_depth_and_split_forms(ctx, rng, monkeypatch, batch, split, words, nt):
     # the whole-tree and the split (parts side by side, then the top) forms,
-    # at every operand prefetch depth and lane width, give the oracle's bits
+    # at every operand prefetch depth and lane width, with cached or
+    # non-temporal row accesses, give the oracle's bits
     monkeypatch.setenv("PHY_FITCH_BATCH", batch)
     monkeypatch.setenv("PHY_FITCH_SPLIT", split)
     monkeypatch.setenv("PHY_FITCH_WORDS", words)
+    monkeypatch.setenv("PHY_FITCH_NT", nt)
     N, S = 300, 3001
-    _say(f"fitch forms batch={batch} split={split} words={words}")
+    _say(f"fitch forms batch={batch} split={split} words={words} nt={nt}")
     ch, root = random_tree(rng, N)
     leaves = rng.integers(1, 16, (N, S)).astype(np.uint8)
     w = rng.integers(1, 5, S).astype(np.uint32)
