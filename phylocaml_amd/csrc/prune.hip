@@ -691,7 +691,7 @@ __device__ __forceinline__ void vecmat_u(const real* __restrict__ A, const real*
 }
 
 template <typename real, int R>
-__global__ __launch_bounds__(256) void k_prune_deriv(
+__global__ __launch_bounds__(256, sizeof(real) == 8 ? 3 : 1) void k_prune_deriv(
     const int4* __restrict__ prog, int n_ops, const uint64_t* __restrict__ tips, int64_t stride,
     int64_t n_pat, int64_t pitch, const real* __restrict__ tab, const real* __restrict__ E,
     const double* __restrict__ pi, const double* __restrict__ probs, const real* __restrict__ part,
@@ -710,6 +710,13 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
   const int64_t wave = ((int64_t)blockIdx.x * bd + tid) >> 6;
   const int lane = tid & 63;
 
+  // FP32 fetches an entry's eigen factors one node ahead with its partials
+  // and keeps a table of (lambda_k r)^2; FP64 reads the factors (a small
+  // table that stays in cache) when the entry starts and forms the squares
+  // as two factors, which keeps it within 168 registers (three waves per
+  // SIMD instead of two: 7.2 against 8.1 ms at 512 x 200k; FP32 is faster
+  // the first way, 4.55 against 4.68 ms)
+  constexpr bool kAhead = sizeof(real) == 4;
   real U[16], Ui[16], l1[4], l2[4];
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
@@ -723,8 +730,13 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
   }
   const double pr = probs[r];
 
-  // children partials (4 values + exponent each) and eigen factors of an
-  // entry, fetched raw one node ahead
+  auto efac = [&](int c, real* ev) {
+    const real* es = E + ((size_t)c * R + r) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ev[k] = es[k];
+  };
+  // children partials (4 values + exponent each) of an entry, fetched raw
+  // one node ahead, with the eigen factors in FP32
   auto fetch = [&](int c, real* Dv, int* e, real* ev) {
     if (c < N) {
       tipvec(tip_code(tips, stride, c, pc), Dv);
@@ -735,9 +747,7 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
       for (int k = 0; k < 4; ++k) Dv[k] = sv[(size_t)k * pitch];
       *e = ex[((size_t)(c - N) * R + r) * pitch + pc];
     }
-    const real* es = E + ((size_t)c * R + r) * 4;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) ev[k] = es[k];
+    if constexpr (kAhead) efac(c, ev);
   };
   // Every factor keeps the power-of-two exponent of its own rate class: the
   // children partials the exponents of the post-order pass, the messages M
@@ -769,6 +779,10 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
     }
     const int fl = op.w, src = fl & 3;
     const int a = op.y, b = op.z;
+    if constexpr (!kAhead) {
+      efac(a, Ec);
+      efac(b, Ec + 4);
+    }
     real Mv[4];
     int eM;
     if (src == SRC_LEAF) {
@@ -824,9 +838,9 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
     for (int k = 0; k < 4; ++k) {
       const real qa = ga[k] * wa[k], qb = gb[k] * wb[k];
       s1a = fma(qa, l1[k], s1a);
-      s2a = fma(qa, l2[k], s2a);
+      s2a = kAhead ? fma(qa, l2[k], s2a) : fma(qa * l1[k], l1[k], s2a);
       s1b = fma(qb, l1[k], s1b);
-      s2b = fma(qb, l2[k], s2b);
+      s2b = kAhead ? fma(qb, l2[k], s2b) : fma(qb * l1[k], l1[k], s2b);
     }
     // class weight prob_r 2^(E_r - m), E_r = eM + e_a + e_b, m the largest E
     // over the pattern's classes with a non-zero term (0 for the others)
@@ -912,7 +926,7 @@ __global__ __launch_bounds__(256) void k_prune_deriv(
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         Dc[k] = Dn[k];
-        Ec[k] = En[k];
+        if constexpr (kAhead) Ec[k] = En[k];
       }
       Xc[0] = Xn[0];
       Xc[1] = Xn[1];
