@@ -1591,9 +1591,10 @@ constexpr int kTbrChunk = 4;
 // P(t_x) below), or, when p is the root, the root of the pruned tree joined
 // to the subtree (pi, then P(t_x)).
 //
-// tbr meta[k]: {subtree program entries, s, 0, subtree stack depth, x (the
-// sibling of s), g >= 0, 0, 0}.  wslot[k][c]: X row of the rerooting at c
-// (-1: not needed); the rows of batch slot k are k * wmax .. k * wmax + 31.
+// tbr meta[k]: {subtree program entries, s, first X row of the slot in its
+// batch, subtree stack depth, x (the sibling of s), g >= 0, 0, 0}.
+// wslot[k][c]: X row of the rerooting at c (-1: not needed); a slot's
+// rerootings take consecutive rows from its first.
 // ---------------------------------------------------------------------------
 template <typename real, int R>
 __global__ __launch_bounds__(64 * R) void k_tbr_reroot(
@@ -1763,7 +1764,7 @@ __global__ __launch_bounds__(64 * R) void k_tbr_insert_w(
     const int32_t* __restrict__ tmeta, const int32_t* __restrict__ ref, const uint32_t* __restrict__ tmask,
     const int32_t* __restrict__ rbase, const real* __restrict__ Pf, const real* __restrict__ Ph,
     const real* __restrict__ Px, const real* __restrict__ Pk, const real* __restrict__ xv,
-    const int32_t* __restrict__ xe, int wmax, const double* __restrict__ pi, const double* __restrict__ probs,
+    const int32_t* __restrict__ xe, const double* __restrict__ pi, const double* __restrict__ probs,
     const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
     const real* __restrict__ part, const int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
     double* __restrict__ wsum, int64_t n_blk, int N, int M, int up_depth) {
@@ -1787,7 +1788,7 @@ __global__ __launch_bounds__(64 * R) void k_tbr_insert_w(
   const int32_t* rb = rbase + (size_t)k * (M + 1);
   const int x = mk[2];
   const bool has_g = tk[5] != 0;
-  const int xrow0 = k * wmax;
+  const int xrow0 = tk[2];
   const int4* pg = prog + (size_t)k * prog_stride;
   const real* pxk = Px + ((size_t)k * R + r) * 16;
   auto pfull = [&](int c) { return c == x ? pxk : Pf + ((size_t)c * R + r) * 16; };
@@ -2879,24 +2880,25 @@ struct IncrPlan {
   size_t off_part = 0, off_ex = 0, off_bsum = 0, off_lnl = 0, off_wsum = 0, off_blh = 0;
   size_t off_xv = 0, off_xe = 0, off_Pk = 0, total = 0;  // TBR: rerooted subtree vectors, keep matrices
   int64_t pitch = 0, n_waves = 0, n_blk = 0, pool_rows = 0;
-  int S = 0, lmax = 0, slots = 0, wmax = 0;
+  int S = 0, lmax = 0, slots = 0;
+  int64_t xrows = 0;  // TBR: rerooted subtree vectors per batch
   size_t esz = 8;
 };
 
 // Workspace of a run: lengths and factors, the base tree's P, the partial
 // pool, the per-target sums; per-slot factors for `slots` pruned subtrees
 // (every pruned subtree a tree can have, M, for SPR).  TBR adds the
-// rerooted subtree vectors of a batch (S slots x wmax rerootings), the keep
+// rerooted subtree vectors of a batch (xrows rows), the keep
 // matrices P(t_p), P(t_x) of each slot, and per-pair sums (max_pairs pairs
 // in a batch).
 IncrPlan incr_plan(const IncrTree& t, int R, int64_t n_pat, int32_t form, int32_t S, int slots = -1,
-                   int wmax = 0, int64_t max_pairs = 0) {
+                   int64_t xrows = 0, int64_t max_pairs = 0) {
   IncrPlan pl;
   const int N = t.N, M = t.M, K = 4;
   pl.esz = (form & PHY_F32) ? 4 : 8;
   pl.S = S;
   pl.slots = slots < 0 ? M : std::max(slots, 1);
-  pl.wmax = wmax;
+  pl.xrows = xrows;
   pl.lmax = std::max(t.lmax, 1);
   pl.pitch = ((n_pat + 255) / 256) * 256;
   pl.n_waves = pl.pitch * R / 64;
@@ -2921,11 +2923,11 @@ IncrPlan incr_plan(const IncrTree& t, int R, int64_t n_pat, int32_t form, int32_
   pl.off_lnl = o; o = align256(o + 8);
   // per-target sums: SPR one row per (subtree of a batch, target), TBR one
   // per needed pair of a batch
-  const size_t rows = wmax > 0 ? (size_t)max_pairs * pl.n_blk : (size_t)S * M * std::max(pl.n_waves, pl.n_blk);
+  const size_t rows = xrows > 0 ? (size_t)max_pairs * pl.n_blk : (size_t)S * M * std::max(pl.n_waves, pl.n_blk);
   pl.off_wsum = o; o = align256(o + rows * 8);
-  if (wmax > 0) {
-    pl.off_xv = o; o = align256(o + (size_t)S * wmax * R * K * pl.pitch * esz);
-    pl.off_xe = o; o = align256(o + (size_t)S * wmax * R * pl.pitch * 4);
+  if (xrows > 0) {
+    pl.off_xv = o; o = align256(o + (size_t)xrows * R * K * pl.pitch * esz);
+    pl.off_xe = o; o = align256(o + (size_t)xrows * R * pl.pitch * 4);
     pl.off_Pk = o; o = align256(o + 2 * ns * R * K * K * esz);
   }
   pl.total = o;
@@ -3213,7 +3215,7 @@ int incr_run(phy_ctx* ctx, const phy_spr_plan* sp, const phy_model* m, const uin
                            reinterpret_cast<const int32_t*>(dm + sp->o_rbase) + (size_t)b0 * (M + 1), Pf, Ph,
                            Px + (size_t)b0 * R * 16,
                            reinterpret_cast<const real*>(work + pl.off_Pk) + (size_t)b0 * 2 * R * 16, xv, xe,
-                           pl.wmax, d_pi, d_probs, tips, stride, n_pat, pl.pitch, part, ex, weights, wsum,
+                           d_pi, d_probs, tips, stride, n_pat, pl.pitch, part, ex, weights, wsum,
                            pl.n_blk, N, M, depth);
       }
       PHY_LAUNCHED(ctx);
@@ -3376,8 +3378,9 @@ int plan_create(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pat, i
 // rerooting bit.  Validation follows phy_tbr_apply: w = s or w under s with
 // s internal; v = -1 (keep) only with w != s; any other v a valid SPR target.
 struct TbrGroups {
-  int n_sub = 0, wmax = 1;
-  int64_t max_pairs = 1;
+  int n_sub = 0;
+  int64_t max_pairs = 1, max_xrows = 1;
+  std::vector<int32_t> xbase;           // first X row of each slot in its batch
   std::vector<int32_t> slot_s;          // subtree of each slot
   std::vector<int64_t> slot_first;      // first (s, w) of each slot in uniq (n_sub + 1)
   std::vector<int64_t> uniq;            // distinct s * M + w, ascending
@@ -3425,8 +3428,17 @@ int tbr_group(const IncrTree& t, int64_t n_moves, const int32_t* moves, int S, T
   }
   const int n_sub = g->n_sub = (int)g->slot_s.size();
   g->slot_first.push_back((int64_t)g->uniq.size());
-  g->wmax = 1;
-  for (int k = 0; k < n_sub; ++k) g->wmax = std::max(g->wmax, (int)(g->slot_first[k + 1] - g->slot_first[k]));
+  // X rows: a slot's rerootings take consecutive rows of its batch
+  g->xbase.assign(std::max(n_sub, 1), 0);
+  g->max_xrows = 1;
+  for (int b0 = 0; b0 < n_sub; b0 += S) {
+    int64_t cur = 0;
+    for (int k = b0; k < std::min(n_sub, b0 + S); ++k) {
+      g->xbase[k] = (int32_t)cur;
+      cur += g->slot_first[k + 1] - g->slot_first[k];
+    }
+    g->max_xrows = std::max(g->max_xrows, cur);
+  }
   std::vector<int32_t> pair_slot(g->uniq.size());
   for (int k = 0; k < n_sub; ++k)
     for (int64_t j = g->slot_first[k]; j < g->slot_first[k + 1]; ++j) pair_slot[j] = k;
@@ -3491,7 +3503,7 @@ int tbr_plan_create(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pa
   const int gr = tbr_group(t, n_moves, moves, S, &grp, &bad);
   if (gr == 1) return set_err(ctx, PHY_EINVAL, "tbr incremental: invalid move " + std::to_string(bad));
   if (gr == 2) return set_err(ctx, PHY_ERANGE, "tbr incremental: too many pairs in a batch");
-  const int n_sub = sp->n_sub = grp.n_sub, wmax = grp.wmax;
+  const int n_sub = sp->n_sub = grp.n_sub;
   const std::vector<int32_t>& slot_s = grp.slot_s;
   const std::vector<int64_t>& slot_first = grp.slot_first;
   const std::vector<int64_t>& uniq = grp.uniq;
@@ -3501,7 +3513,7 @@ int tbr_plan_create(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pa
   const std::vector<int64_t>& idx = grp.idx;
   sp->mv_start = grp.mv_start;
   const int64_t max_pairs = grp.max_pairs;
-  sp->pl = incr_plan(t, R, n_pat, form, S, n_sub, wmax, max_pairs);
+  sp->pl = incr_plan(t, R, n_pat, form, S, n_sub, grp.max_xrows, max_pairs);
   if (phy_topo_create(nullptr, N, children, root, &sp->base) != PHY_OK)
     return set_err(ctx, PHY_EINVAL, "tbr incremental: base topology");
   sp->pl.base = make_plan(sp->base, m, n_pat, form & ~PHY_FUSED, 0);
@@ -3557,6 +3569,7 @@ int tbr_plan_create(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pa
     const bool only_s = slot_first[k + 1] - slot_first[k] == 1 && uniq[slot_first[k]] % M == s;
     tk[0] = only_s ? 0 : (int32_t)tprog[k].size() / 4;
     tk[1] = s;
+    tk[2] = grp.xbase[k];
     tk[3] = tdepth[k];
     tk[4] = t.sibling(s);
     tk[5] = t.parent[t.parent[s]] >= 0 ? 1 : 0;
@@ -3564,7 +3577,7 @@ int tbr_plan_create(phy_ctx* ctx, const phy_model* m, int32_t form, int64_t n_pa
     hxp[2 * k + 1] = q.p;
     hkx[k] = t.sibling(s);
     for (int64_t j = slot_first[k]; j < slot_first[k + 1]; ++j)
-      hwslot[(size_t)k * M + (size_t)(uniq[j] % M)] = (k % S) * wmax + (int32_t)(j - slot_first[k]);
+      hwslot[(size_t)k * M + (size_t)(uniq[j] % M)] = grp.xbase[k] + (int32_t)(j - slot_first[k]);
     sp->batch_depth[k / S] = std::max(sp->batch_depth[k / S], q.depth);
     if (!only_s) sp->batch_rdepth[k / S] = std::max(sp->batch_rdepth[k / S], tdepth[k]);
   }
@@ -3731,7 +3744,7 @@ int phy_lk_tbr_incr_work_size(int32_t N, const int32_t* children, int32_t root, 
     int64_t bad = -1;
     const int gr = tbr_group(t, n_moves, moves, s_batch, &g, &bad);
     if (gr) return gr == 1 ? PHY_EINVAL : PHY_ERANGE;
-    *bytes = incr_plan(t, m->R, n_pat, form, s_batch, g.n_sub, g.wmax, g.max_pairs).total;
+    *bytes = incr_plan(t, m->R, n_pat, form, s_batch, g.n_sub, g.max_xrows, g.max_pairs).total;
     return PHY_OK;
   } catch (...) {
     return PHY_ENOMEM;
