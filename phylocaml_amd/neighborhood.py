@@ -25,7 +25,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import device as D
-from ._lib import check, lib, ptr
+from ._lib import PhyError, check, lib, ptr
 
 
 def num_edges(n: int) -> int:
@@ -211,33 +211,44 @@ class Scorer:
                           device=f"cuda:{self.ctx.device}")
         tbr = mv.ndim == 2 and mv.shape[1] == 3
         if len(mv) and self.uses_incremental(mv):
-            nb = C.c_size_t()
-            if tbr:
-                check(lib.phy_lk_tbr_incr_work_size(
-                    self.N, ptr(ch, C.c_int32), int(root), self.dm.handle, self.n_patterns,
-                    self.form, len(mv), ptr(mv, C.c_int32), self.s_batch, C.byref(nb)),
-                    None, "tbr incremental work size")
-            else:
-                check(lib.phy_lk_spr_incr_work_size(
-                    self.N, ptr(ch, C.c_int32), int(root), self.dm.handle, self.n_patterns,
-                    self.form, self.s_batch, C.byref(nb)), None, "spr incremental work size")
-            if self.iwork.numel() < nb.value:
-                self.iwork = torch.empty((nb.value,), dtype=torch.uint8,
-                                         device=f"cuda:{self.ctx.device}")
-            fn = lib.phy_lk_tbr_scores_incr if tbr else lib.phy_lk_spr_scores_incr
-            self.ctx.check(fn(self.ctx.handle, self.dm.handle, self.form, self.n_patterns,
-                              D.dptr(self.tips), self.stride, D.dptr(self.weights), self.N,
-                              ptr(ch, C.c_int32), int(root), ptr(bl, C.c_double), len(mv),
-                              ptr(mv, C.c_int32), self.s_batch, D.dptr(out),
-                              D.dptr(self.base_lnl), D.dptr(self.iwork), self.iwork.numel()),
-                           "neighbour scores (incremental)")
-            return out[: len(mv)]
+            try:
+                return self._score_incremental(ch, root, bl, mv, tbr, out)
+            except PhyError as e:
+                # the automatic choice falls back to one candidate tree per
+                # move when the incremental form refuses the tree (an LDS
+                # stack too deep for it: PHY_ERANGE)
+                if self.incremental is not None or e.code != 5:
+                    raise
         fn = lib.phy_lk_tbr_scores if tbr else lib.phy_lk_spr_scores
         self.ctx.check(fn(self.ctx.handle, self.dm.handle, self.form, self.n_patterns,
                           D.dptr(self.tips), self.stride, D.dptr(self.weights), self.N,
                           ptr(ch, C.c_int32), int(root), ptr(bl, C.c_double), len(mv),
                           ptr(mv, C.c_int32), self.chunk, D.dptr(out), D.dptr(self.work),
                           self.work_bytes), "neighbour scores")
+        return out[: len(mv)]
+
+    def _score_incremental(self, ch, root, bl, mv, tbr, out):
+        torch = __import__("torch")
+        nb = C.c_size_t()
+        if tbr:
+            check(lib.phy_lk_tbr_incr_work_size(
+                self.N, ptr(ch, C.c_int32), int(root), self.dm.handle, self.n_patterns,
+                self.form, len(mv), ptr(mv, C.c_int32), self.s_batch, C.byref(nb)),
+                None, "tbr incremental work size")
+        else:
+            check(lib.phy_lk_spr_incr_work_size(
+                self.N, ptr(ch, C.c_int32), int(root), self.dm.handle, self.n_patterns,
+                self.form, self.s_batch, C.byref(nb)), None, "spr incremental work size")
+        if self.iwork.numel() < nb.value:
+            self.iwork = torch.empty((nb.value,), dtype=torch.uint8,
+                                     device=f"cuda:{self.ctx.device}")
+        fn = lib.phy_lk_tbr_scores_incr if tbr else lib.phy_lk_spr_scores_incr
+        self.ctx.check(fn(self.ctx.handle, self.dm.handle, self.form, self.n_patterns,
+                          D.dptr(self.tips), self.stride, D.dptr(self.weights), self.N,
+                          ptr(ch, C.c_int32), int(root), ptr(bl, C.c_double), len(mv),
+                          ptr(mv, C.c_int32), self.s_batch, D.dptr(out),
+                          D.dptr(self.base_lnl), D.dptr(self.iwork), self.iwork.numel()),
+                       "neighbour scores (incremental)")
         return out[: len(mv)]
 
 
