@@ -110,6 +110,17 @@ struct phy_topo {
   std::vector<int32_t> parts;      // cut node of each part
   SplitProg split_down, split_up;
   int32_t* d_part_parent = nullptr;
+  // split form of the fused pruning pass (its own cuts, fewer than the Fitch
+  // passes take: every part adds a stored root partial per pattern):
+  // psplit holds the part programs; the top program reads the cut nodes as
+  // stored operands (SRC_EXT; the entry's x field holds part(A) |
+  // part(B) << 16), its fifo the real leaves only (padded)
+  std::vector<int32_t> pparts;
+  SplitProg psplit;
+  std::vector<int32_t> ptop_prog, ptop_fifo;
+  int32_t ptop_depth = 0;
+  int4* d_ptop_prog = nullptr;
+  int32_t* d_ptop_fifo = nullptr;
   // per-part partial site costs of the split down-pass, grown on demand
   mutable uint32_t* d_scratch = nullptr;
   mutable size_t scratch_bytes = 0;

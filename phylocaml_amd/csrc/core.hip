@@ -391,9 +391,11 @@ int phy_compile_topo(phy_topo* t) {
 
   // ---- split programs: parts under cut nodes run side by side ----
   t->parts.clear();
+  // default: up to 16 parts of >= 64 leaves on average; PHY_FITCH_PARTS
+  // asks for a count (parts of >= 16 leaves)
   const char* env_parts = getenv("PHY_FITCH_PARTS");
-  const int max_parts = env_parts ? std::max(1, std::min(kMaxParts, atoi(env_parts))) : 16;
-  const int want = (int)std::min<int32_t>(max_parts, N / 64);
+  const int want = env_parts ? (int)std::min<int32_t>(std::max(1, std::min(kMaxParts, atoi(env_parts))), N / 16)
+                             : (int)std::min<int32_t>(16, N / 64);
   if (want >= 2) {
     const std::vector<int32_t> cuts = choose_cuts(N, ch, t->subtree, t->root, want, 16);
     if (cuts.size() >= 2) {
@@ -437,7 +439,85 @@ int phy_compile_topo(phy_topo* t) {
       for (int32_t c : cuts) t->parts.push_back(c);
     }
   }
+
+  // ---- split form of the fused pruning pass ----
+  // default: up to 16 parts of >= 128 leaves on average; PHY_PRUNE_PARTS
+  // asks for a count (parts of >= 16 leaves)
+  t->pparts.clear();
+  t->psplit = SplitProg();
+  t->ptop_prog.clear();
+  t->ptop_fifo.clear();
+  const char* env_pp = getenv("PHY_PRUNE_PARTS");
+  const int want_p = env_pp ? (int)std::min<int32_t>(std::max(1, std::min(kMaxParts, atoi(env_pp))), N / 16)
+                            : (int)std::min<int32_t>(16, N / 128);
+  if (want_p >= 2) {
+    const std::vector<int32_t> cuts = choose_cuts(N, ch, t->subtree, t->root, want_p, 16);
+    if (cuts.size() >= 2) {
+      const int P = (int)cuts.size();
+      std::vector<int32_t> slot(M, -1);
+      for (int k = 0; k < P; ++k) slot[cuts[k]] = k;
+      SplitProg& ps = t->psplit;
+      std::vector<int32_t> pg, ff;
+      int32_t d = 0, n_ops = 0;
+      for (int32_t c : cuts) {
+        if (!compile_down(N, ch, c, is_leaf, &pg, &ff, &d)) return 8;
+        ps.prog_off.push_back((int32_t)ps.prog.size() / 4);
+        ps.n_ops.push_back((int32_t)pg.size() / 4);
+        ps.fifo_off.push_back((int32_t)ps.fifo.size());
+        ps.prog.insert(ps.prog.end(), pg.begin(), pg.end());
+        std::vector<int32_t> pad(phy_fifo_padded(ff.size()), ff[0]);
+        std::copy(ff.begin(), ff.end(), pad.begin());
+        ps.fifo.insert(ps.fifo.end(), pad.begin(), pad.end());
+        ps.depth = std::max(ps.depth, d);
+        n_ops += (int32_t)pg.size() / 4;
+      }
+      // the top: cut nodes stop the walk; their "leaf" operands become
+      // stored operands and leave the leaf stream
+      const Stop stop_top = [N, &slot](int32_t v) { return v < N || slot[v] >= 0; };
+      if (!compile_down(N, ch, t->root, stop_top, &pg, &ff, &d)) return 8;
+      t->ptop_depth = d;
+      n_ops += (int32_t)pg.size() / 4;
+      if (n_ops != N - 1) return 9;
+      std::vector<int32_t> leaves;
+      for (size_t i = 0; i < pg.size() / 4; ++i) {
+        const int32_t A = pg[4 * i + 1], B = pg[4 * i + 2], fl = pg[4 * i + 3];
+        int sa = fl & 3, sb = (fl >> 2) & 3, ka = 0, kb = 0;
+        if (sa == SRC_LEAF && slot[A] >= 0) sa = SRC_EXT, ka = slot[A];
+        if (sb == SRC_LEAF && slot[B] >= 0) sb = SRC_EXT, kb = slot[B];
+        if (sa == SRC_LEAF) leaves.push_back(A);
+        if (sb == SRC_LEAF) leaves.push_back(B);
+        t->ptop_prog.insert(t->ptop_prog.end(), {ka | (kb << 16), A, B, (fl & ~15) | sa | (sb << 2)});
+      }
+      // the leaf stream must be the walk's own with the cut nodes left out
+      size_t j = 0;
+      for (int32_t v : ff) {
+        if (slot[v] >= 0) continue;
+        if (j >= leaves.size() || v != leaves[j]) return 10;
+        ++j;
+      }
+      if (j != leaves.size() || ff.size() != leaves.size() + cuts.size()) return 10;
+      t->ptop_fifo.assign(phy_fifo_padded(leaves.size()), leaves.empty() ? 0 : leaves[0]);
+      std::copy(leaves.begin(), leaves.end(), t->ptop_fifo.begin());
+      t->pparts = cuts;
+    }
+  }
   return 0;
+}
+
+// device copies of a split program: programs, leaf streams and the offsets
+// [prog_off | n_ops | fifo_off], one per program each
+static hipError_t upload_split(SplitProg* sp) {
+  std::vector<int32_t> meta;
+  meta.insert(meta.end(), sp->prog_off.begin(), sp->prog_off.end());
+  meta.insert(meta.end(), sp->n_ops.begin(), sp->n_ops.end());
+  meta.insert(meta.end(), sp->fifo_off.begin(), sp->fifo_off.end());
+  hipError_t e = hipMalloc((void**)&sp->d_prog, sp->prog.size() * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&sp->d_fifo, sp->fifo.size() * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&sp->d_meta, meta.size() * 4);
+  if (e == hipSuccess) e = hipMemcpy(sp->d_prog, sp->prog.data(), sp->prog.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(sp->d_fifo, sp->fifo.data(), sp->fifo.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(sp->d_meta, meta.data(), meta.size() * 4, hipMemcpyHostToDevice);
+  return e;
 }
 
 extern "C" {
@@ -504,20 +584,19 @@ int phy_topo_create(phy_ctx* ctx, int32_t N, const int32_t* children, int32_t ro
   if (e == hipSuccess && !t->parts.empty()) {
     std::vector<int32_t> pp(t->parts.size());
     for (size_t k = 0; k < t->parts.size(); ++k) pp[k] = t->parent[t->parts[k]];
-    for (SplitProg* sp : {&t->split_down, &t->split_up}) {
-      std::vector<int32_t> meta;  // [prog_off | n_ops | fifo_off] x (parts + 1)
-      meta.insert(meta.end(), sp->prog_off.begin(), sp->prog_off.end());
-      meta.insert(meta.end(), sp->n_ops.begin(), sp->n_ops.end());
-      meta.insert(meta.end(), sp->fifo_off.begin(), sp->fifo_off.end());
-      if (e == hipSuccess) e = hipMalloc((void**)&sp->d_prog, sp->prog.size() * 4);
-      if (e == hipSuccess) e = hipMalloc((void**)&sp->d_fifo, sp->fifo.size() * 4);
-      if (e == hipSuccess) e = hipMalloc((void**)&sp->d_meta, meta.size() * 4);
-      if (e == hipSuccess) e = hipMemcpy(sp->d_prog, sp->prog.data(), sp->prog.size() * 4, hipMemcpyHostToDevice);
-      if (e == hipSuccess) e = hipMemcpy(sp->d_fifo, sp->fifo.data(), sp->fifo.size() * 4, hipMemcpyHostToDevice);
-      if (e == hipSuccess) e = hipMemcpy(sp->d_meta, meta.data(), meta.size() * 4, hipMemcpyHostToDevice);
-    }
+    for (SplitProg* sp : {&t->split_down, &t->split_up})
+      if (e == hipSuccess) e = upload_split(sp);
     if (e == hipSuccess) e = hipMalloc((void**)&t->d_part_parent, pp.size() * 4);
     if (e == hipSuccess) e = hipMemcpy(t->d_part_parent, pp.data(), pp.size() * 4, hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess && !t->pparts.empty()) {
+    if (e == hipSuccess) e = upload_split(&t->psplit);
+    if (e == hipSuccess) e = hipMalloc((void**)&t->d_ptop_prog, t->ptop_prog.size() * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&t->d_ptop_fifo, t->ptop_fifo.size() * 4);
+    if (e == hipSuccess)
+      e = hipMemcpy(t->d_ptop_prog, t->ptop_prog.data(), t->ptop_prog.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(t->d_ptop_fifo, t->ptop_fifo.data(), t->ptop_fifo.size() * 4, hipMemcpyHostToDevice);
   }
   if (e != hipSuccess) {
     phy_topo_destroy(t);
@@ -533,12 +612,14 @@ void phy_topo_destroy(phy_topo* t) {
   if (t->d_down_fifo) (void)hipFree(t->d_down_fifo);
   if (t->d_up_prog) (void)hipFree(t->d_up_prog);
   if (t->d_up_fifo) (void)hipFree(t->d_up_fifo);
-  for (SplitProg* sp : {&t->split_down, &t->split_up}) {
+  for (SplitProg* sp : {&t->split_down, &t->split_up, &t->psplit}) {
     if (sp->d_prog) (void)hipFree(sp->d_prog);
     if (sp->d_fifo) (void)hipFree(sp->d_fifo);
     if (sp->d_meta) (void)hipFree(sp->d_meta);
   }
   if (t->d_part_parent) (void)hipFree(t->d_part_parent);
+  if (t->d_ptop_prog) (void)hipFree(t->d_ptop_prog);
+  if (t->d_ptop_fifo) (void)hipFree(t->d_ptop_fifo);
   if (t->d_scratch) (void)hipFree(t->d_scratch);
   delete t;
 }

@@ -334,10 +334,18 @@ __device__ __forceinline__ void rescale4(real* v, int* e) {
 // stack addressing are shared by G classes.  The bottom LV levels of the
 // fused stack live in registers (selected by the wave-uniform stack pointer),
 // so the LDS holds LV levels less.
-template <typename real, int R, int G, bool FUSED, int LV = 0>
+// MODE 0 runs the whole tree.  The split form runs it in two launches: MODE 1
+// evaluates one part (a subtree, blockIdx.y) and stores its root partial in
+// slot y of the part rows; MODE 2 evaluates the top program, whose SRC_EXT
+// operands read those rows (slot of A in the low, of B in the high 16 bits
+// of the entry's first field).  Part row of slot k, class r:
+// xt[((k * R + r) * 4 + i) * pitch + p], exponent xte[(k * R + r) * pitch + p].
+template <typename real, int R, int G, bool FUSED, int LV = 0, int MODE = 0>
 struct DownState {
   static_assert(R % G == 0, "rate classes per wave must divide R");
   static_assert(FUSED || LV == 0, "register stack levels belong to the fused form");
+  static_assert(MODE == 0 || (FUSED && G == 1), "the split form is fused, one class per wave");
+  static constexpr int kR = R, kG = G;
   static constexpr int BD = 64 * (R / G);
   static constexpr int NB = LV > 0 ? LV : 1;
   real acc[G][4];
@@ -352,7 +360,29 @@ struct DownState {
   bool act;
   real* part;
   int32_t* ex;
+  real* xt;
+  int32_t* xte;
   TipNibbles ts;
+
+  // one class's stored part partial (slot k) times P
+  __device__ __forceinline__ void ext_product(int k, const real* P, real* y, int* e, int g) {
+    const real* sv = xt + (((size_t)k * R + r0 + g) * 4) * pitch + pc;
+    real x[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) x[i] = sv[(size_t)i * pitch];
+    *e = xte[((size_t)k * R + r0 + g) * pitch + pc];
+    matvec4(P, x, y);
+  }
+  // store the accumulator (the part's root partial) in slot k
+  __device__ __forceinline__ void ext_store(int k) const {
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      real* d = xt + (((size_t)k * R + r0 + g) * 4) * pitch + p;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) d[(size_t)i * pitch] = acc[g][i];
+      xte[((size_t)k * R + r0 + g) * pitch + p] = acc_e[g];
+    }
+  }
 
   __device__ __forceinline__ void spill_if(int fl) {
     if (FUSED && (fl & DOWN_SPILL)) {
@@ -379,9 +409,13 @@ struct DownState {
       ++sp;
     }
   }
-  // y[g] = P[g] x[g] for operand A (leaf, accumulator, stack or stored child a)
-  __device__ __forceinline__ void product_a(int sa, int a, const real (*P)[16], real (*y)[4], int* e) {
-    if (sa == SRC_LEAF) {
+  // y[g] = P[g] x[g] for operand A (leaf, accumulator, stack, stored child a
+  // or part row ka)
+  __device__ __forceinline__ void product_a(int sa, int a, int ka, const real (*P)[16], real (*y)[4],
+                                            int* e) {
+    if (MODE == 2 && sa == SRC_EXT) {
+      ext_product(ka, P[0], y[0], &e[0], 0);
+    } else if (sa == SRC_LEAF) {
       real x[4];
       tipvec(ts.get(), x);
 #pragma unroll
@@ -436,9 +470,11 @@ struct DownState {
       }
     }
   }
-  // operand B (leaf or accumulator)
-  __device__ __forceinline__ void product_b(int sb, const real (*P)[16], real (*y)[4], int* e) {
-    if (sb == SRC_LEAF) {
+  // operand B (leaf, accumulator or part row kb)
+  __device__ __forceinline__ void product_b(int sb, int kb, const real (*P)[16], real (*y)[4], int* e) {
+    if (MODE == 2 && sb == SRC_EXT) {
+      ext_product(kb, P[0], y[0], &e[0], 0);
+    } else if (sb == SRC_LEAF) {
       real x[4];
       tipvec(ts.get(), x);
 #pragma unroll
@@ -473,12 +509,13 @@ struct DownState {
 };
 
 // root: S_r = sum_i pi_i L[r][i]; combine the R classes in wave 0
-template <typename real, int R, int G, bool FUSED, int LV>
-__device__ __forceinline__ void down_root(const DownState<real, R, G, FUSED, LV>& st, unsigned char* lds_raw,
+template <class State>
+__device__ __forceinline__ void down_root(const State& st, unsigned char* lds_raw,
                                           const double* __restrict__ pi,
                                           const double* __restrict__ probs,
                                           const uint32_t* __restrict__ weights,
                                           double* __restrict__ site_lnl, double* __restrict__ wave_sums) {
+  constexpr int R = State::kR, G = State::kG;
   const int lane = st.tid & 63;
   double* rootS = reinterpret_cast<double*>(lds_raw);
   int* rootE = reinterpret_cast<int*>(rootS + R * 64);
@@ -520,8 +557,8 @@ __device__ __forceinline__ void down_root(const DownState<real, R, G, FUSED, LV>
   }
 }
 
-template <typename real, int R, int G, bool FUSED, int LV>
-__device__ __forceinline__ void down_init(DownState<real, R, G, FUSED, LV>& st, unsigned char* lds_raw,
+template <typename real, int R, int G, bool FUSED, int LV, int MODE>
+__device__ __forceinline__ void down_init(DownState<real, R, G, FUSED, LV, MODE>& st, unsigned char* lds_raw,
                                           const int32_t* leaf_ids, const uint64_t* tips,
                                           int64_t stride, int64_t n_pat, int64_t pitch, real* part,
                                           int32_t* ex, int N, int stack_depth) {
@@ -534,6 +571,8 @@ __device__ __forceinline__ void down_init(DownState<real, R, G, FUSED, LV>& st, 
   st.N = N;
   st.part = part;
   st.ex = ex;
+  st.xt = nullptr;
+  st.xte = nullptr;
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     st.acc_e[g] = 0;
@@ -551,16 +590,10 @@ __device__ __forceinline__ void down_init(DownState<real, R, G, FUSED, LV>& st, 
 
 // The wave's 4x4 P of both children for each of its G classes are
 // wave-uniform (SGPRs): 16 FMAs with an SGPR operand per product.
-template <typename real, int R, int G, bool FUSED, int LV>
-__device__ __forceinline__ void prune_down_body(
-    const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids,
-    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
-    const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
-    real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
-    double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth) {
-  extern __shared__ unsigned char lds_raw[];
-  DownState<real, R, G, FUSED, LV> st;
-  down_init(st, lds_raw, leaf_ids, tips, stride, n_pat, pitch, part, ex, N, stack_depth);
+template <class State, typename real>
+__device__ __forceinline__ void down_program(State& st, const int4* __restrict__ prog, int n_ops,
+                                             const real* __restrict__ Pm) {
+  constexpr int R = State::kR, G = State::kG;
   const int r0 = st.r0;
   for_each_entry(prog, n_ops, [&](int, const int4 op) {
     real PA[G][16], PB[G][16];
@@ -573,22 +606,65 @@ __device__ __forceinline__ void prune_down_body(
     st.spill_if(fl);
     real xa[G][4], xb[G][4];
     int ea[G], eb[G];
-    st.product_a(fl & 3, op.y, PA, xa, ea);
-    st.product_b((fl >> 2) & 3, PB, xb, eb);
+    st.product_a(fl & 3, op.y, op.x & 0xffff, PA, xa, ea);
+    st.product_b((fl >> 2) & 3, op.x >> 16, PB, xb, eb);
     st.combine(op.x, xa, xb, ea, eb);
   });
+}
+
+template <typename real, int R, int G, bool FUSED, int LV, int MODE = 0>
+__device__ __forceinline__ void prune_down_body(
+    const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids,
+    const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
+    const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
+    real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
+    double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth,
+    real* __restrict__ xt = nullptr, int32_t* __restrict__ xte = nullptr) {
+  extern __shared__ unsigned char lds_raw[];
+  DownState<real, R, G, FUSED, LV, MODE> st;
+  down_init(st, lds_raw, leaf_ids, tips, stride, n_pat, pitch, part, ex, N, stack_depth);
+  st.xt = xt;
+  st.xte = xte;
+  down_program(st, prog, n_ops, Pm);
   down_root(st, lds_raw, pi, probs, weights, site_lnl, wave_sums);
 }
 
-template <typename real, int R, int G, bool FUSED, int LV>
+// Whole tree (MODE 0) or the top program of the split form (MODE 2).
+template <typename real, int R, int G, bool FUSED, int LV, int MODE = 0>
 __global__ __launch_bounds__(512) void k_prune_down(
     const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids,
     const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
     const real* __restrict__ Pm, const double* __restrict__ pi, const double* __restrict__ probs,
     real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
-    double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth) {
-  prune_down_body<real, R, G, FUSED, LV>(prog, n_ops, leaf_ids, tips, stride, n_pat, pitch, Pm, pi,
-                                     probs, part, ex, weights, site_lnl, wave_sums, N, stack_depth);
+    double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth,
+    real* __restrict__ xt, int32_t* __restrict__ xte) {
+  prune_down_body<real, R, G, FUSED, LV, MODE>(prog, n_ops, leaf_ids, tips, stride, n_pat, pitch, Pm, pi,
+                                           probs, part, ex, weights, site_lnl, wave_sums, N,
+                                           stack_depth, xt, xte);
+}
+
+// The parts of the split form side by side: block (x, y) runs part y over
+// patterns 64x .. 64x+63 and stores the part's root partial in slot y.  The
+// part programs and leaf streams sit at the offsets of meta (prog_off |
+// n_ops | fifo_off, n_parts each).
+template <typename real, int R, int LV>
+__global__ __launch_bounds__(512) void k_prune_part(
+    const int4* __restrict__ prog, const int32_t* __restrict__ meta, int n_parts,
+    const int32_t* __restrict__ fifo, const uint64_t* __restrict__ tips, int64_t stride,
+    int64_t n_pat, int64_t pitch, const real* __restrict__ Pm, real* __restrict__ xt,
+    int32_t* __restrict__ xte, int N, int stack_depth) {
+  extern __shared__ unsigned char lds_raw[];
+  const int y = blockIdx.y;
+  const int4* pg = prog + meta[y];
+  const int n_ops = meta[n_parts + y];
+  const int32_t* li = fifo + meta[2 * n_parts + y];
+  DownState<real, R, 1, true, LV, 1> st;
+  down_init(st, lds_raw, li, tips, stride, n_pat, pitch, (real*)nullptr, (int32_t*)nullptr, N,
+            stack_depth);
+  st.xt = xt;
+  st.xte = xte;
+  down_program(st, pg, n_ops, Pm);
+  st.ext_store(y);
 }
 
 // Many candidate topologies over the same tips and weights (fused form):
@@ -2258,6 +2334,10 @@ struct Plan {
   int rpw = 1;          // rate classes per wave of the post-order pass
   size_t esz = 8, lds = 0, dlds = 0;
   bool fused = false;
+  // split form of the fused pass: the parts side by side, then the top
+  bool split = false;
+  int n_parts = 0;
+  size_t off_xt = 0, off_xte = 0;
 };
 
 constexpr size_t kLdsMax = 64 * 1024;
@@ -2285,7 +2365,15 @@ int fit_block(int want, F lds_of, int min_bs = 64) {
   return 0;
 }
 
-Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t form, int32_t derivs) {
+// The fused pass runs in split form (parts side by side, then the top) when
+// the topology has parts; PHY_PRUNE_SPLIT=0 runs the whole tree per block.
+bool split_env() {
+  const char* s = getenv("PHY_PRUNE_SPLIT");
+  return !(s && s[0] == '0' && s[1] == 0);
+}
+
+Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t form, int32_t derivs,
+               bool allow_split = true) {
   Plan pl;
   const int N = t->n_leaves, M = 2 * N - 1, R = m->R, K = m->K;
   pl.esz = (form & PHY_F32) ? 4 : 8;
@@ -2319,6 +2407,13 @@ Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t for
   if (!pl.fused) {
     pl.off_part = o; o = align256(o + (size_t)(N - 1) * R * K * pl.pitch * pl.esz);
     pl.off_ex = o; o = align256(o + (size_t)(N - 1) * R * pl.pitch * 4);
+  }
+  pl.split = allow_split && pl.fused && pl.rpw == 1 && t->pparts.size() >= 2 && t->d_ptop_prog &&
+             down_lds(R, esz, true, std::max(t->psplit.depth, t->ptop_depth)) <= kLdsMax && split_env();
+  if (pl.split) {
+    pl.n_parts = (int)t->pparts.size();
+    pl.off_xt = o; o = align256(o + (size_t)pl.n_parts * R * K * pl.pitch * pl.esz);
+    pl.off_xte = o; o = align256(o + (size_t)pl.n_parts * R * pl.pitch * 4);
   }
   pl.off_bsum = o; o = align256(o + (size_t)pl.n_waves * 8);
   if (derivs) {
@@ -2388,13 +2483,39 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
   auto down = [&](auto kern, size_t lds) {
     hipLaunchKernelGGL(kern, dim3((unsigned)pl.n_blocks), dim3(pl.bs), lds, ctx->stream,
                        t->d_down_prog, N - 1, t->d_down_fifo, tips, stride, n_pat, pl.pitch, P, d_pi,
-                       d_probs, part, ex, weights, site_lnl, bsum, N, t->down_depth);
+                       d_probs, part, ex, weights, site_lnl, bsum, N, t->down_depth, (real*)nullptr,
+                       (int32_t*)nullptr);
   };
   constexpr int G2 = (R % 2 == 0 && sizeof(real) == 4) ? 2 : 1;
   const int lv = (fused && pl.rpw != 2) ? register_levels<real, R>(t->down_depth, pl.bs) : 0;
   {
     KernelSpan span(ctx, "prune_down");
-    if (!fused) {
+    if (pl.split) {
+      const SplitProg& sd = t->psplit;
+      const int np = pl.n_parts;
+      real* xt = reinterpret_cast<real*>(work + pl.off_xt);
+      int32_t* xte = reinterpret_cast<int32_t*>(work + pl.off_xte);
+      const int lvp = register_levels<real, R>(sd.depth, pl.bs);
+      const size_t ldsp = down_lds(R, sizeof(real), true, sd.depth, lvp);
+      auto parts = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3((unsigned)pl.n_blocks, (unsigned)np), dim3(pl.bs), ldsp, ctx->stream,
+                           sd.d_prog, sd.d_meta, np, sd.d_fifo, tips, stride, n_pat, pl.pitch, P, xt, xte,
+                           N, sd.depth);
+      };
+      if (lvp == 2)
+        parts(k_prune_part<real, R, 2>);
+      else if (lvp == 1)
+        parts(k_prune_part<real, R, 1>);
+      else
+        parts(k_prune_part<real, R, 0>);
+      PHY_LAUNCHED(ctx);
+      const int dt = std::max(t->ptop_depth, 1);
+      hipLaunchKernelGGL((k_prune_down<real, R, 1, true, 0, 2>), dim3((unsigned)pl.n_blocks), dim3(pl.bs),
+                         down_lds(R, sizeof(real), true, dt), ctx->stream, t->d_ptop_prog,
+                         (int)t->ptop_prog.size() / 4,
+                         t->d_ptop_fifo, tips, stride, n_pat, pl.pitch, P, d_pi, d_probs, (real*)nullptr,
+                         (int32_t*)nullptr, weights, site_lnl, bsum, N, dt, xt, xte);
+    } else if (!fused) {
       pl.rpw == 2 ? down(k_prune_down<real, R, G2, false, 0>, pl.lds)
                   : down(k_prune_down<real, R, 1, false, 0>, pl.lds);
     } else if (pl.rpw == 2) {
@@ -2557,7 +2678,7 @@ int phy_lk_tree_noalloc(phy_ctx* ctx, const phy_topo* t, const phy_model* m, int
   if (n_pat < 1) return set_err(ctx, PHY_EINVAL, "lk_tree_noalloc: need at least one pattern");
   if (stride < phy_words(n_pat)) return set_err(ctx, PHY_EINVAL, "lk_tree_noalloc: tip stride < words");
   if (!tips || !weights || !blen || !lnl) return set_err(ctx, PHY_EINVAL, "lk_tree_noalloc: null argument");
-  const Plan pl = make_plan(t, m, n_pat, form, 0);
+  const Plan pl = make_plan(t, m, n_pat, form, 0, false);
   if (!pl.fused)
     return set_err(ctx, PHY_ERANGE, "lk_tree_noalloc: the tree's stack does not fit the fused form");
   if (pl.total > kNoAllocScratchBytes)

@@ -317,3 +317,31 @@ def test_unsupported_rate_class_count_is_refused(ctx, rng):
         ev = LK.Evaluator(ctx, topo, dm, P, F64)
         ev.run(tips, D.words(P), w, bl)
     assert e.value.code == 5
+
+
+@pytest.mark.parametrize("parts", ["2", "8", "16"])
+@pytest.mark.parametrize("R", [1, 4, 8])
+def test_split_form_is_bit_identical(ctx, rng, monkeypatch, R, parts):
+    # the split form (parts side by side, then the top program reading the
+    # parts' root partials) performs the whole-tree pass's arithmetic in the
+    # same order: its lnL and site lnL equal the whole-tree fused form's bit
+    # for bit, in both precisions, and the oracle's within tolerance
+    monkeypatch.setenv("PHY_PRUNE_PARTS", parts)
+    N = 1100
+    _say(f"prune split form R={R} parts<={parts} N={N}")
+    ch, root = random_tree(rng, N)
+    P = 777
+    bl = rng.exponential(0.1, 2 * N - 1) + 1e-3
+    bl[root] = 0.0
+    model = _model(rng, R)
+    tips = _tips(rng, N, P)
+    w = rng.integers(1, 6, P).astype(np.uint32)
+    ov = _oracle(N, ch, root, bl, model, tips, w, False)[0]
+    for form, tol in ((F32 | FUSED, 1e-6), (F64 | FUSED, 1e-10)):
+        monkeypatch.setenv("PHY_PRUNE_SPLIT", "0")
+        whole = _run(ctx, N, ch, root, bl, model, tips, w, form, False)
+        monkeypatch.setenv("PHY_PRUNE_SPLIT", "1")
+        split = _run(ctx, N, ch, root, bl, model, tips, w, form, False)
+        assert split["lnl"] == whole["lnl"], (form, split["lnl"], whole["lnl"])
+        assert np.array_equal(split["site"], whole["site"])
+        assert abs(split["lnl"] - ov) <= tol * abs(ov), (form, split["lnl"], ov)
