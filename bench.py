@@ -156,8 +156,8 @@ def stream_copy(torch, ctx, clock, gib=2):
     log(f"  copy_stream: {gib} GiB in {k_ms:.3f} ms = {gbs:.0f} GB/s (read + write)")
     return {"bytes": nbytes, "kernel_ms": k_ms, "read_plus_write_GBs": gbs,
             "frac_of_peak": gbs / HBM_PEAK_GBS,
-            "kernel": "k_copy_stream: 16-byte non-temporal loads and stores, grid-stride, "
-                      "4 workgroups of 256 threads per CU"}
+            "kernel": "k_copy_stream: one 16-byte non-temporal load and store per thread, "
+                      "a grid of 256-thread workgroups covering the buffer"}
 
 
 def bench_prune(torch, ctx, clock, steps, warm, cpu_baseline):
@@ -780,9 +780,12 @@ def main():
     ms_main, k_main = pr["step_ms"]["f32_fused"], pr["kernel_ms"]["f32_fused"]
     value = units / (ms_main / 1e3)
     tflops = PRUNE_FLOPS * units / (k_main / 1e3) / 1e12
-    # the 512 x 200k launch: pitch / 64 blocks of 64 R threads
+    # the 512 x 200k pass in split form: the parts side by side (one launch),
+    # then the top program (pitch / 64 blocks of 64 R threads)
     grid = ((pr["P"] + 255) // 256) * 256 * 4
-    traffic = measured_traffic("k_prune_down<float, 4, 1, true,", grid)
+    t_parts = measured_traffic("k_prune_part<float, 4,")
+    t_top = measured_traffic("k_prune_down<float, 4, 1, true, 0, 2>", grid)
+    traffic = None if t_parts is None or t_top is None else t_parts + t_top
     line = {
         "metric": "FP32-scaled vs FP64 pruning updates/s at rel ΔlnL≤1e-6; "
                   "Fitch site-node/s",
@@ -810,12 +813,14 @@ def main():
                      "hbm_frac": None if traffic is None
                      else traffic / (k_main / 1e3) / 1e9 / HBM_PEAK_GBS,
                      "note": "achieved = 272 flop per internal pattern.node x 511 nodes x "
-                             "200,000 patterns / mean k_prune_down duration (HIP events "
-                             "around each launch); packed FP32 is kept out of the build, so "
+                             "200,000 patterns / mean duration of the fused pass (HIP events "
+                             "around its two launches: k_prune_part, the tree's 4 parts "
+                             "side by side, then the top program k_prune_down<float, 4, 1, "
+                             "true, 0, 2>); packed FP32 is kept out of the build, so "
                              "78.6 TFLOP/s is the reachable vector peak; traffic = HBM bytes "
-                             "per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE "
-                             "(copy-calibrated): the fused kernel keeps interior partials "
-                             "on chip"},
+                             "per pass from rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                             "(copy-calibrated) of both kernels: interior partials stay "
+                             "on chip, the parts' root partials go through HBM"},
         "cpu_baseline": cpu,
         "prune": {
             "step_ms": pr["step_ms"], "kernel_ms": pr["kernel_ms"],
