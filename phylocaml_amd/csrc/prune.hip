@@ -238,6 +238,18 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   return (readlane(v, 0) + readlane(v, 16)) + (readlane(v, 32) + readlane(v, 48));
 }
 
+// XCD-aware tile order of the split form's launches.  Workgroups are dealt
+// round-robin to the 8 XCDs (linear id L to XCD L % 8), so consecutive
+// 64-pattern tiles would sit in 8 different L2s, each fetching the 128-byte
+// lines of tip nibbles that 4 neighbouring tiles share.  Block L takes tile
+// t = 32 (i / 4) + 4 x + i % 4 (x = L % 8, i = L / 8): the 4 tiles of a line
+// run on one XCD, and tiles still start in about their order.  A bijection
+// on [0, 32 k); the grid is padded to a multiple of 32.
+__device__ __forceinline__ int64_t xcd_tile(int64_t L) {
+  const int64_t x = L & 7, i = L >> 3;
+  return ((i >> 2) << 5) + (x << 2) + (i & 3);
+}
+
 // Tip codes of the post-order leaf stream, 8 per packed register: the loads
 // of batch k+2 are in flight while batch k is consumed by shifts (no LDS).
 // The id stream is padded (phy_fifo_padded): up to 24 ids past the last one
@@ -356,7 +368,7 @@ struct DownState {
   real* stk;       // [depth - LV][G * 4][BD]
   int32_t* stke;   // [depth - LV][G][BD]
   int tid, r0, N;
-  int64_t pitch, pc, p;
+  int64_t tile, pitch, pc, p;  // tile: the block's 64 patterns
   bool act;
   real* part;
   int32_t* ex;
@@ -553,7 +565,7 @@ __device__ __forceinline__ void down_root(const State& st, unsigned char* lds_ra
     }
     // one partial sum per 64 patterns, independent of the launch shape
     const double ws = wave_sum_d(contrib);
-    if (lane == 0) wave_sums[blockIdx.x] = ws;
+    if (lane == 0) wave_sums[st.tile] = ws;
   }
 }
 
@@ -561,10 +573,11 @@ template <typename real, int R, int G, bool FUSED, int LV, int MODE>
 __device__ __forceinline__ void down_init(DownState<real, R, G, FUSED, LV, MODE>& st, unsigned char* lds_raw,
                                           const int32_t* leaf_ids, const uint64_t* tips,
                                           int64_t stride, int64_t n_pat, int64_t pitch, real* part,
-                                          int32_t* ex, int N, int stack_depth) {
+                                          int32_t* ex, int N, int stack_depth, int64_t tile) {
   st.tid = threadIdx.x;
   st.r0 = __builtin_amdgcn_readfirstlane((st.tid >> 6) * G);  // first rate class of this wave
-  st.p = (int64_t)blockIdx.x * 64 + (st.tid & 63);
+  st.tile = tile;
+  st.p = tile * 64 + (st.tid & 63);
   st.act = st.p < n_pat;
   st.pc = st.act ? st.p : 0;
   st.pitch = pitch;
@@ -621,8 +634,13 @@ __device__ __forceinline__ void prune_down_body(
     double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth,
     real* __restrict__ xt = nullptr, int32_t* __restrict__ xte = nullptr) {
   extern __shared__ unsigned char lds_raw[];
+  int64_t tile = blockIdx.x;
+  if constexpr (MODE == 2) {
+    tile = xcd_tile(blockIdx.x);
+    if (tile >= pitch / 64) return;  // padding of the grid (whole block)
+  }
   DownState<real, R, G, FUSED, LV, MODE> st;
-  down_init(st, lds_raw, leaf_ids, tips, stride, n_pat, pitch, part, ex, N, stack_depth);
+  down_init(st, lds_raw, leaf_ids, tips, stride, n_pat, pitch, part, ex, N, stack_depth, tile);
   st.xt = xt;
   st.xte = xte;
   down_program(st, prog, n_ops, Pm);
@@ -643,10 +661,11 @@ __global__ __launch_bounds__(512) void k_prune_down(
                                            stack_depth, xt, xte);
 }
 
-// The parts of the split form side by side: block (x, y) runs part y over
-// patterns 64x .. 64x+63 and stores the part's root partial in slot y.  The
-// part programs and leaf streams sit at the offsets of meta (prog_off |
-// n_ops | fifo_off, n_parts each).
+// The parts of the split form side by side: tile t = y * (pitch / 64) + x
+// runs part y over patterns 64x .. 64x+63 and stores the part's root partial
+// in slot y (tiles in XCD-aware order over a 1-D grid padded to a multiple
+// of 32).  The part programs and leaf streams sit at the offsets of meta
+// (prog_off | n_ops | fifo_off, n_parts each).
 template <typename real, int R, int LV>
 __global__ __launch_bounds__(512) void k_prune_part(
     const int4* __restrict__ prog, const int32_t* __restrict__ meta, int n_parts,
@@ -654,13 +673,16 @@ __global__ __launch_bounds__(512) void k_prune_part(
     int64_t n_pat, int64_t pitch, const real* __restrict__ Pm, real* __restrict__ xt,
     int32_t* __restrict__ xte, int N, int stack_depth) {
   extern __shared__ unsigned char lds_raw[];
-  const int y = blockIdx.y;
+  const int64_t nb = pitch / 64;
+  const int64_t t = xcd_tile(blockIdx.x);
+  if (t >= nb * n_parts) return;  // padding of the grid (whole block)
+  const int y = (int)(t / nb);
   const int4* pg = prog + meta[y];
   const int n_ops = meta[n_parts + y];
   const int32_t* li = fifo + meta[2 * n_parts + y];
   DownState<real, R, 1, true, LV, 1> st;
   down_init(st, lds_raw, li, tips, stride, n_pat, pitch, (real*)nullptr, (int32_t*)nullptr, N,
-            stack_depth);
+            stack_depth, t - (int64_t)y * nb);
   st.xt = xt;
   st.xte = xte;
   down_program(st, pg, n_ops, Pm);
@@ -2497,8 +2519,9 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
       int32_t* xte = reinterpret_cast<int32_t*>(work + pl.off_xte);
       const int lvp = register_levels<real, R>(sd.depth, pl.bs);
       const size_t ldsp = down_lds(R, sizeof(real), true, sd.depth, lvp);
+      auto pad32 = [](int64_t n) { return (unsigned)((n + 31) / 32 * 32); };
       auto parts = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3((unsigned)pl.n_blocks, (unsigned)np), dim3(pl.bs), ldsp, ctx->stream,
+        hipLaunchKernelGGL(kern, dim3(pad32(pl.n_blocks * np)), dim3(pl.bs), ldsp, ctx->stream,
                            sd.d_prog, sd.d_meta, np, sd.d_fifo, tips, stride, n_pat, pl.pitch, P, xt, xte,
                            N, sd.depth);
       };
@@ -2510,7 +2533,7 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
         parts(k_prune_part<real, R, 0>);
       PHY_LAUNCHED(ctx);
       const int dt = std::max(t->ptop_depth, 1);
-      hipLaunchKernelGGL((k_prune_down<real, R, 1, true, 0, 2>), dim3((unsigned)pl.n_blocks), dim3(pl.bs),
+      hipLaunchKernelGGL((k_prune_down<real, R, 1, true, 0, 2>), dim3(pad32(pl.n_blocks)), dim3(pl.bs),
                          down_lds(R, sizeof(real), true, dt), ctx->stream, t->d_ptop_prog,
                          (int)t->ptop_prog.size() / 4,
                          t->d_ptop_fifo, tips, stride, n_pat, pl.pitch, P, d_pi, d_probs, (real*)nullptr,
