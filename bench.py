@@ -781,10 +781,11 @@ def main():
     value = units / (ms_main / 1e3)
     tflops = PRUNE_FLOPS * units / (k_main / 1e3) / 1e12
     # the 512 x 200k pass in split form: the parts side by side (one launch),
-    # then the top program (pitch / 64 blocks of 64 R threads)
+    # then the top program; both grids are padded for the XCD-aware tile
+    # order, and the traffic file keeps each kernel's largest launches
     grid = ((pr["P"] + 255) // 256) * 256 * 4
     t_parts = measured_traffic("k_prune_part<float, 4,")
-    t_top = measured_traffic("k_prune_down<float, 4, 1, true, 0, 2>", grid)
+    t_top = measured_traffic("k_prune_down<float, 4, 1, true, 0, 2>")
     traffic = None if t_parts is None or t_top is None else t_parts + t_top
     line = {
         "metric": "FP32-scaled vs FP64 pruning updates/s at rel ΔlnL≤1e-6; "

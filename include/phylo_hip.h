@@ -328,7 +328,9 @@ int phy_lk_tree_work_size(const phy_topo* topo, const phy_model* model,
                           size_t* bytes);
 
 /* Whole-tree evaluation (Node.Make1D.median_2 in post-order + root_cost,
- * lib/node.ml:183-198,272), one launch for the partials:
+ * lib/node.ml:183-198,272), one launch for the partials (PHY_FUSED on a tree
+ * of >= 256 leaves that splits into parts: two, the parts side by side and
+ * then the top of the tree, with the same result bit for bit):
  *   d_tips     N rows x stride words: tip state sets (nibble blocks; a gap is
  *              all four bits = missing data).
  *   d_weights  n_patterns uint32 pattern weights.
@@ -359,9 +361,10 @@ int phy_lk_tree(phy_ctx* ctx, const phy_topo* topo, const phy_model* model,
  * every branch and rate class, one partial sum per 64 patterns) lives in a
  * fixed 8 MiB device array per GPU, resolved when a context is created and
  * held until the process ends; a tree and pattern count whose workspace
- * (phy_lk_tree_work_size) exceeds it, or whose stack does not fit the fused
- * form, gives PHY_ERANGE.  Results are those of phy_lk_tree with the same
- * arguments.  Every call on one GPU shares that array: calls that overlap on
+ * (the P matrices and the sums, at most phy_lk_tree_work_size) exceeds it,
+ * or whose stack does not fit the fused form, gives PHY_ERANGE.  It runs the
+ * whole tree per launch (no part rows); results are those of phy_lk_tree
+ * with the same arguments, bit for bit.  Every call on one GPU shares that array: calls that overlap on
  * one GPU (other streams, other contexts) race. */
 int phy_lk_tree_noalloc(phy_ctx* ctx, const phy_topo* topo, const phy_model* model,
                         int32_t form, int64_t n_patterns, const uint64_t* d_tips,
