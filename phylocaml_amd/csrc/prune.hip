@@ -352,11 +352,13 @@ __device__ __forceinline__ void rescale4(real* v, int* e) {
 // operands read those rows (slot of A in the low, of B in the high 16 bits
 // of the entry's first field).  Part row of slot k, class r:
 // xt[((k * R + r) * 4 + i) * pitch + p], exponent xte[(k * R + r) * pitch + p].
-template <typename real, int R, int G, bool FUSED, int LV = 0, int MODE = 0>
+// STORE (with FUSED): the operand stack stays in LDS and every partial is
+// written too, so the stored form writes its partials and reads none back.
+template <typename real, int R, int G, bool FUSED, int LV = 0, int MODE = 0, bool STORE = false>
 struct DownState {
   static_assert(R % G == 0, "rate classes per wave must divide R");
   static_assert(FUSED || LV == 0, "register stack levels belong to the fused form");
-  static_assert(MODE == 0 || (FUSED && G == 1), "the split form is fused, one class per wave");
+  static_assert(MODE == 0 || (FUSED && G == 1 && !STORE), "the split form is fused, one class per wave");
   static constexpr int kR = R, kG = G;
   static constexpr int BD = 64 * (R / G);
   static constexpr int NB = LV > 0 ? LV : 1;
@@ -510,7 +512,7 @@ struct DownState {
       for (int k = 0; k < 4; ++k) acc[g][k] = xa[g][k] * xb[g][k];
       acc_e[g] = ea[g] + eb[g];
       rescale4(acc[g], &acc_e[g]);
-      if (!FUSED && act) {
+      if ((!FUSED || STORE) && act) {
         real* d = part + (((size_t)(v - N) * R + r0 + g) * 4) * pitch + p;
 #pragma unroll
         for (int k = 0; k < 4; ++k) d[(size_t)k * pitch] = acc[g][k];
@@ -569,8 +571,8 @@ __device__ __forceinline__ void down_root(const State& st, unsigned char* lds_ra
   }
 }
 
-template <typename real, int R, int G, bool FUSED, int LV, int MODE>
-__device__ __forceinline__ void down_init(DownState<real, R, G, FUSED, LV, MODE>& st, unsigned char* lds_raw,
+template <typename real, int R, int G, bool FUSED, int LV, int MODE, bool STORE>
+__device__ __forceinline__ void down_init(DownState<real, R, G, FUSED, LV, MODE, STORE>& st, unsigned char* lds_raw,
                                           const int32_t* leaf_ids, const uint64_t* tips,
                                           int64_t stride, int64_t n_pat, int64_t pitch, real* part,
                                           int32_t* ex, int N, int stack_depth, int64_t tile) {
@@ -625,7 +627,7 @@ __device__ __forceinline__ void down_program(State& st, const int4* __restrict__
   });
 }
 
-template <typename real, int R, int G, bool FUSED, int LV, int MODE = 0>
+template <typename real, int R, int G, bool FUSED, int LV, int MODE = 0, bool STORE = false>
 __device__ __forceinline__ void prune_down_body(
     const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids,
     const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
@@ -639,7 +641,7 @@ __device__ __forceinline__ void prune_down_body(
     tile = xcd_tile(blockIdx.x);
     if (tile >= pitch / 64) return;  // padding of the grid (whole block)
   }
-  DownState<real, R, G, FUSED, LV, MODE> st;
+  DownState<real, R, G, FUSED, LV, MODE, STORE> st;
   down_init(st, lds_raw, leaf_ids, tips, stride, n_pat, pitch, part, ex, N, stack_depth, tile);
   st.xt = xt;
   st.xte = xte;
@@ -648,7 +650,7 @@ __device__ __forceinline__ void prune_down_body(
 }
 
 // Whole tree (MODE 0) or the top program of the split form (MODE 2).
-template <typename real, int R, int G, bool FUSED, int LV, int MODE = 0>
+template <typename real, int R, int G, bool FUSED, int LV, int MODE = 0, bool STORE = false>
 __global__ __launch_bounds__(512) void k_prune_down(
     const int4* __restrict__ prog, int n_ops, const int32_t* __restrict__ leaf_ids,
     const uint64_t* __restrict__ tips, int64_t stride, int64_t n_pat, int64_t pitch,
@@ -656,9 +658,9 @@ __global__ __launch_bounds__(512) void k_prune_down(
     real* __restrict__ part, int32_t* __restrict__ ex, const uint32_t* __restrict__ weights,
     double* __restrict__ site_lnl, double* __restrict__ wave_sums, int N, int stack_depth,
     real* __restrict__ xt, int32_t* __restrict__ xte) {
-  prune_down_body<real, R, G, FUSED, LV, MODE>(prog, n_ops, leaf_ids, tips, stride, n_pat, pitch, Pm, pi,
-                                           probs, part, ex, weights, site_lnl, wave_sums, N,
-                                           stack_depth, xt, xte);
+  prune_down_body<real, R, G, FUSED, LV, MODE, STORE>(prog, n_ops, leaf_ids, tips, stride, n_pat, pitch,
+                                                  Pm, pi, probs, part, ex, weights, site_lnl,
+                                                  wave_sums, N, stack_depth, xt, xte);
 }
 
 // The parts of the split form side by side: tile t = y * (pitch / 64) + x
@@ -2356,6 +2358,7 @@ struct Plan {
   int rpw = 1;          // rate classes per wave of the post-order pass
   size_t esz = 8, lds = 0, dlds = 0;
   bool fused = false;
+  bool lds_stack = false;  // stored form with the operand stack in LDS
   // split form of the fused pass: the parts side by side, then the top
   bool split = false;
   int n_parts = 0;
@@ -2404,9 +2407,16 @@ Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t for
   const int depth = t->down_depth;
   // a fused stack that does not fit in LDS falls back to stored partials
   // (same arithmetic and result, more HBM traffic)
-  pl.fused = fused && down_lds(R, esz, true, depth) <= kLdsMax;
-  pl.rpw = rates_per_wave(esz, R, pl.fused);
-  pl.lds = down_lds(R, esz, pl.fused, depth);  // upper bound: no level in registers
+  const bool stack_fits = down_lds(R, esz, true, depth) <= kLdsMax;
+  pl.fused = fused && stack_fits;
+  // the stored form keeps its operand stack in LDS too when it fits, so it
+  // writes its partials and reads none back (PHY_PRUNE_STORED_LDS=0, or two
+  // rate classes per wave, reads the children from HBM)
+  const char* sl = getenv("PHY_PRUNE_STORED_LDS");
+  const bool want_stack = !pl.fused && stack_fits && !(sl && sl[0] == '0' && sl[1] == 0);
+  pl.rpw = rates_per_wave(esz, R, pl.fused || want_stack);
+  pl.lds_stack = want_stack && pl.rpw == 1;
+  pl.lds = down_lds(R, esz, pl.fused || pl.lds_stack, depth);  // upper bound: no level in registers
   pl.bs = 64 * R / pl.rpw;  // R / rpw waves over 64 patterns
   if (derivs) {
     pl.dbs = fit_block(env_block("PHY_PRUNE_DERIV_BLOCK", 256),
@@ -2509,7 +2519,7 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
                        (int32_t*)nullptr);
   };
   constexpr int G2 = (R % 2 == 0 && sizeof(real) == 4) ? 2 : 1;
-  const int lv = (fused && pl.rpw != 2) ? register_levels<real, R>(t->down_depth, pl.bs) : 0;
+  const int lv = ((fused || pl.lds_stack) && pl.rpw != 2) ? register_levels<real, R>(t->down_depth, pl.bs) : 0;
   {
     KernelSpan span(ctx, "prune_down");
     if (pl.split) {
@@ -2538,6 +2548,14 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
                          (int)t->ptop_prog.size() / 4,
                          t->d_ptop_fifo, tips, stride, n_pat, pl.pitch, P, d_pi, d_probs, (real*)nullptr,
                          (int32_t*)nullptr, weights, site_lnl, bsum, N, dt, xt, xte);
+    } else if (pl.lds_stack) {
+      const size_t lds = down_lds(R, sizeof(real), true, t->down_depth, lv);
+      if (lv == 2)
+        down(k_prune_down<real, R, 1, true, 2, 0, true>, lds);
+      else if (lv == 1)
+        down(k_prune_down<real, R, 1, true, 1, 0, true>, lds);
+      else
+        down(k_prune_down<real, R, 1, true, 0, 0, true>, lds);
     } else if (!fused) {
       pl.rpw == 2 ? down(k_prune_down<real, R, G2, false, 0>, pl.lds)
                   : down(k_prune_down<real, R, 1, false, 0>, pl.lds);
