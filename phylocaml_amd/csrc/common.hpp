@@ -113,8 +113,8 @@ struct phy_topo {
   // split form of the fused pruning pass (its own cuts, fewer than the Fitch
   // passes take: every part adds a stored root partial per pattern):
   // psplit holds the part programs; the top program reads the cut nodes as
-  // stored operands (SRC_EXT; the entry's x field holds part(A) |
-  // part(B) << 16), its fifo the real leaves only (padded)
+  // stored operands (SRC_EXT; the part of A in bits 8-15 and that of B in
+  // bits 16-23 of the entry's flags), its fifo the real leaves only (padded)
   std::vector<int32_t> pparts;
   SplitProg psplit;
   std::vector<int32_t> ptop_prog, ptop_fifo;

@@ -486,7 +486,8 @@ int phy_compile_topo(phy_topo* t) {
         if (sb == SRC_LEAF && slot[B] >= 0) sb = SRC_EXT, kb = slot[B];
         if (sa == SRC_LEAF) leaves.push_back(A);
         if (sb == SRC_LEAF) leaves.push_back(B);
-        t->ptop_prog.insert(t->ptop_prog.end(), {ka | (kb << 16), A, B, (fl & ~15) | sa | (sb << 2)});
+        const int32_t v = pg[4 * i];
+        t->ptop_prog.insert(t->ptop_prog.end(), {v, A, B, (fl & ~15) | sa | (sb << 2) | (ka << 8) | (kb << 16)});
       }
       // the leaf stream must be the walk's own with the cut nodes left out
       size_t j = 0;

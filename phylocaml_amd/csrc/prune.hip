@@ -349,8 +349,8 @@ __device__ __forceinline__ void rescale4(real* v, int* e) {
 // MODE 0 runs the whole tree.  The split form runs it in two launches: MODE 1
 // evaluates one part (a subtree, blockIdx.y) and stores its root partial in
 // slot y of the part rows; MODE 2 evaluates the top program, whose SRC_EXT
-// operands read those rows (slot of A in the low, of B in the high 16 bits
-// of the entry's first field).  Part row of slot k, class r:
+// operands read those rows (slot of A in bits 8-15, of B in bits 16-23 of
+// the entry's flags).  Part row of slot k, class r:
 // xt[((k * R + r) * 4 + i) * pitch + p], exponent xte[(k * R + r) * pitch + p].
 // STORE (with FUSED): the operand stack stays in LDS and every partial is
 // written too, so the stored form writes its partials and reads none back.
@@ -358,7 +358,7 @@ template <typename real, int R, int G, bool FUSED, int LV = 0, int MODE = 0, boo
 struct DownState {
   static_assert(R % G == 0, "rate classes per wave must divide R");
   static_assert(FUSED || LV == 0, "register stack levels belong to the fused form");
-  static_assert(MODE == 0 || (FUSED && G == 1 && !STORE), "the split form is fused, one class per wave");
+  static_assert(MODE == 0 || (FUSED && G == 1), "the split form is fused, one class per wave");
   static constexpr int kR = R, kG = G;
   static constexpr int BD = 64 * (R / G);
   static constexpr int NB = LV > 0 ? LV : 1;
@@ -378,13 +378,15 @@ struct DownState {
   int32_t* xte;
   TipNibbles ts;
 
-  // one class's stored part partial (slot k) times P
-  __device__ __forceinline__ void ext_product(int k, const real* P, real* y, int* e, int g) {
-    const real* sv = xt + (((size_t)k * R + r0 + g) * 4) * pitch + pc;
+  // one class's stored part partial (slot k; with STORE the cut node v's
+  // own stored partial) times P
+  __device__ __forceinline__ void ext_product(int k, int v, const real* P, real* y, int* e, int g) {
+    const real* sv = STORE ? part + (((size_t)(v - N) * R + r0 + g) * 4) * pitch + pc
+                           : xt + (((size_t)k * R + r0 + g) * 4) * pitch + pc;
     real x[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) x[i] = sv[(size_t)i * pitch];
-    *e = xte[((size_t)k * R + r0 + g) * pitch + pc];
+    *e = STORE ? ex[((size_t)(v - N) * R + r0 + g) * pitch + pc] : xte[((size_t)k * R + r0 + g) * pitch + pc];
     matvec4(P, x, y);
   }
   // store the accumulator (the part's root partial) in slot k
@@ -428,7 +430,7 @@ struct DownState {
   __device__ __forceinline__ void product_a(int sa, int a, int ka, const real (*P)[16], real (*y)[4],
                                             int* e) {
     if (MODE == 2 && sa == SRC_EXT) {
-      ext_product(ka, P[0], y[0], &e[0], 0);
+      ext_product(ka, a, P[0], y[0], &e[0], 0);
     } else if (sa == SRC_LEAF) {
       real x[4];
       tipvec(ts.get(), x);
@@ -484,10 +486,11 @@ struct DownState {
       }
     }
   }
-  // operand B (leaf, accumulator or part row kb)
-  __device__ __forceinline__ void product_b(int sb, int kb, const real (*P)[16], real (*y)[4], int* e) {
+  // operand B (leaf, accumulator or the part root b, row kb)
+  __device__ __forceinline__ void product_b(int sb, int b, int kb, const real (*P)[16], real (*y)[4],
+                                            int* e) {
     if (MODE == 2 && sb == SRC_EXT) {
-      ext_product(kb, P[0], y[0], &e[0], 0);
+      ext_product(kb, b, P[0], y[0], &e[0], 0);
     } else if (sb == SRC_LEAF) {
       real x[4];
       tipvec(ts.get(), x);
@@ -621,8 +624,8 @@ __device__ __forceinline__ void down_program(State& st, const int4* __restrict__
     st.spill_if(fl);
     real xa[G][4], xb[G][4];
     int ea[G], eb[G];
-    st.product_a(fl & 3, op.y, op.x & 0xffff, PA, xa, ea);
-    st.product_b((fl >> 2) & 3, op.x >> 16, PB, xb, eb);
+    st.product_a(fl & 3, op.y, (fl >> 8) & 255, PA, xa, ea);
+    st.product_b((fl >> 2) & 3, op.z, (fl >> 16) & 255, PB, xb, eb);
     st.combine(op.x, xa, xb, ea, eb);
   });
 }
@@ -667,13 +670,16 @@ __global__ __launch_bounds__(512) void k_prune_down(
 // runs part y over patterns 64x .. 64x+63 and stores the part's root partial
 // in slot y (tiles in XCD-aware order over a 1-D grid padded to a multiple
 // of 32).  The part programs and leaf streams sit at the offsets of meta
-// (prog_off | n_ops | fifo_off, n_parts each).
-template <typename real, int R, int LV>
+// (prog_off | n_ops | fifo_off, n_parts each).  With STORE (the stored
+// form) every partial of the part goes to its node's rows instead, the
+// part root's among them, and the top program reads it there.
+template <typename real, int R, int LV, bool STORE = false>
 __global__ __launch_bounds__(512) void k_prune_part(
     const int4* __restrict__ prog, const int32_t* __restrict__ meta, int n_parts,
     const int32_t* __restrict__ fifo, const uint64_t* __restrict__ tips, int64_t stride,
     int64_t n_pat, int64_t pitch, const real* __restrict__ Pm, real* __restrict__ xt,
-    int32_t* __restrict__ xte, int N, int stack_depth) {
+    int32_t* __restrict__ xte, real* __restrict__ part, int32_t* __restrict__ ex, int N,
+    int stack_depth) {
   extern __shared__ unsigned char lds_raw[];
   const int64_t nb = pitch / 64;
   const int64_t t = xcd_tile(blockIdx.x);
@@ -682,13 +688,12 @@ __global__ __launch_bounds__(512) void k_prune_part(
   const int4* pg = prog + meta[y];
   const int n_ops = meta[n_parts + y];
   const int32_t* li = fifo + meta[2 * n_parts + y];
-  DownState<real, R, 1, true, LV, 1> st;
-  down_init(st, lds_raw, li, tips, stride, n_pat, pitch, (real*)nullptr, (int32_t*)nullptr, N,
-            stack_depth, t - (int64_t)y * nb);
+  DownState<real, R, 1, true, LV, 1, STORE> st;
+  down_init(st, lds_raw, li, tips, stride, n_pat, pitch, part, ex, N, stack_depth, t - (int64_t)y * nb);
   st.xt = xt;
   st.xte = xte;
   down_program(st, pg, n_ops, Pm);
-  st.ext_store(y);
+  if (!STORE) st.ext_store(y);
 }
 
 // Many candidate topologies over the same tips and weights (fused form):
@@ -2440,10 +2445,10 @@ Plan make_plan(const phy_topo* t, const phy_model* m, int64_t n_pat, int32_t for
     pl.off_part = o; o = align256(o + (size_t)(N - 1) * R * K * pl.pitch * pl.esz);
     pl.off_ex = o; o = align256(o + (size_t)(N - 1) * R * pl.pitch * 4);
   }
-  pl.split = allow_split && pl.fused && pl.rpw == 1 && t->pparts.size() >= 2 && t->d_ptop_prog &&
+  pl.split = allow_split && (pl.fused || pl.lds_stack) && pl.rpw == 1 && t->pparts.size() >= 2 && t->d_ptop_prog &&
              down_lds(R, esz, true, std::max(t->psplit.depth, t->ptop_depth)) <= kLdsMax && split_env();
-  if (pl.split) {
-    pl.n_parts = (int)t->pparts.size();
+  if (pl.split) pl.n_parts = (int)t->pparts.size();
+  if (pl.split && pl.fused) {
     pl.off_xt = o; o = align256(o + (size_t)pl.n_parts * R * K * pl.pitch * pl.esz);
     pl.off_xte = o; o = align256(o + (size_t)pl.n_parts * R * pl.pitch * 4);
   }
@@ -2525,29 +2530,43 @@ int launch_tree(phy_ctx* ctx, const phy_topo* t, const phy_model* m, bool fused,
     if (pl.split) {
       const SplitProg& sd = t->psplit;
       const int np = pl.n_parts;
-      real* xt = reinterpret_cast<real*>(work + pl.off_xt);
-      int32_t* xte = reinterpret_cast<int32_t*>(work + pl.off_xte);
+      real* xt = fused ? reinterpret_cast<real*>(work + pl.off_xt) : nullptr;
+      int32_t* xte = fused ? reinterpret_cast<int32_t*>(work + pl.off_xte) : nullptr;
       const int lvp = register_levels<real, R>(sd.depth, pl.bs);
       const size_t ldsp = down_lds(R, sizeof(real), true, sd.depth, lvp);
       auto pad32 = [](int64_t n) { return (unsigned)((n + 31) / 32 * 32); };
       auto parts = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(pad32(pl.n_blocks * np)), dim3(pl.bs), ldsp, ctx->stream,
                            sd.d_prog, sd.d_meta, np, sd.d_fifo, tips, stride, n_pat, pl.pitch, P, xt, xte,
-                           N, sd.depth);
+                           part, ex, N, sd.depth);
       };
-      if (lvp == 2)
-        parts(k_prune_part<real, R, 2>);
-      else if (lvp == 1)
-        parts(k_prune_part<real, R, 1>);
-      else
-        parts(k_prune_part<real, R, 0>);
+      if (fused) {
+        if (lvp == 2)
+          parts(k_prune_part<real, R, 2>);
+        else if (lvp == 1)
+          parts(k_prune_part<real, R, 1>);
+        else
+          parts(k_prune_part<real, R, 0>);
+      } else {
+        if (lvp == 2)
+          parts(k_prune_part<real, R, 2, true>);
+        else if (lvp == 1)
+          parts(k_prune_part<real, R, 1, true>);
+        else
+          parts(k_prune_part<real, R, 0, true>);
+      }
       PHY_LAUNCHED(ctx);
       const int dt = std::max(t->ptop_depth, 1);
-      hipLaunchKernelGGL((k_prune_down<real, R, 1, true, 0, 2>), dim3(pad32(pl.n_blocks)), dim3(pl.bs),
-                         down_lds(R, sizeof(real), true, dt), ctx->stream, t->d_ptop_prog,
-                         (int)t->ptop_prog.size() / 4,
-                         t->d_ptop_fifo, tips, stride, n_pat, pl.pitch, P, d_pi, d_probs, (real*)nullptr,
-                         (int32_t*)nullptr, weights, site_lnl, bsum, N, dt, xt, xte);
+      auto top = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(pad32(pl.n_blocks)), dim3(pl.bs), down_lds(R, sizeof(real), true, dt),
+                           ctx->stream, t->d_ptop_prog, (int)t->ptop_prog.size() / 4, t->d_ptop_fifo, tips,
+                           stride, n_pat, pl.pitch, P, d_pi, d_probs, part, ex, weights, site_lnl, bsum, N, dt,
+                           xt, xte);
+      };
+      if (fused)
+        top(k_prune_down<real, R, 1, true, 0, 2>);
+      else
+        top(k_prune_down<real, R, 1, true, 0, 2, true>);
     } else if (pl.lds_stack) {
       const size_t lds = down_lds(R, sizeof(real), true, t->down_depth, lv);
       if (lv == 2)

@@ -345,3 +345,62 @@ def test_split_form_is_bit_identical(ctx, rng, monkeypatch, R, parts):
         assert split["lnl"] == whole["lnl"], (form, split["lnl"], whole["lnl"])
         assert np.array_equal(split["site"], whole["site"])
         assert abs(split["lnl"] - ov) <= tol * abs(ov), (form, split["lnl"], ov)
+
+
+@pytest.mark.parametrize("P", [1, 63, 65, 300, 4099])
+def test_split_form_small_and_ragged_pattern_counts(ctx, rng, monkeypatch, P):
+    # pattern counts below, at and past a 64-pattern tile and the 256-pattern
+    # pitch: the split form's padded grids and XCD-ordered tiles cover every
+    # tile once, so it still equals the whole-tree form bit for bit
+    N = 400
+    _say(f"prune split form ragged P={P}")
+    ch, root = random_tree(rng, N)
+    bl = rng.exponential(0.1, 2 * N - 1) + 1e-3
+    bl[root] = 0.0
+    model = _model(rng, 4)
+    tips = _tips(rng, N, P)
+    w = rng.integers(1, 6, P).astype(np.uint32)
+    ov = _oracle(N, ch, root, bl, model, tips, w, False)[0]
+    for form, tol in ((F32 | FUSED, 1e-6), (F64 | FUSED, 1e-10)):
+        monkeypatch.setenv("PHY_PRUNE_SPLIT", "0")
+        whole = _run(ctx, N, ch, root, bl, model, tips, w, form, False)
+        monkeypatch.setenv("PHY_PRUNE_SPLIT", "1")
+        split = _run(ctx, N, ch, root, bl, model, tips, w, form, False)
+        assert split["lnl"] == whole["lnl"], (form, split["lnl"], whole["lnl"])
+        assert np.array_equal(split["site"], whole["site"])
+        assert abs(split["lnl"] - ov) <= tol * abs(ov), (form, split["lnl"], ov)
+
+
+@pytest.mark.parametrize("split", ["0", "1"])
+@pytest.mark.parametrize("R", [1, 4])
+def test_stored_form_stack_in_lds_matches_hbm_stack(ctx, rng, monkeypatch, R, split):
+    # the stored form with its operand stack in LDS (whole tree, or split:
+    # the parts side by side writing every partial, then the top reading the
+    # part roots from their rows) writes the same partials, exponents, lnL
+    # and derivatives as the form that reads children back from HBM, bit for
+    # bit
+    monkeypatch.setenv("PHY_PRUNE_SPLIT", split)
+    N = 300
+    P = 1000
+    _say(f"prune stored form, LDS stack vs HBM, R={R} split={split}")
+    ch, root = random_tree(rng, N)
+    bl = rng.exponential(0.1, 2 * N - 1) + 1e-3
+    bl[root] = 0.0
+    model = _model(rng, R)
+    tips = _tips(rng, N, P)
+    w = rng.integers(1, 6, P).astype(np.uint32)
+    for form in (F32, F64):
+        got = {}
+        for flag in ("0", "1"):
+            monkeypatch.setenv("PHY_PRUNE_STORED_LDS", flag)
+            monkeypatch.setenv("PHY_PRUNE_RPW", "1")
+            got[flag] = _run(ctx, N, ch, root, bl, model, tips, w, form, True)
+        a, b = got["0"], got["1"]
+        assert a["lnl"] == b["lnl"]
+        assert np.array_equal(a["site"], b["site"])
+        assert np.array_equal(a["d1"], b["d1"]) and np.array_equal(a["d2"], b["d2"])
+        for v in (N, N + N // 2, 2 * N - 2):
+            pa, ea = a["ev"].partial(v)
+            pb, eb = b["ev"].partial(v)
+            assert torch.equal(pa[:, :P].cpu(), pb[:, :P].cpu())
+            assert torch.equal(ea[:, :P].cpu(), eb[:, :P].cpu())
