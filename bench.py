@@ -60,9 +60,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(prefix, grid=None):
+def measured_traffic(prefix, grid=None, suffix=""):
     """HBM bytes per launch of the kernels whose name starts with `prefix`
-    (and whose launch has `grid` work-items, when given), from the rocprofv3
+    and ends with `suffix` (and whose launch has `grid` work-items, when
+    given), from the rocprofv3
     FETCH_SIZE / WRITE_SIZE passes summarised in profiles/hbm_traffic.json
     (already scaled by the copy calibration); None when that kernel was not
     profiled."""
@@ -73,7 +74,8 @@ def measured_traffic(prefix, grid=None):
     except (OSError, ValueError, KeyError):
         return None
     hits = [v for name, v in k.items()
-            if name.startswith(prefix) and (grid is None or v.get("grid") == grid)]
+            if name.startswith(prefix) and name.endswith(suffix)
+            and (grid is None or v.get("grid") == grid)]
     if not hits:
         return None
     return sum(v["read_bytes"] + v["write_bytes"] for v in hits)
@@ -784,8 +786,9 @@ def main():
     # then the top program; both grids are padded for the XCD-aware tile
     # order, and the traffic file keeps each kernel's largest launches
     grid = ((pr["P"] + 255) // 256) * 256 * 4
-    t_parts = measured_traffic("k_prune_part<float, 4,")
-    t_top = measured_traffic("k_prune_down<float, 4, 1, true, 0, 2>")
+    # (the fused pass's kernels: STORE = false; the stored form's end in true)
+    t_parts = measured_traffic("k_prune_part<float, 4,", suffix="false>")
+    t_top = measured_traffic("k_prune_down<float, 4, 1, true, 0, 2, false>")
     traffic = None if t_parts is None or t_top is None else t_parts + t_top
     line = {
         "metric": "FP32-scaled vs FP64 pruning updates/s at rel ΔlnL≤1e-6; "
