@@ -320,7 +320,7 @@ def test_unsupported_rate_class_count_is_refused(ctx, rng):
 
 
 @pytest.mark.parametrize("parts", ["2", "8", "16"])
-@pytest.mark.parametrize("R", [1, 4, 8])
+@pytest.mark.parametrize("R", [1, 2, 4, 8])
 def test_split_form_is_bit_identical(ctx, rng, monkeypatch, R, parts):
     # the split form (parts side by side, then the top program reading the
     # parts' root partials) performs the whole-tree pass's arithmetic in the
@@ -372,7 +372,7 @@ def test_split_form_small_and_ragged_pattern_counts(ctx, rng, monkeypatch, P):
 
 
 @pytest.mark.parametrize("split", ["0", "1"])
-@pytest.mark.parametrize("R", [1, 4])
+@pytest.mark.parametrize("R", [1, 2, 4, 8])
 def test_stored_form_stack_in_lds_matches_hbm_stack(ctx, rng, monkeypatch, R, split):
     # the stored form with its operand stack in LDS (whole tree, or split:
     # the parts side by side writing every partial, then the top reading the
