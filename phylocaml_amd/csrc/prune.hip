@@ -347,13 +347,15 @@ __device__ __forceinline__ void rescale4(real* v, int* e) {
 // fused stack live in registers (selected by the wave-uniform stack pointer),
 // so the LDS holds LV levels less.
 // MODE 0 runs the whole tree.  The split form runs it in two launches: MODE 1
-// evaluates one part (a subtree, blockIdx.y) and stores its root partial in
-// slot y of the part rows; MODE 2 evaluates the top program, whose SRC_EXT
-// operands read those rows (slot of A in bits 8-15, of B in bits 16-23 of
-// the entry's flags).  Part row of slot k, class r:
+// evaluates one part y (a subtree) and stores its root partial in slot y of
+// the part rows; MODE 2 evaluates the top program, whose SRC_EXT operands
+// read those rows (slot of A in bits 8-15, of B in bits 16-23 of the
+// entry's flags).  Part row of slot k, class r:
 // xt[((k * R + r) * 4 + i) * pitch + p], exponent xte[(k * R + r) * pitch + p].
 // STORE (with FUSED): the operand stack stays in LDS and every partial is
-// written too, so the stored form writes its partials and reads none back.
+// written too, so the stored form writes its partials and reads none back;
+// split, its parts write their roots' partials like any other, and the top
+// program's SRC_EXT operands read the cut nodes' own rows (no part rows).
 template <typename real, int R, int G, bool FUSED, int LV = 0, int MODE = 0, bool STORE = false>
 struct DownState {
   static_assert(R % G == 0, "rate classes per wave must divide R");
